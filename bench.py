@@ -1,0 +1,200 @@
+"""Benchmark of the RRGalaxSim hot path on one MI355X.
+
+Headline workload (one JSON line on stdout): the plain PM path -- exponential
+disk N = 1,000,000 (scale 1, sech^2 h = 0.1, v_c from the enclosed mass + 10%
+dispersion, seed 1, R < 7.5), FP32 particles and grid, 128^3 cells over
+[-8, 8]^3, CIC deposit, 200 damped-wave iterations per step (tolerance 0, so
+no early exit), -grad phi, trilinear gather, dt = 5e-3; `--steps` steps timed
+(50 by default).  value = N * steps / seconds, in Mparticle-steps/s.
+
+Also reported (same line): the mixed-precision direct sum on the disk+bulge
+workload (N = 262,144, eps = 0.005) and the plain FP64 direct-sum kernel at
+the same N, as interactions/s; the dominant kernel's roofline; and the CPU
+oracle timed on a bounded sample of the headline workload.
+
+python bench.py [--gpus 1] [--steps 50] [--warmup 3] [--no-direct] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "N-body Ginteractions/s, PM Mparticle-steps/s at stated err; adjoint/fwd time"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+C = 4574.337022617616
+
+PM = dict(n=1_000_000, nx=128, dx=0.125, dt=5e-3, iterations=200, setup_iterations=2000, diss_base=0.785,
+          diss_extra=0.01)
+
+
+def _device_sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def pm_bench(rrg, ics, steps, warmup):
+    pos, vel, m = ics.exp_disk(PM["n"], seed=1)
+    wave = dict(diss_base=PM["diss_base"], diss_extra=PM["diss_extra"], wave_speed=C,
+                setup_iterations=PM["setup_iterations"], iterations=PM["iterations"], tolerance=0.0)
+    p = rrg.make_params(dt=PM["dt"], grid=dict(nx=PM["nx"], dx=PM["dx"], bc_frac=0.0), wave=wave, real="fp32")
+    sim = rrg.Simulation(p, pos, vel, m)
+    sim.precalc()
+    sim.sync_particles_and_grid()
+    if warmup:
+        sim.calc_step(warmup)
+    sim.synchronize()
+    sim.set_timing(True)
+    _device_sync()
+    t0 = time.perf_counter()
+    sim.calc_step(steps)
+    sim.synchronize()
+    _device_sync()
+    t1 = time.perf_counter()
+    tm = sim.timing()
+    stats = sim.stats()
+    sim.close()
+    ncell = PM["nx"] ** 3
+    wave_ms, wave_launches = tm["wave"]
+    avg_wave_s = wave_ms / 1e3 / max(wave_launches, 1)
+    wave_bytes = 16 * ncell  # read phi^n, phi^(n-1), source; write phi^(n+1) (fp32)
+    n = PM["n"]
+    # algorithmic bytes per step (fp32): wave, gradient (4 in, 16 out padded),
+    # source (8 in, 4+4 out, 8 clear), deposit (x,v,a,m in + x,v* out + 8 atomics
+    # of 8 B), gather+kick (x, v, v* in; a, v out; 8 corners of 16 B)
+    step_bytes = (PM["iterations"] * wave_bytes + 20 * ncell + 24 * ncell + n * (40 + 24 + 64) +
+                  n * (36 + 24 + 128))
+    secs = t1 - t0
+    return {
+        "value": n * steps / secs / 1e6,
+        "ms_per_step": secs / steps * 1e3,
+        "stage_ms_per_step": {k: v[0] / steps for k, v in tm.items() if v[1]},
+        "wave_avg_launch_us": avg_wave_s * 1e6,
+        "wave_bytes_per_launch": wave_bytes,
+        "step_algorithmic_bytes": step_bytes,
+        "wave_iterations_per_step": stats["last_wave_iterations"],
+    }
+
+
+def direct_bench(rrg, ics, steps=3):
+    pos, vel, m = ics.disk_bulge()
+    n = len(m)
+    s2 = np.full(n, 0.005 ** 2)
+    out = {}
+    for force, k in (("mixed", steps), ("fp64", 1)):
+        p = rrg.make_params(dt=1e-3, force=force)
+        sim = rrg.Simulation(p, pos, vel, m, s2)
+        sim.precalc()
+        sim.calc_step(1)
+        sim.synchronize()
+        sim.set_timing(True)
+        t0 = time.perf_counter()
+        sim.calc_step(k)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        tm = sim.timing()
+        sim.close()
+        kern_s = tm["direct"][0] / 1e3 / k
+        inter = float(n) * n
+        out[force] = {
+            "ginteractions_per_s": inter * k / (t1 - t0) / 1e9,
+            "kernel_ginteractions_per_s": inter / kern_s / 1e9,
+            "kernel_ms": kern_s * 1e3,
+            "tflops_20_per_interaction": 20 * inter / kern_s / 1e12,
+        }
+    out["mixed_over_fp64"] = out["mixed"]["kernel_ginteractions_per_s"] / out["fp64"]["kernel_ginteractions_per_s"]
+    out["config"] = {"workload": "disk N=196608 + Plummer bulge N=65536 (M=0.2, a=0.2), eps=0.005, dt=1e-3",
+                     "n": n, "timed_steps": {"mixed": steps, "fp64": 1}}
+    return out
+
+
+def cpu_baseline(ics):
+    """The CPU oracle (FP32 restatement) on a bounded sample of the headline
+    workload: the full 1e6-particle / 128^3 problem, 2 PM steps (each with the
+    200 wave iterations), timed as (precalc + 2 steps) - (precalc only)."""
+    import oracle
+
+    pos, vel, m = ics.exp_disk(PM["n"], seed=1)
+    kw = dict(nx=PM["nx"], dx=PM["dx"], dt=PM["dt"], diss_base=PM["diss_base"], wave_speed=C, setup_iterations=0,
+              iterations=PM["iterations"], tolerance=0.0, fixed_e=60, dtype=np.float32)
+    oracle.lib()
+    t0 = time.perf_counter()
+    oracle.pm_run(pos, vel, m, 0, **kw)
+    t1 = time.perf_counter()
+    oracle.pm_run(pos, vel, m, 2, **kw)
+    t2 = time.perf_counter()
+    per_step = ((t2 - t1) - (t1 - t0)) / 2
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": PM["n"] / per_step / 1e6, "unit": "Mparticle-steps/s", "cores": cores, "kind": "port",
+            "sample": "oracle/rrg_oracle.cpp (fp32, OpenMP over cells/particles; deposit sequential) on the full "
+                      "workload, 2 PM steps of 200 wave iterations, minus precalc"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-direct", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the hot path has no "
+              "multi-GPU implementation", file=sys.stderr)
+        sys.exit(2)
+    import rrgalaxsim_amd as rrg
+    from rrgalaxsim_amd import ics
+
+    rrg.lib()
+    pm = pm_bench(rrg, ics, args.steps, args.warmup)
+    achieved = pm["wave_bytes_per_launch"] / (pm["wave_avg_launch_us"] * 1e-6) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": pm["value"],
+        "unit": "Mparticle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": pm["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "PM: exponential disk N=1e6 (seed 1), 128^3 grid on [-8,8]^3, CIC + 200 damped-wave "
+                               "iterations + gradient + trilinear gather, dt=5e-3, fp32",
+                   "n": PM["n"], "nx": PM["nx"], "wave_iterations": PM["iterations"]},
+        "roofline": {"bound": "hbm", "kernel": "k_wave<float>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "step_roofline": {"algorithmic_bytes": pm["step_algorithmic_bytes"],
+                          "achieved_gbs": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9,
+                          "frac": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "stage_ms_per_step": pm["stage_ms_per_step"],
+        "wave_avg_launch_us": pm["wave_avg_launch_us"],
+    }
+    if not args.no_direct:
+        d = direct_bench(rrg, ics)
+        d["mixed"]["fp32_vector_frac"] = d["mixed"]["tflops_20_per_interaction"] / FP32_PEAK_TFLOPS
+        d["fp64"]["fp64_vector_frac"] = d["fp64"]["tflops_20_per_interaction"] / FP64_PEAK_TFLOPS
+        line["direct_sum"] = d
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(ics)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,266 @@
+"""CPU oracle for the RRGalaxSim hot path -- TEST INFRASTRUCTURE ONLY.
+
+Thin ctypes/numpy front end over ``oracle/_build/liboracle.so`` (built from
+``oracle/rrg_oracle.cpp`` by ``oracle/Makefile``).  Each function names the
+reference routine it restates; the C++ file carries the file:line citations.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker.  The product
+(``rrgalaxsim_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+
+
+class PmConfig(ctypes.Structure):
+    """Mirror of the C++ ``PmConfig`` (PM run description)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int),
+        ("dx", ctypes.c_double),
+        ("bc_frac", ctypes.c_double),
+        ("diss_base", ctypes.c_double),
+        ("wave_speed", ctypes.c_double),
+        ("tolerance", ctypes.c_double),
+        ("setup_iterations", ctypes.c_int),
+        ("iterations", ctypes.c_int),
+        ("check_step", ctypes.c_int),
+        ("hold", ctypes.c_int),
+        ("periodic", ctypes.c_int),
+        ("fixed_e", ctypes.c_int),
+        ("dt", ctypes.c_double),
+    ]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        for suf, ct in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
+            getattr(_lib, f"orc_wave_dt_{suf}").restype = ct
+            getattr(_lib, f"orc_wave_dt_{suf}").argtypes = [ct, ct]
+            getattr(_lib, f"orc_rho_mean_{suf}").restype = ct
+            getattr(_lib, f"orc_rho_mean_{suf}").argtypes = [ct, _int, ct]
+            getattr(_lib, f"orc_steady_err_{suf}").restype = ctypes.c_double
+            getattr(_lib, f"orc_solve_wave_{suf}").restype = _int
+    return _lib
+
+
+def _suf(dtype) -> str:
+    dtype = np.dtype(dtype)
+    if dtype == np.float64:
+        return "f64"
+    if dtype == np.float32:
+        return "f32"
+    raise TypeError(f"unsupported dtype {dtype}")
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp)
+
+
+def _r(x, dtype):
+    return ctypes.c_double(x) if np.dtype(dtype) == np.float64 else ctypes.c_float(x)
+
+
+def accel(pos, mass, soft2, dtype=np.float64):
+    """acceleration_kernel_blocked_ (nbody.cu:19-65); pos is (N,3)."""
+    pos = _c(pos, dtype); mass = _c(mass, dtype); soft2 = _c(soft2, dtype)
+    out = np.empty_like(pos)
+    getattr(lib(), f"orc_accel_{_suf(dtype)}")(_i64(len(mass)), _p(pos), _p(mass), _p(soft2), _p(out))
+    return out
+
+
+def accel_subset(pos, mass, soft2, idx, dtype=np.float64):
+    """acceleration_kernel_blocked_ evaluated for the targets ``idx`` only."""
+    pos = _c(pos, dtype); mass = _c(mass, dtype); soft2 = _c(soft2, dtype)
+    idx = _c(idx, np.int64)
+    out = np.empty((len(idx), 3), dtype=dtype)
+    getattr(lib(), f"orc_accel_subset_{_suf(dtype)}")(
+        _i64(len(mass)), _p(pos), _p(mass), _p(soft2), _i64(len(idx)), _p(idx), _p(out))
+    return out
+
+
+def grav(pos, mass, soft2, dtype=np.float64):
+    """grav_kernel_blocked_ (nbody.cu:67-114), self term included."""
+    pos = _c(pos, dtype); mass = _c(mass, dtype); soft2 = _c(soft2, dtype)
+    out = np.empty(len(mass), dtype=dtype)
+    getattr(lib(), f"orc_grav_{_suf(dtype)}")(_i64(len(mass)), _p(pos), _p(mass), _p(soft2), _p(out))
+    return out
+
+
+def predict(acc, vel, pos, dt, dtype=np.float64):
+    """predict_step_ (nbody.cu:122-152). Returns (new_pos, vel_pred)."""
+    acc = _c(acc, dtype); vel = _c(vel, dtype); pos = _c(pos, dtype).copy()
+    vp = np.empty_like(vel)
+    getattr(lib(), f"orc_predict_{_suf(dtype)}")(_i64(len(pos)), _p(acc), _p(vel), _p(pos), _p(vp), _r(dt, dtype))
+    return pos, vp
+
+
+def correct(acc, vel, vel_pred, dt, dtype=np.float64):
+    """correct_step_ (nbody.cu:159-181). Returns the new velocities."""
+    acc = _c(acc, dtype); vel = _c(vel, dtype).copy(); vp = _c(vel_pred, dtype)
+    getattr(lib(), f"orc_correct_{_suf(dtype)}")(_i64(len(vel)), _p(acc), _p(vel), _p(vp), _r(dt, dtype))
+    return vel
+
+
+def conservation(pos, vel, mass, gr, dtype=np.float64):
+    """calc_conservation (nbody.cu:246-291) -> dict(Ek, Ep, P, L)."""
+    pos = _c(pos, dtype); vel = _c(vel, dtype); mass = _c(mass, dtype); gr = _c(gr, dtype)
+    out = np.empty(8, dtype=dtype)
+    getattr(lib(), f"orc_conservation_{_suf(dtype)}")(_i64(len(mass)), _p(pos), _p(vel), _p(mass), _p(gr), _p(out))
+    return {"Ek": float(out[0]), "Ep": float(out[1]), "P": out[2:5].astype(np.float64),
+            "L": out[5:8].astype(np.float64)}
+
+
+def cic_float(pos, mass, nx, dx, bc_frac=0.0, dtype=np.float64):
+    """deposit_cell_mass_CIC_ (rm_p2mesh.cu:176-236), float adds in particle order."""
+    pos = _c(pos, dtype); mass = _c(mass, dtype)
+    grid = np.empty(nx ** 3, dtype=dtype)
+    getattr(lib(), f"orc_cic_float_{_suf(dtype)}")(_i64(len(mass)), _p(pos), _p(mass), _int(nx),
+                                                    _r(dx, dtype), _r(bc_frac, dtype), _p(grid))
+    return grid
+
+
+def cic_fixed(pos, mass, nx, dx, e, bc_frac=0.0, dtype=np.float64):
+    """CIC deposit with int64 fixed-point accumulation (resolution 2**-e)."""
+    pos = _c(pos, dtype); mass = _c(mass, dtype)
+    grid = np.empty(nx ** 3, dtype=np.int64)
+    getattr(lib(), f"orc_cic_fixed_{_suf(dtype)}")(_i64(len(mass)), _p(pos), _p(mass), _int(nx),
+                                                    _r(dx, dtype), _r(bc_frac, dtype), _int(e), _p(grid))
+    return grid
+
+
+def fixed_to_real(grid, e, dtype=np.float64):
+    grid = _c(grid, np.int64)
+    out = np.empty(len(grid), dtype=dtype)
+    getattr(lib(), f"orc_fixed_to_real_{_suf(dtype)}")(_i64(len(grid)), _p(grid), _int(e), _p(out))
+    return out
+
+
+def wave_dt(dx, c, dtype=np.float64):
+    """CFL step of validate_dependent_fields (io_json.cpp:174-178)."""
+    f = getattr(lib(), f"orc_wave_dt_{_suf(dtype)}")
+    return f(dx, c)
+
+
+def rho_mean(total_mass, nx, dx, dtype=np.float64):
+    """set_boundary_periodic's mean density (wave.cu:214-228)."""
+    return getattr(lib(), f"orc_rho_mean_{_suf(dtype)}")(total_mass, nx, dx)
+
+
+def wave_coef(diss, c, dt, dx, dtype=np.float64):
+    out = np.empty(4, dtype=dtype)
+    getattr(lib(), f"orc_wave_coef_{_suf(dtype)}")(_r(diss, dtype), _r(c, dtype), _r(dt, dtype), _r(dx, dtype), _p(out))
+    return {"k_main": out[0], "k_f": out[1], "k_actual": out[2], "k_old": out[3]}
+
+
+def wave_iteration(nx, dx, diss, c, dt, periodic, rho_mean_, phi, phi_old, mass, dtype=np.float64):
+    """wave_diss_iteration (wave.cu:106-193). Returns phi_new."""
+    phi = _c(phi, dtype); phi_old = _c(phi_old, dtype); mass = _c(mass, dtype)
+    out = np.empty_like(phi)
+    r = lambda x: _r(x, dtype)  # noqa: E731
+    getattr(lib(), f"orc_wave_iteration_{_suf(dtype)}")(_int(nx), r(dx), r(diss), r(c), r(dt), _int(periodic),
+                                                         r(rho_mean_), _p(out), _p(phi), _p(phi_old), _p(mass))
+    return out
+
+
+def solve_wave(nx, dx, diss, c, dt, periodic, rho_mean_, prev, curr, mass, max_iter, tol,
+               check_step=30, hold=3, dtype=np.float64):
+    """solve_wave_equation (wave.cu:230-297). Returns (prev, curr, iterations)."""
+    b = [_c(prev, dtype).copy(), _c(curr, dtype).copy(), np.zeros_like(_c(curr, dtype))]
+    mass = _c(mass, dtype)
+    perm = (ctypes.c_int * 3)()
+    r = lambda x: _r(x, dtype)  # noqa: E731
+    it = getattr(lib(), f"orc_solve_wave_{_suf(dtype)}")(
+        _int(nx), r(dx), r(diss), r(c), r(dt), _int(periodic), r(rho_mean_),
+        _p(b[0]), _p(b[1]), _p(b[2]), _p(mass), _int(max_iter), r(tol), _int(check_step), _int(hold), perm)
+    return b[perm[0]], b[perm[1]], int(it)
+
+
+def steady_err(u_new, u_old, dtype=np.float64):
+    u_new = _c(u_new, dtype); u_old = _c(u_old, dtype)
+    return getattr(lib(), f"orc_steady_err_{_suf(dtype)}")(_i64(len(u_new)), _p(u_new), _p(u_old))
+
+
+def gradient(nx, dx, phi, dtype=np.float64):
+    """calc_acceleration_field_ (rm_p2mesh.cu:238-290). Returns (nx^3, 3)."""
+    phi = _c(phi, dtype)
+    out = np.empty((nx ** 3, 3), dtype=dtype)
+    getattr(lib(), f"orc_gradient_{_suf(dtype)}")(_int(nx), _r(dx, dtype), _p(phi), _p(out))
+    return out
+
+
+def gather(pos, nx, dx, field, bc_frac=0.0, dtype=np.float64):
+    """interpolate_acceleration_to_particles_kernel (rm_mesh2p.cu:48-114)."""
+    pos = _c(pos, dtype); field = _c(field, dtype)
+    out = np.empty_like(pos)
+    getattr(lib(), f"orc_gather_{_suf(dtype)}")(_i64(len(pos)), _p(pos), _int(nx), _r(dx, dtype),
+                                                 _r(bc_frac, dtype), _p(field), _p(out))
+    return out
+
+
+def ngp_phi(pos, nx, dx, phi, bc_frac=0.0, dtype=np.float64):
+    """NGP phi projection (rm_p2mesh.cu:34-53 + rm_mesh2p.cu:29-38)."""
+    pos = _c(pos, dtype); phi = _c(phi, dtype)
+    out = np.empty(len(pos), dtype=dtype)
+    getattr(lib(), f"orc_ngp_phi_{_suf(dtype)}")(_i64(len(pos)), _p(pos), _int(nx), _r(dx, dtype),
+                                                  _r(bc_frac, dtype), _p(phi), _p(out))
+    return out
+
+
+def nbody_run(pos, vel, mass, soft2, dt, nsteps, dtype=np.float64):
+    """N-body strategy: precalc + nsteps x (predict, accel, correct).
+
+    Returns (pos, vel, acc, grav_at_t0)."""
+    pos = _c(pos, dtype).copy(); vel = _c(vel, dtype).copy()
+    mass = _c(mass, dtype); soft2 = _c(soft2, dtype)
+    acc = np.empty_like(pos); gr = np.empty(len(mass), dtype=dtype)
+    getattr(lib(), f"orc_nbody_run_{_suf(dtype)}")(_i64(len(mass)), _p(pos), _p(vel), _p(mass), _p(soft2),
+                                                    _r(dt, dtype), _int(nsteps), _p(acc), _p(gr))
+    return pos, vel, acc, gr
+
+
+def pm_run(pos, vel, mass, nsteps, *, nx, dx, dt, diss_base, wave_speed, setup_iterations, iterations,
+           tolerance=0.0, check_step=30, hold=3, bc_frac=0.0, periodic=True, fixed_e=61, dtype=np.float64):
+    """PM strategy: precalc + sync + nsteps x (predict, deposit, wave, gradient, gather, correct).
+
+    Returns dict(pos, vel, acc, phi, phi_prev, grid_acc, grav, iterations)."""
+    cf = PmConfig(nx=nx, dx=dx, bc_frac=bc_frac, diss_base=diss_base, wave_speed=wave_speed,
+                  tolerance=tolerance, setup_iterations=setup_iterations, iterations=iterations,
+                  check_step=check_step, hold=hold, periodic=int(periodic), fixed_e=fixed_e, dt=dt)
+    pos = _c(pos, dtype).copy(); vel = _c(vel, dtype).copy(); mass = _c(mass, dtype)
+    n = len(mass)
+    ncell = nx ** 3
+    acc = np.empty_like(pos)
+    phi2 = np.empty(2 * ncell, dtype=dtype)
+    gacc = np.empty((ncell, 3), dtype=dtype)
+    gr = np.empty(n, dtype=dtype)
+    iters = np.zeros(nsteps + 1, dtype=np.int32)
+    getattr(lib(), f"orc_pm_run_{_suf(dtype)}")(ctypes.byref(cf), _i64(n), _p(pos), _p(vel), _p(mass),
+                                                 _int(nsteps), _p(acc), _p(phi2), _p(gacc), _p(gr), _p(iters))
+    return {"pos": pos, "vel": vel, "acc": acc, "phi": phi2[:ncell], "phi_prev": phi2[ncell:],
+            "grid_acc": gacc, "grav": gr, "iterations": iters}

@@ -1,0 +1,650 @@
+// rrg_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A CPU restatement of the RRGalaxSim solver's per-step hot path, used as the
+// checker for the HIP library.  Only the tests, the smoke check and the CPU
+// baseline leg of the benchmark load it; the product never does.
+//
+// Every routine follows one reference function and keeps its operation order,
+// its mixed float/double promotions (the reference's `real` is `float` under
+// USE_FP32, but several literals are `double`), and its quirks.  The file is
+// compiled with -ffp-contract=off so no multiply-add is fused behind our back.
+// All routines are templated on `real` (double = reference default, float =
+// reference USE_FP32 build, co_types.h:4-18) and exported with _f64/_f32
+// suffixes through a plain C ABI for ctypes.
+//
+// Reference files are cited as path:line relative to the reference root.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr double kPi = 3.14159265358979;  // co_info_structs.h:6 (PI macro, a double literal)
+
+template <class real> struct V3 { real x, y, z; };
+
+template <class real> inline real cube(real x) { return x * x * x; }  // co_device_utils.cuh:55-58
+
+template <class T> inline T clampv(T v, T lo, T hi) {  // co_device_utils.cuh:43-48
+    if (v < lo) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Direct sum: acceleration_kernel_blocked_ (nbody.cu:19-65).
+// a_i = sum_j m_j d_ij / sqrt(|d|^2 + 0.5 (s_i + s_j))^3, d = x_j - x_i, j in
+// index order 0..N-1, self pair included.  `1. / sqrt(dot + 0.5 * (..))` is
+// evaluated in double even for real=float (the literals are double) and the
+// quotient is then rounded to real (nbody.cu:50-53).
+// ---------------------------------------------------------------------------
+template <class real>
+void accel(int64_t n, const real* pos, const real* mass, const real* soft2, real* acc) {
+    const V3<real>* p = reinterpret_cast<const V3<real>*>(pos);
+    V3<real>* a = reinterpret_cast<V3<real>*>(acc);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        const V3<real> pc = p[i];
+        const real sc = soft2[i];
+        V3<real> f{0, 0, 0};
+        for (int64_t j = 0; j < n; ++j) {
+            const real dx = p[j].x - pc.x;
+            const real dy = p[j].y - pc.y;
+            const real dz = p[j].z - pc.z;
+            const real dot = dx * dx + dy * dy + dz * dz;
+            const real den = 1. / std::sqrt(dot + 0.5 * (sc + soft2[j]));
+            const real k = mass[j] * cube(den);
+            f.x += dx * k;
+            f.y += dy * k;
+            f.z += dz * k;
+        }
+        a[i] = f;
+    }
+}
+
+// Subset variant for large N: accelerations of the targets idx[0..m) only.
+template <class real>
+void accel_subset(int64_t n, const real* pos, const real* mass, const real* soft2,
+                  int64_t m, const int64_t* idx, real* acc) {
+    const V3<real>* p = reinterpret_cast<const V3<real>*>(pos);
+    V3<real>* a = reinterpret_cast<V3<real>*>(acc);
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < m; ++q) {
+        const int64_t i = idx[q];
+        const V3<real> pc = p[i];
+        const real sc = soft2[i];
+        V3<real> f{0, 0, 0};
+        for (int64_t j = 0; j < n; ++j) {
+            const real dx = p[j].x - pc.x;
+            const real dy = p[j].y - pc.y;
+            const real dz = p[j].z - pc.z;
+            const real dot = dx * dx + dy * dy + dz * dz;
+            const real den = 1. / std::sqrt(dot + 0.5 * (sc + soft2[j]));
+            const real k = mass[j] * cube(den);
+            f.x += dx * k;
+            f.y += dy * k;
+            f.z += dz * k;
+        }
+        a[q] = f;
+    }
+}
+
+// grav_kernel_blocked_ (nbody.cu:67-114): grav_i = -sum_j m_j / sqrt(r^2 + h_ij),
+// self term j = i INCLUDED (nbody.cu:108-113).  `grav_sum += m / sqrt(double)`
+// accumulates through double and rounds back to real at every j.
+template <class real>
+void grav(int64_t n, const real* pos, const real* mass, const real* soft2, real* out) {
+    const V3<real>* p = reinterpret_cast<const V3<real>*>(pos);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        const V3<real> pc = p[i];
+        const real sc = soft2[i];
+        real g = 0.;
+        for (int64_t j = 0; j < n; ++j) {
+            const real dx = p[j].x - pc.x;
+            const real dy = p[j].y - pc.y;
+            const real dz = p[j].z - pc.z;
+            const real dot = dx * dx + dy * dy + dz * dz;
+            g += mass[j] / std::sqrt(dot + 0.5 * (sc + soft2[j]));
+        }
+        out[i] = -g;
+    }
+}
+
+// predict_step_ (nbody.cu:122-152): v* = v + dt a;  x += dt * 0.5 * (v + v*)
+// (`dt * 0.5` is double, so the position update runs in double before the
+// store rounds it to real).
+template <class real>
+void predict(int64_t n, const real* acc, const real* vel, real* pos, real* vel_pred, real dt) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            const real v = vel[3 * i + c];
+            const real r = pos[3 * i + c];
+            const real a = acc[3 * i + c];
+            const real vp = v + dt * a;
+            vel_pred[3 * i + c] = vp;
+            pos[3 * i + c] = r + dt * 0.5 * (v + vp);
+        }
+    }
+}
+
+// correct_step_ (nbody.cu:159-181): v = 0.5 * (v + v* + dt a_new)
+template <class real>
+void correct(int64_t n, const real* acc_new, real* vel, const real* vel_pred, real dt) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            const real v = vel[3 * i + c];
+            const real vv = vel_pred[3 * i + c];
+            const real aa = acc_new[3 * i + c];
+            vel[3 * i + c] = 0.5 * (v + vv + dt * aa);
+        }
+    }
+}
+
+// calc_conservation (nbody.cu:246-291): sequential sums in real.
+// out = {Ek, Ep, Px, Py, Pz, Lx, Ly, Lz}
+template <class real>
+void conservation(int64_t n, const real* pos, const real* vel, const real* mass, const real* gr,
+                  real* out) {
+    real px = 0, py = 0, pz = 0, lx = 0, ly = 0, lz = 0, ek = 0, ep = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const real x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        const real m = mass[i];
+        const real vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+        px += m * vx;
+        py += m * vy;
+        pz += m * vz;
+        lx += m * (vz * y - vy * z);
+        ly += m * (vx * z - vz * x);
+        lz += m * (vy * x - vx * y);
+        ek += m * (vx * vx + vy * vy + vz * vz);
+        ep += m * gr[i];
+    }
+    ek *= 0.5;
+    ep *= 0.5;
+    out[0] = ek; out[1] = ep;
+    out[2] = px; out[3] = py; out[4] = pz;
+    out[5] = lx; out[6] = ly; out[7] = lz;
+}
+
+// ---------------------------------------------------------------------------
+// Grid geometry (io_json.cpp:28-40) and wave parameters (io_json.cpp:53-99,
+// 170-186), kept in `real` like the reference's GridInfo/WaveParams.
+// ---------------------------------------------------------------------------
+template <class real> struct Grid {
+    int nx;
+    real dx, domain_l, domain_min, domain_max, sim_l, bc_l;
+};
+
+template <class real> Grid<real> make_grid(int nx, real dx, real bc_frac) {
+    Grid<real> g;
+    g.nx = nx;
+    g.dx = dx;
+    g.domain_l = g.dx * g.nx;
+    g.sim_l = (1. - bc_frac) * g.domain_l;
+    g.bc_l = 0.5 * bc_frac * g.domain_l;
+    g.domain_max = 0.5 * g.domain_l;
+    g.domain_min = -0.5 * g.domain_l;
+    return g;
+}
+
+inline size_t AT(int x, int y, int z, int NX) {  // co_device_utils.cuh:61
+    return size_t(x) + size_t(y) * NX + size_t(z) * NX * NX;
+}
+
+// CIC weights of deposit_cell_mass_CIC_ (rm_p2mesh.cu:182-226): cell-centred
+// (-0.5 shift), u clamped to [0, nx-1-1e-6], upper index clamped to nx-1.
+template <class real> struct CicStencil {
+    int i0, j0, k0, i1, j1, k1;
+    real w[8];  // order 000,100,010,110,001,101,011,111 (rm_p2mesh.cu:219-226)
+};
+
+template <class real>
+CicStencil<real> cic_stencil(const V3<real>& pos, const Grid<real>& g) {
+    const int NX = g.nx;
+    const real inv_dx = 1.f / g.dx;
+    real u = (pos.x - g.domain_min) * inv_dx - 0.5f;
+    real v = (pos.y - g.domain_min) * inv_dx - 0.5f;
+    real w = (pos.z - g.domain_min) * inv_dx - 0.5f;
+    const real eps = 1E-6;
+    u = clampv<real>(u, 0, (NX - 1) - eps);
+    v = clampv<real>(v, 0, (NX - 1) - eps);
+    w = clampv<real>(w, 0, (NX - 1) - eps);
+    CicStencil<real> s;
+    s.i0 = (int)std::floor(u);
+    s.j0 = (int)std::floor(v);
+    s.k0 = (int)std::floor(w);
+    s.i1 = std::min(s.i0 + 1, NX - 1);
+    s.j1 = std::min(s.j0 + 1, NX - 1);
+    s.k1 = std::min(s.k0 + 1, NX - 1);
+    const real fx = u - s.i0, fy = v - s.j0, fz = w - s.k0;
+    const real wx0 = 1.f - fx, wx1 = fx;
+    const real wy0 = 1.f - fy, wy1 = fy;
+    const real wz0 = 1.f - fz, wz1 = fz;
+    s.w[0] = wx0 * wy0 * wz0;
+    s.w[1] = wx1 * wy0 * wz0;
+    s.w[2] = wx0 * wy1 * wz0;
+    s.w[3] = wx1 * wy1 * wz0;
+    s.w[4] = wx0 * wy0 * wz1;
+    s.w[5] = wx1 * wy0 * wz1;
+    s.w[6] = wx0 * wy1 * wz1;
+    s.w[7] = wx1 * wy1 * wz1;
+    return s;
+}
+
+template <class real> inline void cic_cells(const CicStencil<real>& s, int NX, size_t* c) {
+    c[0] = AT(s.i0, s.j0, s.k0, NX);
+    c[1] = AT(s.i1, s.j0, s.k0, NX);
+    c[2] = AT(s.i0, s.j1, s.k0, NX);
+    c[3] = AT(s.i1, s.j1, s.k0, NX);
+    c[4] = AT(s.i0, s.j0, s.k1, NX);
+    c[5] = AT(s.i1, s.j0, s.k1, NX);
+    c[6] = AT(s.i0, s.j1, s.k1, NX);
+    c[7] = AT(s.i1, s.j1, s.k1, NX);
+}
+
+// Reference deposit: floating-point adds in particle order (the reference's
+// atomicAdd order on the GPU is unspecified; sequential order is one of them).
+template <class real>
+void cic_deposit_float(int64_t n, const real* pos, const real* mass, int nx, real dx, real bc_frac,
+                       real* grid) {
+    const Grid<real> g = make_grid<real>(nx, dx, bc_frac);
+    const V3<real>* p = reinterpret_cast<const V3<real>*>(pos);
+    std::fill(grid, grid + size_t(nx) * nx * nx, real(0));
+    for (int64_t q = 0; q < n; ++q) {
+        const CicStencil<real> s = cic_stencil(p[q], g);
+        size_t c[8];
+        cic_cells(s, nx, c);
+        for (int k = 0; k < 8; ++k) grid[c[k]] += mass[q] * s.w[k];
+    }
+}
+
+// Fixed-point deposit (the build's order-independent semantics): each of the
+// 8 products m*w is rounded once to an integer multiple of 2^-e and summed in
+// int64, so the grid is independent of particle order.
+template <class real> inline int64_t to_fixed(real v, int e) {
+    return (int64_t)std::llrint(std::ldexp((double)v, e));
+}
+
+template <class real>
+void cic_deposit_fixed(int64_t n, const real* pos, const real* mass, int nx, real dx, real bc_frac,
+                       int e, int64_t* grid) {
+    const Grid<real> g = make_grid<real>(nx, dx, bc_frac);
+    const V3<real>* p = reinterpret_cast<const V3<real>*>(pos);
+    std::fill(grid, grid + size_t(nx) * nx * nx, int64_t(0));
+    for (int64_t q = 0; q < n; ++q) {
+        const CicStencil<real> s = cic_stencil(p[q], g);
+        size_t c[8];
+        cic_cells(s, nx, c);
+        for (int k = 0; k < 8; ++k) grid[c[k]] += to_fixed<real>(mass[q] * s.w[k], e);
+    }
+}
+
+template <class real> void fixed_to_real(size_t ncell, const int64_t* fx, int e, real* out) {
+    for (size_t c = 0; c < ncell; ++c) out[c] = (real)std::ldexp((double)fx[c], -e);
+}
+
+// ---------------------------------------------------------------------------
+// Damped wave iteration: wave_diss_iteration (wave.cu:106-193), periodic and
+// zero-Dirichlet boundary types (co_boundary.cuh:4-9) with a uniform base
+// dissipation (no absorbing layer: the live configuration sets
+// periodic_keep_layer = false, rrgsim_integration_rect.cuh:65-69).
+// ---------------------------------------------------------------------------
+template <class real> struct WaveCoef {
+    real k_main, k_f, k_actual, k_old;
+};
+
+template <class real> WaveCoef<real> wave_coef(real diss, real C, real DT, real DX) {
+    const int DIM = 3;  // wave.cu:164-171
+    const real Q = diss * C * DT * real(0.5);
+    const real W = 1 + Q;
+    const real K = C * C * DT * DT / (DX * DX);
+    WaveCoef<real> k;
+    k.k_main = K / W;
+    k.k_f = -C * C * DT * DT / W;
+    k.k_actual = 2 * (1 - DIM * K) / W;
+    k.k_old = (Q - 1) / W;
+    return k;
+}
+
+inline int wrapi(int i, int n) { return (i < 0) ? i + n : ((i >= n) ? i - n : i); }  // wave.cu:22-24
+
+template <class real>
+void wave_iteration(int nx, real dx, real diss, real C, real DT, int periodic, real rho_mean,
+                    real* phi_new, const real* phi, const real* phi_old, const real* mass) {
+    const int NX = nx;
+    const real DX = dx;
+    const WaveCoef<real> k = wave_coef<real>(diss, C, DT, DX);
+    const real G = 1.;
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < NX; ++iz) {
+        for (int iy = 0; iy < NX; ++iy) {
+            for (int ix = 0; ix < NX; ++ix) {
+                const bool on_edge = ix == 0 || ix == NX - 1 || iy == 0 || iy == NX - 1 ||
+                                     iz == 0 || iz == NX - 1;
+                if (!periodic && on_edge) {  // ZeroDirichlet (wave.cu:140-145)
+                    phi_new[AT(ix, iy, iz, NX)] = 0.;
+                    continue;
+                }
+                real rho = mass[AT(ix, iy, iz, NX)] / (DX * DX * DX);
+                if (periodic) rho -= rho_mean;
+                const real f = 4 * kPi * G * rho;
+                int xm = ix - 1, xp = ix + 1, ym = iy - 1, yp = iy + 1, zm = iz - 1, zp = iz + 1;
+                if (periodic) {
+                    xm = wrapi(xm, NX); xp = wrapi(xp, NX);
+                    ym = wrapi(ym, NX); yp = wrapi(yp, NX);
+                    zm = wrapi(zm, NX); zp = wrapi(zp, NX);
+                }
+                const real lap_sum = phi[AT(xm, iy, iz, NX)] + phi[AT(xp, iy, iz, NX)] +
+                                     phi[AT(ix, ym, iz, NX)] + phi[AT(ix, yp, iz, NX)] +
+                                     phi[AT(ix, iy, zm, NX)] + phi[AT(ix, iy, zp, NX)];
+                phi_new[AT(ix, iy, iz, NX)] = lap_sum * k.k_main + phi[AT(ix, iy, iz, NX)] * k.k_actual +
+                                              phi_old[AT(ix, iy, iz, NX)] * k.k_old + f * k.k_f;
+            }
+        }
+    }
+}
+
+// SteadyStateChecker::calc_max_diff (wave_steady_checker.cu:95-127):
+// err = max|u_new - u_old| / (max|u_new| + 1e-30), in real.
+template <class real> double steady_err(size_t ncell, const real* u_new, const real* u_old) {
+    real diff = 0, absu = 0;
+    for (size_t c = 0; c < ncell; ++c) {
+        diff = std::max<real>(diff, std::fabs(u_new[c] - u_old[c]));
+        absu = std::max<real>(absu, std::fabs(u_new[c]));
+    }
+    const real EPS = 1.E-30;
+    const real err = diff / (absu + EPS);
+    return err;
+}
+
+// solve_wave_equation (wave.cu:230-297): three-buffer rotation, steady check
+// every `check_step` iterations after the rotation, stop after `hold`
+// consecutive checks below `tol` (wave_steady_checker.cu:131-143).
+// bufs = {prev, curr, next}; on return bufs[1] holds the newest field.  The
+// rotation is done by permuting the three pointers, and `perm` reports it.
+template <class real>
+int solve_wave(int nx, real dx, real diss, real C, real DT, int periodic, real rho_mean,
+               real* prev, real* curr, real* next, const real* mass, int max_iter, real tol,
+               int check_step, int hold, int* perm) {
+    real* b[3] = {prev, curr, next};
+    int id[3] = {0, 1, 2};
+    int hold_count = 0;
+    const size_t ncell = size_t(nx) * nx * nx;
+    int iter = 0;
+    for (iter = 0; iter < max_iter; ++iter) {
+        wave_iteration<real>(nx, dx, diss, C, DT, periodic, rho_mean, b[2], b[1], b[0], mass);
+        std::swap(b[0], b[1]); std::swap(id[0], id[1]);
+        std::swap(b[1], b[2]); std::swap(id[1], id[2]);
+        if (iter && iter % check_step == 0) {
+            const double err = steady_err<real>(ncell, b[1], b[0]);
+            if (err < tol) ++hold_count; else hold_count = 0;
+            if (hold_count >= hold) { ++iter; break; }
+        }
+    }
+    for (int q = 0; q < 3; ++q) perm[q] = id[q];
+    return iter;  // number of stencil iterations performed
+}
+
+// calc_acceleration_field_ (rm_p2mesh.cu:238-290): a = -grad(phi), central
+// differences inside, one-sided on the faces.  `0.5 * (..)` is a double
+// product and `-1. / DX` a double quotient, both rounded to real.
+template <class real> void gradient(int nx, real dx, const real* phi, real* acc) {
+    const int NX = nx;
+    const real DX = dx;
+    const real coef = -1. / DX;
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < NX; ++iz) {
+        for (int iy = 0; iy < NX; ++iy) {
+            for (int ix = 0; ix < NX; ++ix) {
+                const size_t xyz = AT(ix, iy, iz, NX);
+                real d[3];
+                if (ix == 0) d[0] = phi[AT(ix + 1, iy, iz, NX)] - phi[xyz];
+                else if (ix == NX - 1) d[0] = phi[xyz] - phi[AT(ix - 1, iy, iz, NX)];
+                else d[0] = 0.5 * (phi[AT(ix + 1, iy, iz, NX)] - phi[AT(ix - 1, iy, iz, NX)]);
+                if (iy == 0) d[1] = phi[AT(ix, iy + 1, iz, NX)] - phi[xyz];
+                else if (iy == NX - 1) d[1] = phi[xyz] - phi[AT(ix, iy - 1, iz, NX)];
+                else d[1] = 0.5 * (phi[AT(ix, iy + 1, iz, NX)] - phi[AT(ix, iy - 1, iz, NX)]);
+                if (iz == 0) d[2] = phi[AT(ix, iy, iz + 1, NX)] - phi[xyz];
+                else if (iz == NX - 1) d[2] = phi[xyz] - phi[AT(ix, iy, iz - 1, NX)];
+                else d[2] = 0.5 * (phi[AT(ix, iy, iz + 1, NX)] - phi[AT(ix, iy, iz - 1, NX)]);
+                acc[3 * xyz + 0] = d[0] * coef;
+                acc[3 * xyz + 1] = d[1] * coef;
+                acc[3 * xyz + 2] = d[2] * coef;
+            }
+        }
+    }
+}
+
+// interpolate_acceleration_to_particles_kernel (rm_mesh2p.cu:48-114):
+// trilinear, cell-centred; the cell index goes through floorf (a float floor
+// even when real = double), the weights come from the unclamped offset and
+// only the indices are clamped; corner loop order k, j, i.
+template <class real>
+void gather(int64_t n, const real* pos, int nx, real dx, real bc_frac, const real* field, real* out) {
+    const Grid<real> g = make_grid<real>(nx, dx, bc_frac);
+    const real domain_min = g.domain_min;
+    const V3<real>* p = reinterpret_cast<const V3<real>*>(pos);
+    const V3<real>* F = reinterpret_cast<const V3<real>*>(field);
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < n; ++q) {
+        const V3<real> ps = p[q];
+        const real fx = (ps.x - domain_min) / dx - 0.5f;
+        const real fy = (ps.y - domain_min) / dx - 0.5f;
+        const real fz = (ps.z - domain_min) / dx - 0.5f;
+        int i0 = (int)std::floor((float)fx);
+        int j0 = (int)std::floor((float)fy);
+        int k0 = (int)std::floor((float)fz);
+        int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
+        const real wx = fx - i0, wy = fy - j0, wz = fz - k0;
+        i0 = std::max(0, std::min(i0, nx - 1));
+        i1 = std::max(0, std::min(i1, nx - 1));
+        j0 = std::max(0, std::min(j0, nx - 1));
+        j1 = std::max(0, std::min(j1, nx - 1));
+        k0 = std::max(0, std::min(k0, nx - 1));
+        k1 = std::max(0, std::min(k1, nx - 1));
+        V3<real> a{0.0f, 0.0f, 0.0f};
+        for (int k = 0; k <= 1; k++) {
+            const int kk = (k == 0) ? k0 : k1;
+            const real wk = (k == 0) ? (1.0f - wz) : wz;
+            for (int j = 0; j <= 1; j++) {
+                const int jj = (j == 0) ? j0 : j1;
+                const real wj = (j == 0) ? (1.0f - wy) : wy;
+                for (int i = 0; i <= 1; i++) {
+                    const int ii = (i == 0) ? i0 : i1;
+                    const real wi = (i == 0) ? (1.0f - wx) : wx;
+                    const real weight = wi * wj * wk;
+                    const int idx = kk * nx * nx + jj * nx + ii;
+                    a.x += weight * F[idx].x;
+                    a.y += weight * F[idx].y;
+                    a.z += weight * F[idx].z;
+                }
+            }
+        }
+        reinterpret_cast<V3<real>*>(out)[q] = a;
+    }
+}
+
+// NGP cell of assign_particles_to_cells_ (rm_p2mesh.cu:34-53) and the phi
+// projection projectCellPhiOnParticles (rm_mesh2p.cu:29-38).
+template <class real>
+void ngp_phi(int64_t n, const real* pos, int nx, real dx, real bc_frac, const real* phi, real* out) {
+    const Grid<real> g = make_grid<real>(nx, dx, bc_frac);
+    const V3<real>* p = reinterpret_cast<const V3<real>*>(pos);
+    for (int64_t q = 0; q < n; ++q) {
+        const int ix = clampv<int>((int)((p[q].x - g.domain_min) / g.dx), 0, nx - 1);
+        const int iy = clampv<int>((int)((p[q].y - g.domain_min) / g.dx), 0, nx - 1);
+        const int iz = clampv<int>((int)((p[q].z - g.domain_min) / g.dx), 0, nx - 1);
+        out[q] = phi[AT(ix, iy, iz, nx)];
+    }
+}
+
+// CFL step of validate_dependent_fields (io_json.cpp:174-178).
+template <class real> real wave_dt(real dx, real c) {
+    constexpr real DIM = 3;
+    const real dt = 0.5 * dx / (c * std::sqrt(DIM));
+    return dt;
+}
+
+// set_boundary_periodic (wave.cu:214-228) with the hard-coded total mass 1
+// (rrgsim_integration_rect.cuh:65-69).
+template <class real> real rho_mean_periodic(real total_mass, int nx, real dx) {
+    const real L = nx * dx;
+    return total_mass / (L * L * L);
+}
+
+// ---------------------------------------------------------------------------
+// Whole runs: the N-body strategy (rrgsim_integration_nbody.cuh:26-56) and the
+// PM ("wave") strategy (rrgsim_integration_rect.cuh:53-112) with sync after
+// precalc (rrgsim_integration.cu:49-51).  State is AoS real3.
+// ---------------------------------------------------------------------------
+template <class real>
+void nbody_run(int64_t n, real* pos, real* vel, const real* mass, const real* soft2, real dt,
+               int nsteps, real* acc_out, real* grav_out) {
+    std::vector<real> acc(3 * n), vp(3 * n);
+    grav<real>(n, pos, mass, soft2, grav_out);  // precalc
+    accel<real>(n, pos, mass, soft2, acc.data());
+    for (int s = 0; s < nsteps; ++s) {
+        predict<real>(n, acc.data(), vel, pos, vp.data(), dt);
+        accel<real>(n, pos, mass, soft2, acc.data());
+        correct<real>(n, acc.data(), vel, vp.data(), dt);
+    }
+    if (acc_out) std::memcpy(acc_out, acc.data(), sizeof(real) * 3 * n);
+}
+
+struct PmConfig {
+    int nx;
+    double dx, bc_frac;
+    double diss_base, wave_speed, tolerance;
+    int setup_iterations, iterations, check_step, hold;
+    int periodic;
+    int fixed_e;    // fixed-point exponent (deposit resolution 2^-e)
+    double dt;      // particle time step (sim_params.dt_dynamics)
+};
+
+template <class real>
+void pm_run(const PmConfig& cf, int64_t n, real* pos, real* vel, const real* mass, int nsteps,
+            real* acc_out, real* phi_out, real* grid_acc_out, real* grav_out, int* iters_out) {
+    const int nx = cf.nx;
+    const size_t ncell = size_t(nx) * nx * nx;
+    const real dx = (real)cf.dx, bcf = (real)cf.bc_frac;
+    const real C = (real)cf.wave_speed;
+    const real DT = wave_dt<real>(dx, C);
+    const real diss = (real)cf.diss_base;
+    const real dtp = (real)cf.dt;
+    const real rho_mean = rho_mean_periodic<real>(1., nx, dx);
+    std::vector<int64_t> mfix(ncell);
+    std::vector<real> m(ncell), pa(ncell, 0), pb(ncell, 0), pc(ncell, 0), gacc(3 * ncell);
+    std::vector<real> acc(3 * n), vp(3 * n);
+    real* b[3] = {pa.data(), pb.data(), pc.data()};  // prev, curr, next (zero-filled)
+    auto deposit = [&]() {
+        cic_deposit_fixed<real>(n, pos, mass, nx, dx, bcf, cf.fixed_e, mfix.data());
+        fixed_to_real<real>(ncell, mfix.data(), cf.fixed_e, m.data());
+    };
+    auto solve = [&](int iters) {
+        int perm[3];
+        real* s[3] = {b[0], b[1], b[2]};
+        const int done = solve_wave<real>(nx, dx, diss, C, DT, cf.periodic, rho_mean, s[0], s[1], s[2],
+                                          m.data(), iters, (real)cf.tolerance, cf.check_step, cf.hold,
+                                          perm);
+        real* nb[3] = {s[perm[0]], s[perm[1]], s[perm[2]]};
+        b[0] = nb[0]; b[1] = nb[1]; b[2] = nb[2];
+        return done;
+    };
+    int it_idx = 0;
+    // precalc
+    deposit();
+    int done = solve(cf.setup_iterations);
+    if (iters_out) iters_out[it_idx++] = done;
+    gradient<real>(nx, dx, b[1], gacc.data());
+    // sync: gather acceleration onto the particles
+    gather<real>(n, pos, nx, dx, bcf, gacc.data(), acc.data());
+    for (int s = 0; s < nsteps; ++s) {
+        predict<real>(n, acc.data(), vel, pos, vp.data(), dtp);
+        deposit();
+        done = solve(cf.iterations);
+        if (iters_out) iters_out[it_idx++] = done;
+        gradient<real>(nx, dx, b[1], gacc.data());
+        gather<real>(n, pos, nx, dx, bcf, gacc.data(), acc.data());
+        correct<real>(n, acc.data(), vel, vp.data(), dtp);
+    }
+    if (acc_out) std::memcpy(acc_out, acc.data(), sizeof(real) * 3 * n);
+    if (phi_out) {
+        std::memcpy(phi_out, b[1], sizeof(real) * ncell);               // curr
+        std::memcpy(phi_out + ncell, b[0], sizeof(real) * ncell);       // prev
+    }
+    if (grid_acc_out) std::memcpy(grid_acc_out, gacc.data(), sizeof(real) * 3 * ncell);
+    if (grav_out) ngp_phi<real>(n, pos, nx, dx, bcf, b[1], grav_out);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI (ctypes).  Suffix _f64 = real double, _f32 = real float.
+// ---------------------------------------------------------------------------
+#define ORC_EXPORT extern "C" __attribute__((visibility("default")))
+
+#define ORC_INSTANTIATE(SUF, real)                                                                   \
+    ORC_EXPORT void orc_accel_##SUF(int64_t n, const real* pos, const real* m, const real* s2,     \
+                                    real* acc) { accel<real>(n, pos, m, s2, acc); }                \
+    ORC_EXPORT void orc_accel_subset_##SUF(int64_t n, const real* pos, const real* m,              \
+                                           const real* s2, int64_t k, const int64_t* idx,          \
+                                           real* acc) { accel_subset<real>(n, pos, m, s2, k, idx, acc); } \
+    ORC_EXPORT void orc_grav_##SUF(int64_t n, const real* pos, const real* m, const real* s2,      \
+                                   real* out) { grav<real>(n, pos, m, s2, out); }                  \
+    ORC_EXPORT void orc_predict_##SUF(int64_t n, const real* acc, const real* vel, real* pos,      \
+                                      real* vp, real dt) { predict<real>(n, acc, vel, pos, vp, dt); } \
+    ORC_EXPORT void orc_correct_##SUF(int64_t n, const real* acc, real* vel, const real* vp,       \
+                                      real dt) { correct<real>(n, acc, vel, vp, dt); }             \
+    ORC_EXPORT void orc_conservation_##SUF(int64_t n, const real* pos, const real* vel,            \
+                                           const real* m, const real* g, real* out) {              \
+        conservation<real>(n, pos, vel, m, g, out); }                                              \
+    ORC_EXPORT void orc_cic_float_##SUF(int64_t n, const real* pos, const real* m, int nx,         \
+                                        real dx, real bcf, real* grid) {                           \
+        cic_deposit_float<real>(n, pos, m, nx, dx, bcf, grid); }                                   \
+    ORC_EXPORT void orc_cic_fixed_##SUF(int64_t n, const real* pos, const real* m, int nx,         \
+                                        real dx, real bcf, int e, int64_t* grid) {                 \
+        cic_deposit_fixed<real>(n, pos, m, nx, dx, bcf, e, grid); }                                \
+    ORC_EXPORT void orc_fixed_to_real_##SUF(int64_t ncell, const int64_t* g, int e, real* out) {   \
+        fixed_to_real<real>((size_t)ncell, g, e, out); }                                           \
+    ORC_EXPORT void orc_wave_iteration_##SUF(int nx, real dx, real diss, real c, real dt,          \
+                                             int periodic, real rho_mean, real* pn, const real* p, \
+                                             const real* po, const real* m) {                      \
+        wave_iteration<real>(nx, dx, diss, c, dt, periodic, rho_mean, pn, p, po, m); }             \
+    ORC_EXPORT int orc_solve_wave_##SUF(int nx, real dx, real diss, real c, real dt, int periodic, \
+                                        real rho_mean, real* prev, real* curr, real* next,         \
+                                        const real* m, int max_iter, real tol, int check_step,     \
+                                        int hold, int* perm) {                                     \
+        return solve_wave<real>(nx, dx, diss, c, dt, periodic, rho_mean, prev, curr, next, m,      \
+                                max_iter, tol, check_step, hold, perm); }                          \
+    ORC_EXPORT double orc_steady_err_##SUF(int64_t ncell, const real* un, const real* uo) {        \
+        return steady_err<real>((size_t)ncell, un, uo); }                                          \
+    ORC_EXPORT void orc_gradient_##SUF(int nx, real dx, const real* phi, real* acc) {              \
+        gradient<real>(nx, dx, phi, acc); }                                                        \
+    ORC_EXPORT void orc_gather_##SUF(int64_t n, const real* pos, int nx, real dx, real bcf,        \
+                                     const real* field, real* out) {                               \
+        gather<real>(n, pos, nx, dx, bcf, field, out); }                                           \
+    ORC_EXPORT void orc_ngp_phi_##SUF(int64_t n, const real* pos, int nx, real dx, real bcf,       \
+                                      const real* phi, real* out) {                                \
+        ngp_phi<real>(n, pos, nx, dx, bcf, phi, out); }                                            \
+    ORC_EXPORT real orc_wave_dt_##SUF(real dx, real c) { return wave_dt<real>(dx, c); }            \
+    ORC_EXPORT real orc_rho_mean_##SUF(real total_mass, int nx, real dx) {                         \
+        return rho_mean_periodic<real>(total_mass, nx, dx); }                                      \
+    ORC_EXPORT void orc_wave_coef_##SUF(real diss, real c, real dt, real dx, real* out4) {         \
+        WaveCoef<real> k = wave_coef<real>(diss, c, dt, dx);                                       \
+        out4[0] = k.k_main; out4[1] = k.k_f; out4[2] = k.k_actual; out4[3] = k.k_old; }            \
+    ORC_EXPORT void orc_nbody_run_##SUF(int64_t n, real* pos, real* vel, const real* m,            \
+                                        const real* s2, real dt, int nsteps, real* acc,            \
+                                        real* gr) {                                                \
+        nbody_run<real>(n, pos, vel, m, s2, dt, nsteps, acc, gr); }                                \
+    ORC_EXPORT void orc_pm_run_##SUF(const PmConfig* cf, int64_t n, real* pos, real* vel,          \
+                                     const real* m, int nsteps, real* acc, real* phi2,             \
+                                     real* gacc, real* gr, int* iters) {                           \
+        pm_run<real>(*cf, n, pos, vel, m, nsteps, acc, phi2, gacc, gr, iters); }
+
+ORC_INSTANTIATE(f64, double)
+ORC_INSTANTIATE(f32, float)

@@ -1,0 +1,461 @@
+// Particle-mesh ("wave" model) kernels for gfx950.
+//
+// Reference:
+//   deposit_cell_mass_CIC_       solver/lib/rrgsim_rect_mesh/src/rm_p2mesh.cu:176-236
+//   calc_acceleration_field_     rm_p2mesh.cu:238-290
+//   assign_particles_to_cells_   rm_p2mesh.cu:25-66 (NGP cell only)
+//   interpolate_acceleration_to_particles_kernel  rm_mesh2p.cu:48-114
+//   projectCellPhiOnParticles    rm_mesh2p.cu:29-38
+//   wave_diss_iteration          solver/lib/rrgsim_wave/src/wave.cu:106-193
+//   reduce_step_error            solver/lib/rrgsim_wave/src/wave_steady_checker.cu:30-85
+//
+// Every kernel keeps the reference's operation order and its float/double
+// promotions, and the file is compiled with -ffp-contract=off, so for a given
+// `real` the results are bitwise those of a sequential CPU evaluation.  The
+// deposit is the exception by design: its eight products m*w are rounded to
+// int64 fixed point (resolution 2^-e) and summed with integer atomics, which
+// makes the cell masses independent of particle order.
+#include "rrg_internal.hpp"
+
+#include <cmath>
+#include <type_traits>
+
+namespace rrg {
+namespace pm {
+namespace {
+
+constexpr double kPi = 3.14159265358979;  // co_info_structs.h:6
+
+template <class real> struct Vec4;
+template <> struct Vec4<float> { using T = float4; };
+template <> struct Vec4<double> { using T = double4; };
+
+template <class real> __device__ __forceinline__ real clampr(real v, real lo, real hi) {
+    if (v < lo) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
+__device__ __forceinline__ void atomic_add_fixed(unsigned long long* cell, long long q) {
+    if (q != 0) atomicAdd(cell, (unsigned long long)q);
+}
+
+// ------------------------------------------------------------------ deposit
+template <class real>
+__global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt, int do_predict, Geo<real> g,
+                                                         double scale, unsigned long long* __restrict__ mfix) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const int64_t n = p.n;
+    real r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r[c] = p.x[c * n + i];
+        if (do_predict) {  // predict_step_ (nbody.cu:135-151)
+            const real v = p.v[c * n + i];
+            const real a = p.a[c * n + i];
+            const real vp = v + dt * a;
+            p.w[c * n + i] = vp;
+            r[c] = (real)((double)r[c] + ((double)dt * 0.5) * (double)(v + vp));
+            p.x[c * n + i] = r[c];
+        }
+    }
+    const real m = p.m[i];
+    const int NX = g.nx;
+    real u = (r[0] - g.domain_min) * g.inv_dx - (real)0.5f;
+    real v = (r[1] - g.domain_min) * g.inv_dx - (real)0.5f;
+    real w = (r[2] - g.domain_min) * g.inv_dx - (real)0.5f;
+    u = clampr<real>(u, 0, g.upper);
+    v = clampr<real>(v, 0, g.upper);
+    w = clampr<real>(w, 0, g.upper);
+    // The clamp above already bounds finite u; the index clamp only matters
+    // for a NaN coordinate, which must not address outside the grid.
+    const int i0 = max(0, min((int)floor(u), NX - 1));
+    const int j0 = max(0, min((int)floor(v), NX - 1));
+    const int k0 = max(0, min((int)floor(w), NX - 1));
+    const int i1 = min(i0 + 1, NX - 1), j1 = min(j0 + 1, NX - 1), k1 = min(k0 + 1, NX - 1);
+    const real fx = u - (real)i0, fy = v - (real)j0, fz = w - (real)k0;
+    const real wx0 = (real)1.f - fx, wx1 = fx;
+    const real wy0 = (real)1.f - fy, wy1 = fy;
+    const real wz0 = (real)1.f - fz, wz1 = fz;
+    const size_t sy = size_t(NX), sz = size_t(NX) * NX;
+    const size_t c000 = size_t(i0) + j0 * sy + k0 * sz;
+    const size_t c100 = size_t(i1) + j0 * sy + k0 * sz;
+    const size_t c010 = size_t(i0) + j1 * sy + k0 * sz;
+    const size_t c110 = size_t(i1) + j1 * sy + k0 * sz;
+    const size_t c001 = size_t(i0) + j0 * sy + k1 * sz;
+    const size_t c101 = size_t(i1) + j0 * sy + k1 * sz;
+    const size_t c011 = size_t(i0) + j1 * sy + k1 * sz;
+    const size_t c111 = size_t(i1) + j1 * sy + k1 * sz;
+    auto q = [&](real wt) { return (long long)rint((double)(m * wt) * scale); };
+    atomic_add_fixed(mfix + c000, q(wx0 * wy0 * wz0));
+    atomic_add_fixed(mfix + c100, q(wx1 * wy0 * wz0));
+    atomic_add_fixed(mfix + c010, q(wx0 * wy1 * wz0));
+    atomic_add_fixed(mfix + c110, q(wx1 * wy1 * wz0));
+    atomic_add_fixed(mfix + c001, q(wx0 * wy0 * wz1));
+    atomic_add_fixed(mfix + c101, q(wx1 * wy0 * wz1));
+    atomic_add_fixed(mfix + c011, q(wx0 * wy1 * wz1));
+    atomic_add_fixed(mfix + c111, q(wx1 * wy1 * wz1));
+}
+
+// Fixed point -> cell mass and source term.  rho = m / (DX*DX*DX) - rho_mean
+// and f = 4*PI*G*rho in double then rounded to real (wave.cu:150-153); the
+// stencil's last term f*K_F is formed here once per deposit.
+template <class real>
+__device__ __forceinline__ real source_term(real m, real dx3, real rho_mean, int periodic, real k_f) {
+    real rho = m / dx3;
+    if (periodic) rho -= rho_mean;
+    const real f = (real)(4 * kPi * (double)(real)1. * (double)rho);
+    return f * k_f;
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_fixed_to_source(int64_t ncell, unsigned long long* __restrict__ mfix,
+                                                         double inv_scale, real dx3, real rho_mean, int periodic,
+                                                         real k_f, real* __restrict__ mass, real* __restrict__ src) {
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const long long q = (long long)mfix[c];
+    mfix[c] = 0ull;
+    const real m = (real)((double)q * inv_scale);
+    mass[c] = m;
+    src[c] = source_term<real>(m, dx3, rho_mean, periodic, k_f);
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_mass_to_source(int64_t ncell, const real* __restrict__ mass, real dx3,
+                                                        real rho_mean, int periodic, real k_f, real* __restrict__ src) {
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    src[c] = source_term<real>(mass[c], dx3, rho_mean, periodic, k_f);
+}
+
+// -------------------------------------------------------------- wave stencil
+// Thread per cell, workgroup = 64 x 4 cells of one z-plane.  Workgroups are
+// renumbered so that each of the 8 XCDs (which receive workgroups round-robin)
+// sweeps one contiguous slab of z-planes, keeping the +-z neighbour planes in
+// that XCD's L2.
+template <class real> struct MaxBits;
+template <> struct MaxBits<float> { using T = unsigned int; };
+template <> struct MaxBits<double> { using T = unsigned long long; };
+
+template <class real> __device__ __forceinline__ typename MaxBits<real>::T to_bits(real v) {
+    if constexpr (std::is_same_v<real, float>) return __float_as_uint(v);
+    else return (unsigned long long)__double_as_longlong(v);
+}
+
+template <class real> __device__ __forceinline__ real wave_max(real v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ void xcd_block(int nbx, int nby, int nbz, int& bx, int& by, int& bz) {
+    const int nb = nbx * nby * nbz;
+    const int b = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    int r = b;
+    if ((nb & 7) == 0) {
+        const int per = nb >> 3;
+        r = (b & 7) * per + (b >> 3);
+    }
+    bx = r % nbx;
+    by = (r / nbx) % nby;
+    bz = r / (nbx * nby);
+}
+
+template <class real, bool PERIODIC, bool CHECK>
+__global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real* __restrict__ p,
+                                              const real* __restrict__ po, const real* __restrict__ src, int NX,
+                                              WaveK<real> k, typename MaxBits<real>::T* __restrict__ maxbits) {
+    int bx, by, bz;
+    xcd_block(gridDim.x, gridDim.y, gridDim.z, bx, by, bz);
+    const int ix = bx * 64 + threadIdx.x;
+    const int iy = by * 4 + threadIdx.y;
+    const int iz = bz;
+    real dmax = 0, amax = 0;
+    if (ix < NX && iy < NX) {
+        const size_t sy = size_t(NX), sz = size_t(NX) * NX;
+        const size_t c = size_t(ix) + iy * sy + iz * sz;
+        real out;
+        const bool on_edge = ix == 0 || ix == NX - 1 || iy == 0 || iy == NX - 1 || iz == 0 || iz == NX - 1;
+        if (!PERIODIC && on_edge) {
+            out = 0;  // ZeroDirichlet (wave.cu:140-145)
+        } else {
+            int xm = ix - 1, xp = ix + 1, ym = iy - 1, yp = iy + 1, zm = iz - 1, zp = iz + 1;
+            if (PERIODIC) {
+                xm = xm < 0 ? xm + NX : xm;  xp = xp >= NX ? xp - NX : xp;
+                ym = ym < 0 ? ym + NX : ym;  yp = yp >= NX ? yp - NX : yp;
+                zm = zm < 0 ? zm + NX : zm;  zp = zp >= NX ? zp - NX : zp;
+            }
+            const size_t row = iy * sy + iz * sz;
+            const real lap_sum = p[size_t(xm) + row] + p[size_t(xp) + row] + p[ix + ym * sy + iz * sz] +
+                                 p[ix + yp * sy + iz * sz] + p[ix + iy * sy + zm * sz] + p[ix + iy * sy + zp * sz];
+            out = lap_sum * k.k_main + p[c] * k.k_actual + po[c] * k.k_old + src[c];
+        }
+        pn[c] = out;
+        if (CHECK) {
+            dmax = fabs(out - p[c]);
+            amax = fabs(out);
+        }
+    }
+    if (CHECK) {
+        dmax = wave_max(dmax);
+        amax = wave_max(amax);
+        if (((threadIdx.y * 64 + threadIdx.x) & 63) == 0) {
+            atomicMax(maxbits + 0, to_bits<real>(dmax));
+            atomicMax(maxbits + 1, to_bits<real>(amax));
+        }
+    }
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_max_diff(int64_t ncell, const real* __restrict__ a, const real* __restrict__ b,
+                                                  typename MaxBits<real>::T* __restrict__ maxbits) {
+    real dmax = 0, amax = 0;
+    for (int64_t c = int64_t(blockIdx.x) * 256 + threadIdx.x; c < ncell; c += int64_t(gridDim.x) * 256) {
+        dmax = fmax(dmax, fabs(a[c] - b[c]));
+        amax = fmax(amax, fabs(a[c]));
+    }
+    dmax = wave_max(dmax);
+    amax = wave_max(amax);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(maxbits + 0, to_bits<real>(dmax));
+        atomicMax(maxbits + 1, to_bits<real>(amax));
+    }
+}
+
+// ----------------------------------------------------------------- gradient
+template <class real>
+__global__ __launch_bounds__(256) void k_gradient(int NX, real coef, const real* __restrict__ phi,
+                                                  typename Vec4<real>::T* __restrict__ acc) {
+    int bx, by, bz;
+    xcd_block(gridDim.x, gridDim.y, gridDim.z, bx, by, bz);
+    const int ix = bx * 64 + threadIdx.x;
+    const int iy = by * 4 + threadIdx.y;
+    const int iz = bz;
+    if (ix >= NX || iy >= NX) return;
+    const size_t sy = size_t(NX), sz = size_t(NX) * NX;
+    const size_t c = size_t(ix) + iy * sy + iz * sz;
+    real d[3];
+    if (ix == 0) d[0] = phi[c + 1] - phi[c];
+    else if (ix == NX - 1) d[0] = phi[c] - phi[c - 1];
+    else d[0] = (real)(0.5 * (double)(phi[c + 1] - phi[c - 1]));
+    if (iy == 0) d[1] = phi[c + sy] - phi[c];
+    else if (iy == NX - 1) d[1] = phi[c] - phi[c - sy];
+    else d[1] = (real)(0.5 * (double)(phi[c + sy] - phi[c - sy]));
+    if (iz == 0) d[2] = phi[c + sz] - phi[c];
+    else if (iz == NX - 1) d[2] = phi[c] - phi[c - sz];
+    else d[2] = (real)(0.5 * (double)(phi[c + sz] - phi[c - sz]));
+    typename Vec4<real>::T o;
+    o.x = d[0] * coef;
+    o.y = d[1] * coef;
+    o.z = d[2] * coef;
+    o.w = 0;
+    acc[c] = o;
+}
+
+// ------------------------------------------------------------ gather + kick
+template <class real, bool KICK>
+__global__ __launch_bounds__(256) void k_gather(PState<real> p, Geo<real> g, const typename Vec4<real>::T* __restrict__ F,
+                                                real dt) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= p.n) return;
+    const int64_t n = p.n;
+    const int nx = g.nx;
+    const real dx = g.dx, domain_min = g.domain_min;
+    const real px = p.x[q], py = p.x[n + q], pz = p.x[2 * n + q];
+    const real fx = (px - domain_min) / dx - (real)0.5f;
+    const real fy = (py - domain_min) / dx - (real)0.5f;
+    const real fz = (pz - domain_min) / dx - (real)0.5f;
+    int i0 = (int)floorf((float)fx);
+    int j0 = (int)floorf((float)fy);
+    int k0 = (int)floorf((float)fz);
+    int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
+    const real wx = fx - (real)i0, wy = fy - (real)j0, wz = fz - (real)k0;
+    i0 = max(0, min(i0, nx - 1));
+    i1 = max(0, min(i1, nx - 1));
+    j0 = max(0, min(j0, nx - 1));
+    j1 = max(0, min(j1, nx - 1));
+    k0 = max(0, min(k0, nx - 1));
+    k1 = max(0, min(k1, nx - 1));
+    real ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int k = 0; k <= 1; k++) {
+        const int kk = (k == 0) ? k0 : k1;
+        const real wk = (k == 0) ? ((real)1.0f - wz) : wz;
+#pragma unroll
+        for (int j = 0; j <= 1; j++) {
+            const int jj = (j == 0) ? j0 : j1;
+            const real wj = (j == 0) ? ((real)1.0f - wy) : wy;
+#pragma unroll
+            for (int i = 0; i <= 1; i++) {
+                const int ii = (i == 0) ? i0 : i1;
+                const real wi = (i == 0) ? ((real)1.0f - wx) : wx;
+                const real weight = wi * wj * wk;
+                const typename Vec4<real>::T f = F[kk * nx * nx + jj * nx + ii];
+                ax += weight * f.x;
+                ay += weight * f.y;
+                az += weight * f.z;
+            }
+        }
+    }
+    p.a[q] = ax;
+    p.a[n + q] = ay;
+    p.a[2 * n + q] = az;
+    if (KICK) {  // correct_step_ (nbody.cu:171-180)
+        const real a3[3] = {ax, ay, az};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const real v = p.v[c * n + q];
+            const real vv = p.w[c * n + q];
+            p.v[c * n + q] = (real)(0.5 * (double)(v + vv + dt * a3[c]));
+        }
+    }
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_ngp_phi(PState<real> p, Geo<real> g, const real* __restrict__ phi) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= p.n) return;
+    const int64_t n = p.n;
+    const int NX = g.nx;
+    const int ix = clampr<int>((int)((p.x[q] - g.domain_min) / g.dx), 0, NX - 1);
+    const int iy = clampr<int>((int)((p.x[n + q] - g.domain_min) / g.dx), 0, NX - 1);
+    const int iz = clampr<int>((int)((p.x[2 * n + q] - g.domain_min) / g.dx), 0, NX - 1);
+    p.g[q] = phi[size_t(ix) + size_t(iy) * NX + size_t(iz) * NX * NX];
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t count, unsigned long long value) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
+        p[i] = value;
+}
+
+inline unsigned grid1d(int64_t n, int tpb) { return unsigned((n + tpb - 1) / tpb); }
+inline dim3 grid3d(int nx) { return dim3(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned(nx)); }
+
+}  // namespace
+
+template <class real> Geo<real> make_geo(int nx, double dx_in, double bc_frac) {
+    (void)bc_frac;
+    Geo<real> g;
+    g.nx = nx;
+    g.dx = (real)dx_in;
+    const real domain_l = g.dx * nx;             // io_json.cpp:32
+    g.domain_min = (real)(-0.5 * domain_l);      // io_json.cpp:39
+    g.inv_dx = (real)(1.f / g.dx);               // rm_p2mesh.cu:185
+    const real eps = (real)1E-6;                 // rm_p2mesh.cu:196
+    g.upper = (real)(nx - 1) - eps;
+    return g;
+}
+
+template <class real> WaveK<real> wave_coef(real diss, real C, real DT, real DX) {
+    const int DIM = 3;  // wave.cu:164-171
+    const real Q = diss * C * DT * real(0.5);
+    const real W = 1 + Q;
+    const real K = C * C * DT * DT / (DX * DX);
+    WaveK<real> k;
+    k.k_main = K / W;
+    k.k_f = -C * C * DT * DT / W;
+    k.k_actual = 2 * (1 - DIM * K) / W;
+    k.k_old = (Q - 1) / W;
+    return k;
+}
+
+template <class real>
+void predict_deposit(hipStream_t s, const PState<real>& p, real dt, bool predict, const Geo<real>& g, double scale,
+                     unsigned long long* mfix) {
+    if (p.n == 0) return;
+    k_predict_deposit<real><<<grid1d(p.n, 256), 256, 0, s>>>(p, dt, predict ? 1 : 0, g, scale, mfix);
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real>
+void fixed_to_source(hipStream_t s, int nx, unsigned long long* mfix, double inv_scale, real dx, real rho_mean,
+                     bool periodic, real k_f, real* mass, real* src) {
+    const int64_t ncell = int64_t(nx) * nx * nx;
+    const real dx3 = dx * dx * dx;
+    k_fixed_to_source<real><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mfix, inv_scale, dx3, rho_mean,
+                                                              periodic ? 1 : 0, k_f, mass, src);
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real>
+void mass_to_source(hipStream_t s, int nx, const real* mass, real dx, real rho_mean, bool periodic, real k_f,
+                    real* src) {
+    const int64_t ncell = int64_t(nx) * nx * nx;
+    const real dx3 = dx * dx * dx;
+    k_mass_to_source<real><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mass, dx3, rho_mean, periodic ? 1 : 0, k_f, src);
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real>
+void wave_iteration(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* pn, const real* p,
+                    const real* po, const real* src, void* maxbits) {
+    using MB = typename MaxBits<real>::T;
+    MB* mb = reinterpret_cast<MB*>(maxbits);
+    const dim3 grid = grid3d(nx), block(64, 4, 1);
+    if (periodic) {
+        if (mb) k_wave<real, true, true><<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb);
+        else k_wave<real, true, false><<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb);
+    } else {
+        if (mb) k_wave<real, false, true><<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb);
+        else k_wave<real, false, false><<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb);
+    }
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real> void gradient(hipStream_t s, int nx, real dx, const real* phi, real* acc4) {
+    const real coef = (real)(-1. / (double)dx);  // rm_p2mesh.cu:284
+    k_gradient<real><<<grid3d(nx), dim3(64, 4, 1), 0, s>>>(nx, coef, phi,
+                                                           reinterpret_cast<typename Vec4<real>::T*>(acc4));
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real>
+void gather(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real* acc4, real dt, bool kick) {
+    if (p.n == 0) return;
+    auto F = reinterpret_cast<const typename Vec4<real>::T*>(acc4);
+    if (kick) k_gather<real, true><<<grid1d(p.n, 256), 256, 0, s>>>(p, g, F, dt);
+    else k_gather<real, false><<<grid1d(p.n, 256), 256, 0, s>>>(p, g, F, dt);
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real> void ngp_phi(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real* phi) {
+    if (p.n == 0) return;
+    k_ngp_phi<real><<<grid1d(p.n, 256), 256, 0, s>>>(p, g, phi);
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real> void max_diff(hipStream_t s, int64_t ncell, const real* a, const real* b, void* maxbits) {
+    using MB = typename MaxBits<real>::T;
+    const unsigned nb = unsigned(std::min<int64_t>(4096, (ncell + 255) / 256));
+    k_max_diff<real><<<nb, 256, 0, s>>>(ncell, a, b, reinterpret_cast<MB*>(maxbits));
+    RRG_LAUNCH_CHECK();
+}
+
+void fill_u64(hipStream_t s, unsigned long long* p, size_t count, unsigned long long value) {
+    if (count == 0) return;
+    const unsigned nb = unsigned(std::min<size_t>(4096, (count + 255) / 256));
+    k_fill_u64<<<nb, 256, 0, s>>>(p, count, value);
+    RRG_LAUNCH_CHECK();
+}
+
+#define RRG_PM_INST(real)                                                                                       \
+    template Geo<real> make_geo<real>(int, double, double);                                                     \
+    template WaveK<real> wave_coef<real>(real, real, real, real);                                               \
+    template void predict_deposit<real>(hipStream_t, const PState<real>&, real, bool, const Geo<real>&, double, \
+                                        unsigned long long*);                                                   \
+    template void fixed_to_source<real>(hipStream_t, int, unsigned long long*, double, real, real, bool, real,  \
+                                        real*, real*);                                                          \
+    template void mass_to_source<real>(hipStream_t, int, const real*, real, real, bool, real, real*);           \
+    template void wave_iteration<real>(hipStream_t, int, bool, const WaveK<real>&, real*, const real*,          \
+                                       const real*, const real*, void*);                                        \
+    template void gradient<real>(hipStream_t, int, real, const real*, real*);                                   \
+    template void gather<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*, real, bool);    \
+    template void ngp_phi<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*);               \
+    template void max_diff<real>(hipStream_t, int64_t, const real*, const real*, void*);
+
+RRG_PM_INST(float)
+RRG_PM_INST(double)
+
+}  // namespace pm
+}  // namespace rrg

@@ -1,0 +1,1000 @@
+// librrg.so C ABI: solver contexts (the reference's IntegrationSolver
+// strategies) and stateless operators.
+//
+// Reference:
+//   IntegrationSolver                 solver/app/include/rrgsim_integration.cuh:22-42
+//   NBodyIntegrationSolver            solver/app/include/rrgsim_integration_nbody.cuh:10-65
+//   RectWaveIntegrationSolver         solver/app/include/rrgsim_integration_rect.cuh:13-123
+//   solve_wave_equation               solver/lib/rrgsim_wave/src/wave.cu:230-297
+//   SteadyStateChecker                solver/lib/rrgsim_wave/src/wave_steady_checker.cu:88-152
+//   calc_conservation                 solver/lib/rrgsim_nbody/src/nbody.cu:246-291
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "rrg_internal.hpp"
+#include "rrg_io.hpp"
+
+#define RRG_API extern "C" __attribute__((visibility("default")))
+
+namespace rrg {
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        return RRG_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return RRG_ERR_INVALID;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return RRG_ERR_INVALID;
+    }
+}
+
+void require(bool cond, const std::string& msg) {
+    if (!cond) fail(RRG_ERR_INVALID, msg);
+}
+
+// Makes `dev` current for the lifetime of the guard.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        RRG_HIP(hipGetDevice(&prev));
+        if (prev != dev) RRG_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// Pinned host staging buffer (allocated, never registered).
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t b) {
+        if (b <= bytes) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        RRG_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
+        bytes = b;
+    }
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+// ----------------------------------------------------------------- particles
+template <class real> struct Particles {
+    int64_t n = 0;
+    DBuf<real> X, V, A, W, M, S2, G;
+
+    void alloc(int64_t count) {
+        n = count;
+        X.alloc(3 * n); V.alloc(3 * n); A.alloc(3 * n); W.alloc(3 * n);
+        M.alloc(n); S2.alloc(n); G.alloc(n);
+    }
+    PState<real> view() const {
+        return PState<real>{X.p, V.p, A.p, W.p, M.p, S2.p, G.p, n};
+    }
+};
+
+template <class real> void aos_to_soa(int64_t n, const double* aos, std::vector<real>& soa) {
+    soa.resize(3 * n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) soa[c * n + i] = (real)aos[3 * i + c];
+}
+template <class real> void soa_to_aos(int64_t n, const std::vector<real>& soa, double* aos) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) aos[3 * i + c] = (double)soa[c * n + i];
+}
+template <class real> void upload_vec(hipStream_t s, int64_t n, const double* aos, real* dst) {
+    std::vector<real> tmp;
+    aos_to_soa<real>(n, aos, tmp);
+    RRG_HIP(hipMemcpyAsync(dst, tmp.data(), sizeof(real) * 3 * n, hipMemcpyHostToDevice, s));
+    RRG_HIP(hipStreamSynchronize(s));
+}
+template <class real> void upload_scalar(hipStream_t s, int64_t n, const double* src, real* dst) {
+    std::vector<real> tmp(n);
+    for (int64_t i = 0; i < n; ++i) tmp[i] = (real)src[i];
+    RRG_HIP(hipMemcpyAsync(dst, tmp.data(), sizeof(real) * n, hipMemcpyHostToDevice, s));
+    RRG_HIP(hipStreamSynchronize(s));
+}
+template <class real> void download_vec(hipStream_t s, int64_t n, const real* src, double* aos) {
+    std::vector<real> tmp(3 * n);
+    RRG_HIP(hipMemcpyAsync(tmp.data(), src, sizeof(real) * 3 * n, hipMemcpyDeviceToHost, s));
+    RRG_HIP(hipStreamSynchronize(s));
+    soa_to_aos<real>(n, tmp, aos);
+}
+template <class real> void download_scalar(hipStream_t s, int64_t n, const real* src, double* dst) {
+    std::vector<real> tmp(n);
+    RRG_HIP(hipMemcpyAsync(tmp.data(), src, sizeof(real) * n, hipMemcpyDeviceToHost, s));
+    RRG_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; ++i) dst[i] = (double)tmp[i];
+}
+void zero_async(hipStream_t s, void* p, size_t bytes) {
+    if (bytes) RRG_HIP(hipMemsetAsync(p, 0, bytes, s));
+}
+
+// ------------------------------------------------------------ stage timing
+// HIP events recorded on the context stream around each timed stage.  The
+// event pairs are resolved (and returned to the pool) by collect().
+class StageTimer {
+public:
+    ~StageTimer() {
+        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+        for (const Rec& r : pending_) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+    void set(bool on) {
+        on_ = on;
+        reset();
+    }
+    bool on() const { return on_; }
+    // Returns a token for stop(); -1 when timing is off.
+    int start(hipStream_t s, int stage) {
+        if (!on_) return -1;
+        Rec r{stage, get(), get(), 0};
+        RRG_HIP(hipEventRecord(r.a, s));
+        pending_.push_back(r);
+        return int(pending_.size()) - 1;
+    }
+    void stop(hipStream_t s, int token, int64_t launches) {
+        if (token < 0) return;
+        Rec& r = pending_[size_t(token)];
+        r.launches = launches;
+        RRG_HIP(hipEventRecord(r.b, s));
+    }
+    void collect(hipStream_t s, rrg_timing* out) {
+        RRG_HIP(hipStreamSynchronize(s));
+        for (const Rec& r : pending_) {
+            float ms = 0.f;
+            RRG_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            ms_[r.stage] += ms;
+            launches_[r.stage] += r.launches;
+            pool_.push_back(r.a);
+            pool_.push_back(r.b);
+        }
+        pending_.clear();
+        for (int k = 0; k < RRG_NSTAGES; ++k) {
+            out->ms[k] = ms_[k];
+            out->launches[k] = launches_[k];
+        }
+    }
+
+private:
+    struct Rec {
+        int stage;
+        hipEvent_t a, b;
+        int64_t launches;
+    };
+    hipEvent_t get() {
+        if (!pool_.empty()) {
+            hipEvent_t e = pool_.back();
+            pool_.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        RRG_HIP(hipEventCreate(&e));
+        return e;
+    }
+    void reset() {
+        for (const Rec& r : pending_) {
+            pool_.push_back(r.a);
+            pool_.push_back(r.b);
+        }
+        pending_.clear();
+        for (int k = 0; k < RRG_NSTAGES; ++k) {
+            ms_[k] = 0.0;
+            launches_[k] = 0;
+        }
+    }
+    bool on_ = false;
+    std::vector<Rec> pending_;
+    std::vector<hipEvent_t> pool_;
+    double ms_[RRG_NSTAGES] = {};
+    int64_t launches_[RRG_NSTAGES] = {};
+};
+
+// ------------------------------------------------------------------ solvers
+struct Solver {
+    StageTimer timer;
+    virtual ~Solver() = default;
+    virtual void precalc() = 0;
+    virtual void step(int nsteps) = 0;
+    virtual void sync() = 0;
+    virtual void get_state(double* pos, double* vel, double* acc, double* grav) = 0;
+    virtual void set_state(const double* pos, const double* vel) = 0;
+    virtual void conservation(rrg_conservation* out) = 0;
+    virtual void get_grid(double* mass, double* phi, double* acc) = 0;
+    virtual void stats(rrg_stats* out) = 0;
+};
+
+template <class real> void conservation_common(hipStream_t s, const PState<real>& v, DBuf<double>& part,
+                                               PinnedBuf& host, double time, rrg_conservation* out) {
+    const int64_t nb = nbody::conservation_blocks(v.n);
+    if (part.n < size_t(nb * 8)) part.alloc(nb * 8);
+    nbody::conservation<real>(s, v, part.p);
+    host.ensure(sizeof(double) * nb * 8);
+    RRG_HIP(hipMemcpyAsync(host.p, part.p, sizeof(double) * nb * 8, hipMemcpyDeviceToHost, s));
+    RRG_HIP(hipStreamSynchronize(s));
+    const double* h = static_cast<const double*>(host.p);
+    double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t b = 0; b < nb; ++b)
+        for (int k = 0; k < 8; ++k) q[k] += h[b * 8 + k];
+    out->time = time;
+    out->Ek = 0.5 * q[0];
+    out->Ep = 0.5 * q[1];
+    for (int c = 0; c < 3; ++c) {
+        out->momentum[c] = q[2 + c];
+        out->angular[c] = q[5 + c];
+    }
+}
+
+// Direct-sum strategy (NBodyIntegrationSolver), FP64 particle state.
+class NBodySolver final : public Solver {
+public:
+    NBodySolver(const rrg_params& p, hipStream_t s, int64_t n, const double* pos, const double* vel,
+                const double* mass, const double* soft2)
+        : prm_(p), s_(s), kind_(p.force_kind) {
+        require(soft2 != nullptr, "rrg_create: soft2 is required by the direct-sum model");
+        require(kind_ == RRG_FORCE_F64 || kind_ == RRG_FORCE_MIXED, "rrg_create: unknown force_kind");
+        P_.alloc(n);
+        upload_vec<double>(s_, n, pos, P_.X.p);
+        upload_vec<double>(s_, n, vel, P_.V.p);
+        upload_scalar<double>(s_, n, mass, P_.M.p);
+        upload_scalar<double>(s_, n, soft2, P_.S2.p);
+        zero_async(s_, P_.A.p, P_.A.bytes());
+        zero_async(s_, P_.W.p, P_.W.bytes());
+        zero_async(s_, P_.G.p, P_.G.bytes());
+        plan_ = nbody::plan_direct(n, kind_);
+        if (plan_.slices > 1) part_.alloc(size_t(plan_.slices) * 3 * n);
+        if (kind_ == RRG_FORCE_MIXED) {
+            A_.alloc(n);
+            B_.alloc(n);
+        }
+        RRG_HIP(hipStreamSynchronize(s_));
+    }
+
+    void precalc() override {  // rrgsim_integration_nbody.cuh:26-30
+        if (kind_ == RRG_FORCE_MIXED) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
+        grav();
+        accel(true);
+        grav_actual_ = true;
+        precalc_done_ = true;
+    }
+
+    void step(int nsteps) override {  // rrgsim_integration_nbody.cuh:50-56
+        if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_step before rrg_precalc");
+        const double dt = prm_.sim.dt_dynamics;
+        for (int k = 0; k < nsteps; ++k) {
+            int t = timer.start(s_, RRG_STAGE_PREDICT);
+            nbody::predict<double>(s_, P_.view(), dt, A_.p, B_.p);
+            timer.stop(s_, t, 1);
+            t = timer.start(s_, RRG_STAGE_DIRECT);
+            accel(false);
+            timer.stop(s_, t, 1);
+            t = timer.start(s_, RRG_STAGE_CORRECT);
+            nbody::correct_reduce(s_, P_.view(), dt, plan_.slices, plan_.slices > 1 ? part_.p : nullptr);
+            timer.stop(s_, t, 1);
+            ++steps_;
+            time_ += dt;
+        }
+        grav_actual_ = false;
+    }
+
+    // The reference returns early without a grid and leaves the potential
+    // stale (rrgsim_integration_nbody.cuh:32-35); here it is recomputed.
+    void sync() override {
+        if (!grav_actual_) grav();
+        grav_actual_ = true;
+    }
+
+    void get_state(double* pos, double* vel, double* acc, double* gr) override {
+        const int64_t n = P_.n;
+        if (pos) download_vec<double>(s_, n, P_.X.p, pos);
+        if (vel) download_vec<double>(s_, n, P_.V.p, vel);
+        if (acc) download_vec<double>(s_, n, P_.A.p, acc);
+        if (gr) download_scalar<double>(s_, n, P_.G.p, gr);
+    }
+
+    void set_state(const double* pos, const double* vel) override {
+        if (pos) upload_vec<double>(s_, P_.n, pos, P_.X.p);
+        if (vel) upload_vec<double>(s_, P_.n, vel, P_.V.p);
+        if (pos && kind_ == RRG_FORCE_MIXED) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
+        grav_actual_ = false;
+    }
+
+    void conservation(rrg_conservation* out) override {
+        sync();
+        conservation_common<double>(s_, P_.view(), cpart_, host_, time_, out);
+    }
+
+    void get_grid(double*, double*, double*) override {
+        fail(RRG_ERR_STATE, "rrg_get_grid: the direct-sum model has no grid");
+    }
+
+    void stats(rrg_stats* out) override {
+        std::memset(out, 0, sizeof(*out));
+        out->n = P_.n;
+        out->steps = steps_;
+        out->time = time_;
+    }
+
+private:
+    void accel(bool reduce_now) {
+        const int64_t n = P_.n;
+        double* out = plan_.slices > 1 ? part_.p : P_.A.p;
+        if (kind_ == RRG_FORCE_MIXED) nbody::accel_mixed(s_, plan_, n, A_.p, B_.p, out);
+        else nbody::accel_f64(s_, plan_, P_.view(), out, rsqrt_kind_);
+        if (reduce_now && plan_.slices > 1) nbody::reduce_slices(s_, n, 3, plan_.slices, part_.p, P_.A.p);
+    }
+    void grav() {
+        const int64_t n = P_.n;
+        nbody::Plan pl = plan_;
+        DBuf<double>* tmp = nullptr;
+        double* out = P_.G.p;
+        if (pl.slices > 1) {
+            if (gpart_.n < size_t(pl.slices) * n) gpart_.alloc(size_t(pl.slices) * n);
+            tmp = &gpart_;
+            out = gpart_.p;
+        }
+        if (kind_ == RRG_FORCE_MIXED) nbody::grav_mixed(s_, pl, n, A_.p, B_.p, out);
+        else nbody::grav_f64(s_, pl, P_.view(), out, rsqrt_kind_);
+        if (tmp) nbody::reduce_slices(s_, n, 1, pl.slices, tmp->p, P_.G.p);
+    }
+
+    rrg_params prm_;
+    hipStream_t s_;
+    int kind_;
+    int rsqrt_kind_ = 0;
+    Particles<double> P_;
+    nbody::Plan plan_;
+    DBuf<double> part_, gpart_, cpart_;
+    DBuf<float4> A_, B_;
+    PinnedBuf host_;
+    bool precalc_done_ = false;
+    bool grav_actual_ = false;
+    int64_t steps_ = 0;
+    double time_ = 0.0;
+};
+
+// Particle-mesh strategy (RectWaveIntegrationSolver), state in `real`.
+template <class real> class WaveSolver final : public Solver {
+public:
+    WaveSolver(const rrg_params& p, hipStream_t s, int64_t n, const double* pos, const double* vel, const double* mass)
+        : prm_(p), s_(s) {
+        const rrg_grid_params& g = p.grid;
+        require(g.nx >= 2 && g.nx <= 1290, "rrg_create: grid nx must be in [2, 1290]");
+        require(g.dx > 0, "rrg_create: grid dx must be positive");
+        require(p.wave.iterations >= 0 && p.wave.setup_iterations >= 0, "rrg_create: negative wave iterations");
+        require(p.wave.steady_check_step > 0, "rrg_create: steady_check_step must be positive");
+        require(p.boundary == RRG_BC_PERIODIC || p.boundary == RRG_BC_ZERO_DIRICHLET, "rrg_create: unknown boundary");
+        nx_ = g.nx;
+        ncell_ = int64_t(nx_) * nx_ * nx_;
+        periodic_ = p.boundary == RRG_BC_PERIODIC;
+        geo_ = pm::make_geo<real>(nx_, g.dx, g.bc_frac);
+        const real dxr = (real)g.dx;
+        const real C = (real)p.wave.wave_speed;
+        const real DT = (real)p.wave.dt;
+        k_ = pm::wave_coef<real>((real)p.wave.diss_base, C, DT, dxr);
+        // set_boundary_periodic(1., ...) (rrgsim_integration_rect.cuh:65-69; wave.cu:214-228)
+        {
+            const real L = nx_ * dxr;
+            rho_mean_ = (real)1. / (L * L * L);
+        }
+        tol_ = (real)p.wave.tolerance;
+        // Fixed-point exponent: the largest e with sum|m| * 2^e below 2^62.
+        double msum = 0;
+        for (int64_t i = 0; i < n; ++i) msum += std::fabs(mass[i]);
+        if (p.fixed_exponent > 0) {
+            e_ = p.fixed_exponent;
+        } else {
+            int ex = 0;
+            std::frexp(std::max(msum, 1e-300), &ex);  // msum in [2^(ex-1), 2^ex)
+            e_ = 61 - ex;
+            if (e_ > 1000) e_ = 1000;
+        }
+        require(std::ldexp(std::max(msum, 0.0), e_) < std::ldexp(1.0, 62), "rrg_create: fixed_exponent too large for total mass");
+        scale_ = std::ldexp(1.0, e_);
+        inv_scale_ = std::ldexp(1.0, -e_);
+
+        P_.alloc(n);
+        upload_vec<real>(s_, n, pos, P_.X.p);
+        upload_vec<real>(s_, n, vel, P_.V.p);
+        upload_scalar<real>(s_, n, mass, P_.M.p);
+        zero_async(s_, P_.S2.p, P_.S2.bytes());
+        zero_async(s_, P_.A.p, P_.A.bytes());
+        zero_async(s_, P_.W.p, P_.W.bytes());
+        zero_async(s_, P_.G.p, P_.G.bytes());
+        mfix_.alloc(ncell_);
+        mass_.alloc(ncell_);
+        src_.alloc(ncell_);
+        for (auto& b : phi_) b.alloc(ncell_);
+        gacc_.alloc(4 * ncell_);
+        maxbits_.alloc(2);
+        zero_async(s_, mfix_.p, mfix_.bytes());
+        zero_async(s_, mass_.p, mass_.bytes());
+        zero_async(s_, src_.p, src_.bytes());
+        for (auto& b : phi_) zero_async(s_, b.p, b.bytes());  // initial potential: zero
+        zero_async(s_, gacc_.p, gacc_.bytes());
+        RRG_HIP(hipStreamSynchronize(s_));
+    }
+
+    void precalc() override {  // rrgsim_integration_rect.cuh:53-83
+        deposit(false);
+        last_iters_ = solve(prm_.wave.setup_iterations);
+        pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
+        grav_actual_ = false;
+        acc_actual_ = false;
+        precalc_done_ = true;
+    }
+
+    void step(int nsteps) override {  // rrgsim_integration_rect.cuh:99-112
+        if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_step before rrg_precalc");
+        const real dt = (real)prm_.sim.dt_dynamics;
+        for (int k = 0; k < nsteps; ++k) {
+            deposit(true);
+            int t = timer.start(s_, RRG_STAGE_WAVE);
+            last_iters_ = solve(prm_.wave.iterations);
+            timer.stop(s_, t, last_iters_);
+            t = timer.start(s_, RRG_STAGE_GRADIENT);
+            pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
+            timer.stop(s_, t, 1);
+            t = timer.start(s_, RRG_STAGE_GATHER);
+            pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true);
+            timer.stop(s_, t, 1);
+            ++steps_;
+            time_ += prm_.sim.dt_dynamics;
+        }
+        if (nsteps > 0) {
+            acc_actual_ = true;
+            grav_actual_ = false;
+        }
+    }
+
+    void sync() override {  // rrgsim_integration_rect.cuh:85-97
+        if (!grav_actual_) pm::ngp_phi<real>(s_, P_.view(), geo_, cur());
+        if (!acc_actual_) pm::gather<real>(s_, P_.view(), geo_, gacc_.p, (real)0, false);
+        grav_actual_ = acc_actual_ = true;
+    }
+
+    void get_state(double* pos, double* vel, double* acc, double* gr) override {
+        const int64_t n = P_.n;
+        if (pos) download_vec<real>(s_, n, P_.X.p, pos);
+        if (vel) download_vec<real>(s_, n, P_.V.p, vel);
+        if (acc) download_vec<real>(s_, n, P_.A.p, acc);
+        if (gr) download_scalar<real>(s_, n, P_.G.p, gr);
+    }
+
+    void set_state(const double* pos, const double* vel) override {
+        if (pos) upload_vec<real>(s_, P_.n, pos, P_.X.p);
+        if (vel) upload_vec<real>(s_, P_.n, vel, P_.V.p);
+        grav_actual_ = acc_actual_ = false;
+    }
+
+    void conservation(rrg_conservation* out) override {
+        sync();
+        conservation_common<real>(s_, P_.view(), cpart_, host_, time_, out);
+    }
+
+    void get_grid(double* mass, double* phi, double* acc) override {
+        if (mass) download_scalar<real>(s_, ncell_, mass_.p, mass);
+        if (phi) download_scalar<real>(s_, ncell_, cur(), phi);
+        if (acc) {
+            std::vector<real> tmp(4 * ncell_);
+            RRG_HIP(hipMemcpyAsync(tmp.data(), gacc_.p, sizeof(real) * 4 * ncell_, hipMemcpyDeviceToHost, s_));
+            RRG_HIP(hipStreamSynchronize(s_));
+            for (int64_t c = 0; c < ncell_; ++c)
+                for (int q = 0; q < 3; ++q) acc[3 * c + q] = (double)tmp[4 * c + q];
+        }
+    }
+
+    void stats(rrg_stats* out) override {
+        std::memset(out, 0, sizeof(*out));
+        out->n = P_.n;
+        out->steps = steps_;
+        out->time = time_;
+        out->last_wave_iterations = last_iters_;
+        out->fixed_exponent = e_;
+    }
+
+private:
+    real* cur() { return phi_[ic_].p; }
+
+    void deposit(bool predict) {
+        int t = timer.start(s_, RRG_STAGE_DEPOSIT);
+        pm::predict_deposit<real>(s_, P_.view(), (real)prm_.sim.dt_dynamics, predict, geo_, scale_, mfix_.p);
+        timer.stop(s_, t, 1);
+        t = timer.start(s_, RRG_STAGE_SOURCE);
+        pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, mass_.p, src_.p);
+        timer.stop(s_, t, 1);
+    }
+
+    // solve_wave_equation (wave.cu:230-297) with the SteadyStateChecker
+    // (wave_steady_checker.cu:95-143).  A tolerance <= 0 can never be met
+    // (the error is >= 0), so the checks are skipped and exactly max_iter
+    // iterations run.
+    int solve(int max_iter) {
+        using MB = std::conditional_t<std::is_same_v<real, float>, unsigned int, unsigned long long>;
+        const bool checking = tol_ > (real)0;
+        const int check_step = prm_.wave.steady_check_step;
+        int hold_count = 0;
+        int iter = 0;
+        for (iter = 0; iter < max_iter; ++iter) {
+            const bool check = checking && iter && (iter % check_step == 0);
+            if (check) pm::fill_u64(s_, reinterpret_cast<unsigned long long*>(maxbits_.p), 2, 0ull);
+            const int inx = 3 - ic_ - ip_;
+            pm::wave_iteration<real>(s_, nx_, periodic_, k_, phi_[inx].p, phi_[ic_].p, phi_[ip_].p, src_.p,
+                                     check ? maxbits_.p : nullptr);
+            ip_ = ic_;
+            ic_ = inx;
+            if (check) {
+                host_.ensure(sizeof(MB) * 2);
+                RRG_HIP(hipMemcpyAsync(host_.p, maxbits_.p, sizeof(MB) * 2, hipMemcpyDeviceToHost, s_));
+                RRG_HIP(hipStreamSynchronize(s_));
+                MB bits[2];
+                std::memcpy(bits, host_.p, sizeof(bits));
+                real diff, absu;
+                std::memcpy(&diff, &bits[0], sizeof(real));
+                std::memcpy(&absu, &bits[1], sizeof(real));
+                const real EPS = (real)1.E-30;
+                const real err = diff / (absu + EPS);
+                if ((double)err < (double)tol_) ++hold_count;
+                else hold_count = 0;
+                if (hold_count >= prm_.wave.steady_hold_steps) {
+                    ++iter;
+                    break;
+                }
+            }
+        }
+        return iter;
+    }
+
+    rrg_params prm_;
+    hipStream_t s_;
+    int nx_ = 0;
+    int64_t ncell_ = 0;
+    bool periodic_ = true;
+    pm::Geo<real> geo_{};
+    pm::WaveK<real> k_{};
+    real rho_mean_ = 0, tol_ = 0;
+    int e_ = 61;
+    double scale_ = 0, inv_scale_ = 0;
+    Particles<real> P_;
+    DBuf<unsigned long long> mfix_;
+    DBuf<real> mass_, src_, phi_[3], gacc_;
+    DBuf<unsigned long long> maxbits_;
+    DBuf<double> cpart_;
+    PinnedBuf host_;
+    int ip_ = 0, ic_ = 1;  // indices of phi^{n-1} (prev) and phi^n (curr)
+    bool precalc_done_ = false, grav_actual_ = false, acc_actual_ = false;
+    int last_iters_ = 0;
+    int64_t steps_ = 0;
+    double time_ = 0.0;
+};
+
+}  // namespace
+}  // namespace rrg
+
+struct rrg_ctx {
+    rrg_params p;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<rrg::Solver> solver;
+};
+
+using namespace rrg;
+
+namespace {
+
+void check_ctx(rrg_ctx* ctx) {
+    if (!ctx || !ctx->solver) fail(RRG_ERR_INVALID, "null or destroyed context");
+}
+
+void check_finite(int64_t n, const double* a, const char* what) {
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) fail(RRG_ERR_INVALID, std::string(what) + " holds a non-finite value");
+}
+
+}  // namespace
+
+RRG_API const char* rrg_last_error(void) { return g_last_error.c_str(); }
+
+RRG_API const char* rrg_version(void) { return "rrg-mi355x 0.1 (gfx950)"; }
+
+RRG_API int rrg_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+RRG_API int rrg_params_default(rrg_params* p) {
+    return guarded([&] {
+        require(p != nullptr, "rrg_params_default: null params");
+        io::params_default(p);
+    });
+}
+
+RRG_API int rrg_params_finalize(rrg_params* p) {
+    return guarded([&] {
+        require(p != nullptr, "rrg_params_finalize: null params");
+        io::params_finalize(p);
+    });
+}
+
+RRG_API int rrg_params_parse_json(const char* path, rrg_params* p, rrg_ini_params* ini) {
+    return guarded([&] {
+        require(path && p, "rrg_params_parse_json: null argument");
+        io::params_parse_json(path, p, ini);
+    });
+}
+
+RRG_API int rrg_ini_read(const rrg_ini_params* ini, const char* base_dir, int64_t* n, double* pos, double* vel,
+                         double* mass, double* soft2) {
+    return guarded([&] {
+        require(ini && n, "rrg_ini_read: null argument");
+        io::ini_read(ini, base_dir, n, pos, vel, mass, soft2);
+    });
+}
+
+RRG_API int rrg_create(const rrg_params* p, int64_t n, const double* pos, const double* vel, const double* mass,
+                       const double* soft2, rrg_ctx** out) {
+    return guarded([&] {
+        require(p && out, "rrg_create: null argument");
+        *out = nullptr;
+        require(n > 0, "rrg_create: n must be positive");
+        require(pos && vel && mass, "rrg_create: pos, vel and mass are required");
+        require(n <= (int64_t(1) << 31) - 1, "rrg_create: n exceeds 2^31-1");
+        check_finite(3 * n, pos, "pos");
+        check_finite(3 * n, vel, "vel");
+        check_finite(n, mass, "mass");
+        if (soft2) check_finite(n, soft2, "soft2");
+        int ndev = 0;
+        RRG_HIP(hipGetDeviceCount(&ndev));
+        require(p->device >= 0 && p->device < ndev, "rrg_create: invalid device ordinal");
+        require(p->real_type == RRG_REAL_F64 || p->real_type == RRG_REAL_F32, "rrg_create: unknown real_type");
+        require(std::isfinite(p->sim.dt_dynamics), "rrg_create: sim.dt_dynamics is not set");
+        auto ctx = std::make_unique<rrg_ctx>();
+        ctx->p = *p;
+        io::params_finalize(&ctx->p);
+        ctx->device = p->device;
+        DeviceGuard dg(ctx->device);
+        RRG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        try {
+            if (ctx->p.use_wave_model) {
+                require(std::isfinite(ctx->p.wave.dt) && ctx->p.wave.dt > 0, "rrg_create: wave dt is not positive");
+                if (ctx->p.real_type == RRG_REAL_F32)
+                    ctx->solver = std::make_unique<WaveSolver<float>>(ctx->p, ctx->stream, n, pos, vel, mass);
+                else
+                    ctx->solver = std::make_unique<WaveSolver<double>>(ctx->p, ctx->stream, n, pos, vel, mass);
+            } else {
+                require(ctx->p.real_type == RRG_REAL_F64,
+                        "rrg_create: the direct-sum model keeps FP64 state (use force_kind for FP32 pair math)");
+                ctx->solver = std::make_unique<NBodySolver>(ctx->p, ctx->stream, n, pos, vel, mass, soft2);
+            }
+        } catch (...) {
+            ctx->solver.reset();
+            (void)hipStreamDestroy(ctx->stream);
+            throw;
+        }
+        *out = ctx.release();
+    });
+}
+
+RRG_API int rrg_destroy(rrg_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        {
+            DeviceGuard dg(ctx->device);
+            if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+            ctx->solver.reset();
+            if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        }
+        delete ctx;
+    });
+}
+
+#define RRG_CTX_CALL(ctx, body)              \
+    return guarded([&] {                     \
+        check_ctx(ctx);                      \
+        DeviceGuard dg((ctx)->device);       \
+        body;                                \
+    })
+
+RRG_API int rrg_precalc(rrg_ctx* ctx) { RRG_CTX_CALL(ctx, ctx->solver->precalc()); }
+
+RRG_API int rrg_step(rrg_ctx* ctx, int32_t nsteps) {
+    RRG_CTX_CALL(ctx, {
+        require(nsteps >= 0, "rrg_step: nsteps must be non-negative");
+        ctx->solver->step(nsteps);
+    });
+}
+
+RRG_API int rrg_sync(rrg_ctx* ctx) { RRG_CTX_CALL(ctx, ctx->solver->sync()); }
+
+RRG_API int rrg_get_state(rrg_ctx* ctx, double* pos, double* vel, double* acc, double* grav) {
+    RRG_CTX_CALL(ctx, ctx->solver->get_state(pos, vel, acc, grav));
+}
+
+RRG_API int rrg_set_state(rrg_ctx* ctx, const double* pos, const double* vel) {
+    RRG_CTX_CALL(ctx, ctx->solver->set_state(pos, vel));
+}
+
+RRG_API int rrg_get_conservation(rrg_ctx* ctx, rrg_conservation* out) {
+    RRG_CTX_CALL(ctx, {
+        require(out != nullptr, "rrg_get_conservation: null output");
+        ctx->solver->conservation(out);
+    });
+}
+
+RRG_API int rrg_get_grid(rrg_ctx* ctx, double* mass, double* phi, double* acc) {
+    RRG_CTX_CALL(ctx, ctx->solver->get_grid(mass, phi, acc));
+}
+
+RRG_API int rrg_get_stats(rrg_ctx* ctx, rrg_stats* out) {
+    RRG_CTX_CALL(ctx, {
+        require(out != nullptr, "rrg_get_stats: null output");
+        ctx->solver->stats(out);
+    });
+}
+
+RRG_API int rrg_synchronize(rrg_ctx* ctx) { RRG_CTX_CALL(ctx, RRG_HIP(hipStreamSynchronize(ctx->stream))); }
+
+RRG_API int rrg_set_timing(rrg_ctx* ctx, int32_t enabled) {
+    RRG_CTX_CALL(ctx, {
+        RRG_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->solver->timer.set(enabled != 0);
+    });
+}
+
+RRG_API int rrg_get_timing(rrg_ctx* ctx, rrg_timing* out) {
+    RRG_CTX_CALL(ctx, {
+        require(out != nullptr, "rrg_get_timing: null output");
+        ctx->solver->timer.collect(ctx->stream, out);
+    });
+}
+
+// ------------------------------------------------------------------ operators
+namespace {
+
+struct OpStream {
+    hipStream_t s = nullptr;
+    DeviceGuard dg{0};
+    OpStream() { RRG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~OpStream() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+};
+
+size_t real_size(int32_t real_type) {
+    if (real_type == RRG_REAL_F64) return sizeof(double);
+    if (real_type == RRG_REAL_F32) return sizeof(float);
+    fail(RRG_ERR_INVALID, "unknown real_type");
+}
+
+template <class T> void h2d(hipStream_t s, T* dst, const void* src, size_t bytes) {
+    RRG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+}
+template <class T> void d2h(hipStream_t s, void* dst, const T* src, size_t bytes) {
+    RRG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    RRG_HIP(hipStreamSynchronize(s));
+}
+
+void direct_op(int32_t kind, int64_t n, const double* pos, const double* mass, const double* soft2, double* out,
+               bool potential) {
+    require(n > 0 && pos && mass && soft2 && out, "direct-sum operator: bad arguments");
+    require(kind == RRG_FORCE_F64 || kind == RRG_FORCE_MIXED, "direct-sum operator: unknown force_kind");
+    OpStream os;
+    Particles<double> P;
+    P.alloc(n);
+    upload_vec<double>(os.s, n, pos, P.X.p);
+    upload_scalar<double>(os.s, n, mass, P.M.p);
+    upload_scalar<double>(os.s, n, soft2, P.S2.p);
+    nbody::Plan pl = nbody::plan_direct(n, kind);
+    const int comps = potential ? 1 : 3;
+    DBuf<double> part(size_t(pl.slices) * comps * n);
+    DBuf<double> res(size_t(comps) * n);
+    DBuf<float4> A, B;
+    if (kind == RRG_FORCE_MIXED) {
+        A.alloc(n);
+        B.alloc(n);
+        nbody::pack_mixed(os.s, P.view(), A.p, B.p);
+        if (potential) nbody::grav_mixed(os.s, pl, n, A.p, B.p, part.p);
+        else nbody::accel_mixed(os.s, pl, n, A.p, B.p, part.p);
+    } else {
+        if (potential) nbody::grav_f64(os.s, pl, P.view(), part.p, 0);
+        else nbody::accel_f64(os.s, pl, P.view(), part.p, 0);
+    }
+    nbody::reduce_slices(os.s, n, comps, pl.slices, part.p, res.p);
+    if (potential) {
+        d2h(os.s, out, res.p, sizeof(double) * n);
+    } else {
+        std::vector<double> soa(3 * n);
+        d2h(os.s, soa.data(), res.p, sizeof(double) * 3 * n);
+        soa_to_aos<double>(n, soa, out);
+    }
+}
+
+template <class real>
+void cic_op(int64_t n, const void* pos, const void* mass, int32_t nx, double dx, double bc_frac, int32_t e,
+            int64_t* grid) {
+    OpStream os;
+    const int64_t ncell = int64_t(nx) * nx * nx;
+    Particles<real> P;
+    P.alloc(n);
+    std::vector<real> soa(3 * n);
+    const real* pr = static_cast<const real*>(pos);
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) soa[c * n + i] = pr[3 * i + c];
+    h2d(os.s, P.X.p, soa.data(), sizeof(real) * 3 * n);
+    h2d(os.s, P.M.p, mass, sizeof(real) * n);
+    DBuf<unsigned long long> g(ncell);
+    pm::fill_u64(os.s, g.p, ncell, 0ull);
+    const pm::Geo<real> geo = pm::make_geo<real>(nx, dx, bc_frac);
+    pm::predict_deposit<real>(os.s, P.view(), (real)0, false, geo, std::ldexp(1.0, e), g.p);
+    d2h(os.s, grid, g.p, sizeof(int64_t) * ncell);
+}
+
+template <class real>
+void wave_op(int32_t nx, double dx, double diss, double c, double wdt, int32_t boundary, double rho_mean,
+             int32_t iters, void* prev, void* curr, const void* mass) {
+    OpStream os;
+    const int64_t ncell = int64_t(nx) * nx * nx;
+    DBuf<real> b[3], m(ncell), src(ncell);
+    for (auto& x : b) x.alloc(ncell);
+    h2d(os.s, b[0].p, prev, sizeof(real) * ncell);
+    h2d(os.s, b[1].p, curr, sizeof(real) * ncell);
+    h2d(os.s, m.p, mass, sizeof(real) * ncell);
+    const bool periodic = boundary == RRG_BC_PERIODIC;
+    const pm::WaveK<real> k = pm::wave_coef<real>((real)diss, (real)c, (real)wdt, (real)dx);
+    pm::mass_to_source<real>(os.s, nx, m.p, (real)dx, (real)rho_mean, periodic, k.k_f, src.p);
+    int ip = 0, ic = 1;
+    for (int it = 0; it < iters; ++it) {
+        const int in = 3 - ic - ip;
+        pm::wave_iteration<real>(os.s, nx, periodic, k, b[in].p, b[ic].p, b[ip].p, src.p, nullptr);
+        ip = ic;
+        ic = in;
+    }
+    d2h(os.s, prev, b[ip].p, sizeof(real) * ncell);
+    d2h(os.s, curr, b[ic].p, sizeof(real) * ncell);
+}
+
+template <class real> void gradient_op(int32_t nx, double dx, const void* phi, void* acc) {
+    OpStream os;
+    const int64_t ncell = int64_t(nx) * nx * nx;
+    DBuf<real> p(ncell), a(4 * ncell);
+    h2d(os.s, p.p, phi, sizeof(real) * ncell);
+    pm::gradient<real>(os.s, nx, (real)dx, p.p, a.p);
+    std::vector<real> tmp(4 * ncell);
+    d2h(os.s, tmp.data(), a.p, sizeof(real) * 4 * ncell);
+    real* out = static_cast<real*>(acc);
+    for (int64_t c = 0; c < ncell; ++c)
+        for (int q = 0; q < 3; ++q) out[3 * c + q] = tmp[4 * c + q];
+}
+
+template <class real>
+void gather_op(int64_t n, const void* pos, int32_t nx, double dx, double bc_frac, const void* field, void* acc) {
+    OpStream os;
+    const int64_t ncell = int64_t(nx) * nx * nx;
+    Particles<real> P;
+    P.alloc(n);
+    std::vector<real> soa(3 * n);
+    const real* pr = static_cast<const real*>(pos);
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) soa[c * n + i] = pr[3 * i + c];
+    h2d(os.s, P.X.p, soa.data(), sizeof(real) * 3 * n);
+    std::vector<real> f4(4 * ncell);
+    const real* fr = static_cast<const real*>(field);
+    for (int64_t c = 0; c < ncell; ++c) {
+        for (int q = 0; q < 3; ++q) f4[4 * c + q] = fr[3 * c + q];
+        f4[4 * c + 3] = 0;
+    }
+    DBuf<real> F(4 * ncell);
+    h2d(os.s, F.p, f4.data(), sizeof(real) * 4 * ncell);
+    const pm::Geo<real> geo = pm::make_geo<real>(nx, dx, bc_frac);
+    pm::gather<real>(os.s, P.view(), geo, F.p, (real)0, false);
+    d2h(os.s, soa.data(), P.A.p, sizeof(real) * 3 * n);
+    real* out = static_cast<real*>(acc);
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) out[3 * i + c] = soa[c * n + i];
+}
+
+template <class real> void steady_op(int64_t ncell, const void* un, const void* uo, double* err) {
+    using MB = std::conditional_t<std::is_same_v<real, float>, unsigned int, unsigned long long>;
+    OpStream os;
+    DBuf<real> a(ncell), b(ncell);
+    DBuf<unsigned long long> mb(2);
+    h2d(os.s, a.p, un, sizeof(real) * ncell);
+    h2d(os.s, b.p, uo, sizeof(real) * ncell);
+    pm::fill_u64(os.s, mb.p, 2, 0ull);
+    pm::max_diff<real>(os.s, ncell, a.p, b.p, mb.p);
+    MB bits[2];
+    d2h(os.s, bits, reinterpret_cast<MB*>(mb.p), sizeof(bits));
+    real diff, absu;
+    std::memcpy(&diff, &bits[0], sizeof(real));
+    std::memcpy(&absu, &bits[1], sizeof(real));
+    const real EPS = (real)1.E-30;
+    const real e = diff / (absu + EPS);
+    *err = (double)e;
+}
+
+}  // namespace
+
+RRG_API int rrg_op_direct_accel(int32_t force_kind, int64_t n, const double* pos, const double* mass,
+                                const double* soft2, double* acc) {
+    return guarded([&] { direct_op(force_kind, n, pos, mass, soft2, acc, false); });
+}
+
+RRG_API int rrg_op_direct_grav(int32_t force_kind, int64_t n, const double* pos, const double* mass,
+                               const double* soft2, double* grav) {
+    return guarded([&] { direct_op(force_kind, n, pos, mass, soft2, grav, true); });
+}
+
+RRG_API int rrg_op_cic_deposit(int32_t real_type, int64_t n, const void* pos, const void* mass, int32_t nx, double dx,
+                               double bc_frac, int32_t e, int64_t* grid) {
+    return guarded([&] {
+        require(n > 0 && pos && mass && grid && nx >= 2 && dx > 0 && e >= 0 && e < 1000, "rrg_op_cic_deposit: bad arguments");
+        real_size(real_type);
+        if (real_type == RRG_REAL_F64) cic_op<double>(n, pos, mass, nx, dx, bc_frac, e, grid);
+        else cic_op<float>(n, pos, mass, nx, dx, bc_frac, e, grid);
+    });
+}
+
+RRG_API int rrg_op_wave_iterate(int32_t real_type, int32_t nx, double dx, double diss_base, double wave_speed,
+                                double wave_dt, int32_t boundary, double rho_mean, int32_t iterations, void* phi_prev,
+                                void* phi_curr, const void* mass) {
+    return guarded([&] {
+        require(nx >= 2 && dx > 0 && iterations >= 0 && phi_prev && phi_curr && mass, "rrg_op_wave_iterate: bad arguments");
+        require(boundary == RRG_BC_PERIODIC || boundary == RRG_BC_ZERO_DIRICHLET, "rrg_op_wave_iterate: unknown boundary");
+        real_size(real_type);
+        if (real_type == RRG_REAL_F64)
+            wave_op<double>(nx, dx, diss_base, wave_speed, wave_dt, boundary, rho_mean, iterations, phi_prev, phi_curr, mass);
+        else
+            wave_op<float>(nx, dx, diss_base, wave_speed, wave_dt, boundary, rho_mean, iterations, phi_prev, phi_curr, mass);
+    });
+}
+
+RRG_API int rrg_op_gradient(int32_t real_type, int32_t nx, double dx, const void* phi, void* acc) {
+    return guarded([&] {
+        require(nx >= 2 && dx > 0 && phi && acc, "rrg_op_gradient: bad arguments");
+        real_size(real_type);
+        if (real_type == RRG_REAL_F64) gradient_op<double>(nx, dx, phi, acc);
+        else gradient_op<float>(nx, dx, phi, acc);
+    });
+}
+
+RRG_API int rrg_op_gather(int32_t real_type, int64_t n, const void* pos, int32_t nx, double dx, double bc_frac,
+                          const void* field, void* acc) {
+    return guarded([&] {
+        require(n > 0 && pos && field && acc && nx >= 2 && dx > 0, "rrg_op_gather: bad arguments");
+        real_size(real_type);
+        if (real_type == RRG_REAL_F64) gather_op<double>(n, pos, nx, dx, bc_frac, field, acc);
+        else gather_op<float>(n, pos, nx, dx, bc_frac, field, acc);
+    });
+}
+
+RRG_API int rrg_op_steady_err(int32_t real_type, int64_t ncell, const void* u_new, const void* u_old, double* err) {
+    return guarded([&] {
+        require(ncell > 0 && u_new && u_old && err, "rrg_op_steady_err: bad arguments");
+        real_size(real_type);
+        if (real_type == RRG_REAL_F64) steady_op<double>(ncell, u_new, u_old, err);
+        else steady_op<float>(ncell, u_new, u_old, err);
+    });
+}

@@ -1,0 +1,178 @@
+// Internal declarations shared by the librrg.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "rrg.h"
+#include "rrg_error.hpp"
+
+namespace rrg {
+
+#define RRG_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t rrg_e_ = (call);                                                            \
+        if (rrg_e_ != hipSuccess)                                                              \
+            ::rrg::fail(RRG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(rrg_e_) +   \
+                                         " (" __FILE__ ":" + std::to_string(__LINE__) + ")");  \
+    } while (0)
+
+#define RRG_LAUNCH_CHECK() RRG_HIP(hipGetLastError())
+
+// Owning device array.  Allocation failures surface as rrg::Error.
+template <class T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    void alloc(size_t count) {
+        release();
+        if (count) RRG_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n;
+            o.p = nullptr; o.n = 0;
+        }
+        return *this;
+    }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Particle state in structure-of-arrays form: component c of a vector array
+// starts at c * n.  `w` is the predicted velocity v* of the predictor step.
+template <class real> struct PState {
+    real* x;   // 3n
+    real* v;   // 3n
+    real* a;   // 3n
+    real* w;   // 3n
+    real* m;   // n
+    real* s2;  // n (direct sum only)
+    real* g;   // n, particle potential
+    int64_t n;
+};
+
+// ----------------------------------------------------------------- direct sum
+namespace nbody {
+
+// Launch shape of a direct-sum evaluation: `ib` targets per thread, the j
+// range split into `slices` contiguous slices of `slice_len` sources, each
+// handled by its own workgroup row; partial results are summed in slice order.
+struct Plan {
+    int ib = 1;
+    int slices = 1;
+    int64_t slice_len = 0;
+    int64_t tiles = 0;  // workgroups along the target axis
+};
+
+Plan plan_direct(int64_t n, int force_kind, int target_workgroups = 0);
+
+// Packed FP32 source records for the mixed kernel:
+//   A = (x_hi, y_hi, z_hi, m), B = (x_lo, y_lo, z_lo, 0.5*soft2).
+void pack_mixed(hipStream_t s, const PState<double>& p, float4* A, float4* B);
+
+// Acceleration partials: out[(slice*3 + c)*n + i].
+void accel_f64(hipStream_t s, const Plan& pl, const PState<double>& p, double* out, int rsqrt_kind);
+void accel_mixed(hipStream_t s, const Plan& pl, int64_t n, const float4* A, const float4* B, double* out);
+// Potential partials: out[slice*n + i].
+void grav_f64(hipStream_t s, const Plan& pl, const PState<double>& p, double* out, int rsqrt_kind);
+void grav_mixed(hipStream_t s, const Plan& pl, int64_t n, const float4* A, const float4* B, double* out);
+
+// Sum `slices` partial blocks of `comps` components into dst (comps*n).
+void reduce_slices(hipStream_t s, int64_t n, int comps, int slices, const double* part, double* dst);
+
+// predict_step_ (nbody.cu:122-152); with A/B non-null also repacks the
+// mixed-kernel source records from the new positions.
+template <class real>
+void predict(hipStream_t s, const PState<real>& p, real dt, float4* A, float4* B);
+// correct_step_ (nbody.cu:159-181) on a[] (after summing `slices` partials
+// from `part` into a[] when part != nullptr).
+void correct_reduce(hipStream_t s, const PState<double>& p, double dt, int slices, const double* part);
+template <class real> void correct(hipStream_t s, const PState<real>& p, real dt);
+
+// Conservation sums (nbody.cu:246-291) as FP64 per-block partials:
+// partial[b*8 + q], q = {Ek*2, Ep*2, Px, Py, Pz, Lx, Ly, Lz}.  Returns the
+// number of blocks written (at most conservation_blocks(n)).
+int64_t conservation_blocks(int64_t n);
+template <class real> int64_t conservation(hipStream_t s, const PState<real>& p, double* partial);
+
+}  // namespace nbody
+
+// ------------------------------------------------------------ particle-mesh
+namespace pm {
+
+// Grid geometry in `real`, derived like GridInfo (io_json.cpp:28-40).
+template <class real> struct Geo {
+    int nx;
+    real dx;
+    real domain_min;
+    real inv_dx;  // 1.f / dx  (rm_p2mesh.cu:185)
+    real upper;   // (nx - 1) - 1e-6 (rm_p2mesh.cu:196-199)
+};
+template <class real> Geo<real> make_geo(int nx, double dx, double bc_frac);
+
+// Wave-iteration coefficients (wave.cu:164-171), computed on the host in `real`.
+template <class real> struct WaveK {
+    real k_main, k_f, k_actual, k_old;
+};
+template <class real> WaveK<real> wave_coef(real diss, real c, real dt, real dx);
+
+// predict_step_ fused with deposit_cell_mass_CIC_ at the new position into an
+// int64 fixed-point grid (scale = 2^e, exact).  When `predict` is false only
+// the deposit runs (precalc).
+template <class real>
+void predict_deposit(hipStream_t s, const PState<real>& p, real dt, bool predict, const Geo<real>& g,
+                     double scale, unsigned long long* mfix);
+
+// Fixed-point grid -> cell mass (real) and stencil source term
+// K_F * 4 pi G (m/dx^3 - rho_mean); clears the fixed-point grid for the next deposit.
+template <class real>
+void fixed_to_source(hipStream_t s, int nx, unsigned long long* mfix, double inv_scale, real dx,
+                     real rho_mean, bool periodic, real k_f, real* mass, real* src);
+
+// Cell mass (real) -> stencil source term, for callers that hold a mass grid.
+template <class real>
+void mass_to_source(hipStream_t s, int nx, const real* mass, real dx, real rho_mean, bool periodic, real k_f,
+                    real* src);
+
+// One wave_diss_iteration over the grid.  When `maxbits` is non-null the
+// kernel also accumulates max|new - cur| and max|new| as bit patterns into
+// maxbits[0], maxbits[1] (which must hold zero beforehand).
+template <class real>
+void wave_iteration(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* pn,
+                    const real* p, const real* po, const real* src, void* maxbits);
+
+// calc_acceleration_field_ into a padded (x, y, z, 0) real4 grid.
+template <class real> void gradient(hipStream_t s, int nx, real dx, const real* phi, real* acc4);
+
+// interpolate_acceleration_to_particles_kernel into p.a, followed by the
+// correct_step_ kick when `kick` is true.
+template <class real>
+void gather(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real* acc4, real dt, bool kick);
+
+// NGP phi projection onto p.g (rm_p2mesh.cu:34-53, rm_mesh2p.cu:29-38).
+template <class real> void ngp_phi(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real* phi);
+
+// Standalone max|a-b|, max|a| reduction (bit patterns, see wave_iteration).
+template <class real> void max_diff(hipStream_t s, int64_t ncell, const real* a, const real* b, void* maxbits);
+
+// Fill kernel (used instead of hipMemsetAsync so the work stays capturable).
+void fill_u64(hipStream_t s, unsigned long long* p, size_t count, unsigned long long value);
+
+}  // namespace pm
+
+}  // namespace rrg

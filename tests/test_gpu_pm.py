@@ -1,0 +1,140 @@
+"""PM ("wave") path on the GPU vs the CPU oracle (through the C ABI).
+
+The PM kernels keep the reference's operation order and promotions and are
+built without FP contraction, and the deposit is int64 fixed point, so the
+bar here is bitwise equality with the oracle for both FP32 and FP64 state.
+Oracle: oracle/rrg_oracle.cpp restating rm_p2mesh.cu:176-290,
+rm_mesh2p.cu:29-114, wave.cu:106-297, wave_steady_checker.cu:95-143.
+"""
+import numpy as np
+import pytest
+
+from rrgalaxsim_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = [np.float64, np.float32]
+C = 4574.337022617616
+
+
+def _pts(n, half, seed, dtype, outside=0.1):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-half * (1 + outside), half * (1 + outside), (n, 3))
+    return p.astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("n,nx", [(1, 8), (1000, 17), (50_000, 32)])
+def test_cic_bitwise(rrg, orc, dtype, n, nx):
+    dx = 0.25
+    pos = _pts(n, nx * dx / 2, n + nx, dtype)
+    m = np.random.default_rng(3).uniform(0.5, 1.5, n).astype(dtype) / n
+    e = 60
+    ref = orc.cic_fixed(pos, m, nx, dx, e, dtype=dtype)
+    got = rrg.cic_deposit(pos, m, nx, dx, e, dtype=dtype)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_cic_order_independent(rrg, dtype):
+    pos, _, m = ics.exp_disk(100_000, seed=1)
+    pos = pos.astype(dtype); m = m.astype(dtype)
+    g1 = rrg.cic_deposit(pos, m, 64, 0.25, 60, dtype=dtype)
+    perm = np.random.default_rng(0).permutation(len(m))
+    g2 = rrg.cic_deposit(pos[perm], m[perm], 64, 0.25, 60, dtype=dtype)
+    assert np.array_equal(g1, g2)
+    # total: the sum of the per-contribution roundings is exact in int64
+    assert abs(int(g1.sum()) / 2.0 ** 60 - float(np.sum(m.astype(np.float64)))) < 1e-9
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("boundary", ["periodic", "zero_dirichlet"])
+@pytest.mark.parametrize("nx", [8, 17, 64])
+def test_wave_iterations_bitwise(rrg, orc, dtype, boundary, nx):
+    rng = np.random.default_rng(nx)
+    dx = 0.25
+    ncell = nx ** 3
+    prev = rng.standard_normal(ncell).astype(dtype) * 0.1
+    curr = prev + rng.standard_normal(ncell).astype(dtype) * 0.01
+    mass = (rng.uniform(0, 1, ncell) / ncell).astype(dtype)
+    wdt = orc.wave_dt(dx, C, dtype=dtype)
+    rm = orc.rho_mean(1.0, nx, dx, dtype=dtype)
+    periodic = boundary == "periodic"
+    rp, rc, it = orc.solve_wave(nx, dx, 0.785, C, wdt, periodic, rm, prev, curr, mass, 7, 0.0, dtype=dtype)
+    assert it == 7
+    gp, gc = rrg.wave_iterate(nx, dx, 0.785, C, wdt, rm, 7, prev, curr, mass, boundary=boundary, dtype=dtype)
+    assert np.array_equal(gp, rp) and np.array_equal(gc, rc)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("nx", [2, 9, 32])
+def test_gradient_bitwise(rrg, orc, dtype, nx):
+    phi = np.random.default_rng(nx).standard_normal(nx ** 3).astype(dtype)
+    assert np.array_equal(rrg.gradient(nx, 0.3, phi, dtype=dtype), orc.gradient(nx, 0.3, phi, dtype=dtype))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("n,nx", [(1, 4), (3333, 16), (100_000, 64)])
+def test_gather_bitwise(rrg, orc, dtype, n, nx):
+    dx = 0.25
+    pos = _pts(n, nx * dx / 2, n, dtype, outside=0.2)
+    field = np.random.default_rng(1).standard_normal((nx ** 3, 3)).astype(dtype)
+    got = rrg.gather(pos, nx, dx, field, dtype=dtype)
+    ref = orc.gather(pos, nx, dx, field, dtype=dtype)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_steady_err(rrg, orc, dtype):
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(10_000).astype(dtype)
+    b = a + rng.standard_normal(10_000).astype(dtype) * 1e-3
+    assert rrg.steady_err(a, b, dtype=dtype) == orc.steady_err(a, b, dtype=dtype)
+
+
+def _pm(rrg, orc, dtype, *, n=20_000, nx=32, dx=0.5, steps=5, setup=40, iters=20, tol=0.0, check=30, hold=3,
+        boundary="periodic"):
+    pos, vel, m = ics.exp_disk(n, seed=1)
+    wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=setup, iterations=iters,
+                tolerance=tol, steady_check_step=check, steady_hold_steps=hold)
+    real = "fp32" if dtype == np.float32 else "fp64"
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=dx, bc_frac=0.0), wave=wave, real=real, boundary=boundary)
+    with rrg.Simulation(p, pos, vel, m) as sim:
+        sim.precalc()
+        sim.sync_particles_and_grid()
+        st0 = sim.state()
+        sim.calc_step(steps)
+        sim.sync_particles_and_grid()
+        st = sim.state()
+        gr = sim.grid()
+        stats = sim.stats()
+    o = orc.pm_run(pos, vel, m, steps, nx=nx, dx=dx, dt=5e-3, diss_base=0.785, wave_speed=C, setup_iterations=setup,
+                   iterations=iters, tolerance=tol, check_step=check, hold=hold, periodic=boundary == "periodic",
+                   fixed_e=stats["fixed_exponent"], dtype=dtype)
+    return st0, st, gr, stats, o
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_pm_run_bitwise(rrg, orc, dtype):
+    _, st, gr, stats, o = _pm(rrg, orc, dtype)
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(st[k].astype(dtype), o[k]), k
+    assert np.array_equal(gr["phi"].astype(dtype), o["phi"])
+    assert np.array_equal(gr["acc"].astype(dtype), o["grid_acc"])
+    assert np.array_equal(st["grav"].astype(dtype), o["grav"])
+    assert stats["last_wave_iterations"] == 20
+
+
+def test_pm_steady_check_semantics(rrg, orc):
+    """tolerance > 0: the early exit happens at the same iteration as the
+    oracle's SteadyStateChecker and the fields stay bitwise equal."""
+    _, st, gr, stats, o = _pm(rrg, orc, np.float64, setup=3000, iters=400, tol=1e-3, check=10, hold=2, steps=3)
+    assert stats["last_wave_iterations"] == int(o["iterations"][-1])
+    assert o["iterations"][0] < 3000  # the setup solve converged early
+    assert np.array_equal(st["pos"], o["pos"]) and np.array_equal(gr["phi"], o["phi"])
+
+
+def test_pm_zero_dirichlet_bitwise(rrg, orc):
+    _, st, gr, _, o = _pm(rrg, orc, np.float32, boundary="zero_dirichlet", steps=2)
+    assert np.array_equal(st["vel"].astype(np.float32), o["vel"])
+    assert np.array_equal(gr["phi"].astype(np.float32), o["phi"])
