@@ -138,14 +138,16 @@ enum {
     RRG_STAGE_DIRECT = 5,   /* direct-sum force launches (N-body) */
     RRG_STAGE_PREDICT = 6,  /* predictor (N-body) */
     RRG_STAGE_CORRECT = 7,  /* corrector (N-body) */
-    RRG_NSTAGES = 8
+    RRG_STAGE_SORT = 8,     /* spatial re-sort of the particles (PM) */
+    RRG_NSTAGES = 9
 };
 
 /* GPU time per stage from HIP events recorded on the context stream around
- * each stage's launches (ms), and the number of kernel launches timed. */
+ * each stage's launches (ms), and the number of kernel launches timed
+ * (entries past RRG_NSTAGES are zero). */
 typedef struct {
-    double ms[8];
-    int64_t launches[8];
+    double ms[12];
+    int64_t launches[12];
 } rrg_timing;
 
 typedef struct rrg_ctx rrg_ctx;

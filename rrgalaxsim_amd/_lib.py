@@ -70,10 +70,10 @@ class Stats(ctypes.Structure):
 
 
 class Timing(ctypes.Structure):
-    _fields_ = [("ms", _f64 * 8), ("launches", _i64 * 8)]
+    _fields_ = [("ms", _f64 * 12), ("launches", _i64 * 12)]
 
 
-STAGES = ["deposit", "source", "wave", "gradient", "gather", "direct", "predict", "correct"]
+STAGES = ["deposit", "source", "wave", "gradient", "gather", "direct", "predict", "correct", "sort"]
 
 
 # Every symbol the header declares, with its ctypes signature.

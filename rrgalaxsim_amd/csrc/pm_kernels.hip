@@ -36,31 +36,12 @@ template <class real> __device__ __forceinline__ real clampr(real v, real lo, re
     return v;
 }
 
-__device__ __forceinline__ void atomic_add_fixed(unsigned long long* cell, long long q) {
-    if (q != 0) atomicAdd(cell, (unsigned long long)q);
-}
-
 // ------------------------------------------------------------------ deposit
+// CIC stencil of deposit_cell_mass_CIC_ at position r: the 8 cells and the 8
+// fixed-point contributions rint(m*w * 2^e).  Returns the base cell index.
 template <class real>
-__global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt, int do_predict, Geo<real> g,
-                                                         double scale, unsigned long long* __restrict__ mfix) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    const int64_t n = p.n;
-    real r[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        r[c] = p.x[c * n + i];
-        if (do_predict) {  // predict_step_ (nbody.cu:135-151)
-            const real v = p.v[c * n + i];
-            const real a = p.a[c * n + i];
-            const real vp = v + dt * a;
-            p.w[c * n + i] = vp;
-            r[c] = (real)((double)r[c] + ((double)dt * 0.5) * (double)(v + vp));
-            p.x[c * n + i] = r[c];
-        }
-    }
-    const real m = p.m[i];
+__device__ __forceinline__ unsigned cic_fixed(const real r[3], real m, const Geo<real>& g, double scale, long long q[8],
+                                              unsigned c[8]) {
     const int NX = g.nx;
     real u = (r[0] - g.domain_min) * g.inv_dx - (real)0.5f;
     real v = (r[1] - g.domain_min) * g.inv_dx - (real)0.5f;
@@ -78,24 +59,115 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
     const real wx0 = (real)1.f - fx, wx1 = fx;
     const real wy0 = (real)1.f - fy, wy1 = fy;
     const real wz0 = (real)1.f - fz, wz1 = fz;
-    const size_t sy = size_t(NX), sz = size_t(NX) * NX;
-    const size_t c000 = size_t(i0) + j0 * sy + k0 * sz;
-    const size_t c100 = size_t(i1) + j0 * sy + k0 * sz;
-    const size_t c010 = size_t(i0) + j1 * sy + k0 * sz;
-    const size_t c110 = size_t(i1) + j1 * sy + k0 * sz;
-    const size_t c001 = size_t(i0) + j0 * sy + k1 * sz;
-    const size_t c101 = size_t(i1) + j0 * sy + k1 * sz;
-    const size_t c011 = size_t(i0) + j1 * sy + k1 * sz;
-    const size_t c111 = size_t(i1) + j1 * sy + k1 * sz;
-    auto q = [&](real wt) { return (long long)rint((double)(m * wt) * scale); };
-    atomic_add_fixed(mfix + c000, q(wx0 * wy0 * wz0));
-    atomic_add_fixed(mfix + c100, q(wx1 * wy0 * wz0));
-    atomic_add_fixed(mfix + c010, q(wx0 * wy1 * wz0));
-    atomic_add_fixed(mfix + c110, q(wx1 * wy1 * wz0));
-    atomic_add_fixed(mfix + c001, q(wx0 * wy0 * wz1));
-    atomic_add_fixed(mfix + c101, q(wx1 * wy0 * wz1));
-    atomic_add_fixed(mfix + c011, q(wx0 * wy1 * wz1));
-    atomic_add_fixed(mfix + c111, q(wx1 * wy1 * wz1));
+    const unsigned sy = unsigned(NX), sz = unsigned(NX) * unsigned(NX);
+    c[0] = i0 + j0 * sy + k0 * sz;
+    c[1] = i1 + j0 * sy + k0 * sz;
+    c[2] = i0 + j1 * sy + k0 * sz;
+    c[3] = i1 + j1 * sy + k0 * sz;
+    c[4] = i0 + j0 * sy + k1 * sz;
+    c[5] = i1 + j0 * sy + k1 * sz;
+    c[6] = i0 + j1 * sy + k1 * sz;
+    c[7] = i1 + j1 * sy + k1 * sz;
+    const real wt[8] = {wx0 * wy0 * wz0, wx1 * wy0 * wz0, wx0 * wy1 * wz0, wx1 * wy1 * wz0,
+                        wx0 * wy0 * wz1, wx1 * wy0 * wz1, wx0 * wy1 * wz1, wx1 * wy1 * wz1};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) q[k] = (long long)rint((double)(m * wt[k]) * scale);
+    return c[0];
+}
+
+// Predictor fused with the deposit.  Lanes of a wave whose particles share a
+// base cell (runs of equal keys; the particle arrays are kept sorted by cell,
+// so dense regions form long runs) are merged by a segmented inclusive scan
+// over the wave before one lane of each run issues the 8 atomics.  Integer
+// addition is exact, so the grid does not depend on the merging.
+constexpr unsigned kNoCell = 0xffffffffu;
+
+template <class real>
+__global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt, int do_predict, Geo<real> g,
+                                                         double scale, unsigned long long* __restrict__ mfix) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t n = p.n;
+    unsigned key = kNoCell;
+    long long q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        real r[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            r[d] = p.x[d * n + i];
+            if (do_predict) {  // predict_step_ (nbody.cu:135-151)
+                const real v = p.v[d * n + i];
+                const real a = p.a[d * n + i];
+                const real vp = v + dt * a;
+                p.w[d * n + i] = vp;
+                r[d] = (real)((double)r[d] + ((double)dt * 0.5) * (double)(v + vp));
+                p.x[d * n + i] = r[d];
+            }
+        }
+        key = cic_fixed<real>(r, p.m[i], g, scale, q, c);
+    }
+    const unsigned kprev = __shfl_up(key, 1, 64);
+    const unsigned knext = __shfl_down(key, 1, 64);
+    int f = (lane == 0 || kprev != key) ? 1 : 0;  // run head
+    const bool tail = (lane == 63 || knext != key);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int fu = __shfl_up(f, off, 64);
+        long long qu[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) qu[k] = __shfl_up(q[k], off, 64);
+        if (lane >= off) {
+            if (!f) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) q[k] += qu[k];
+            }
+            f |= fu;
+        }
+    }
+    if (tail && key != kNoCell) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (q[k] != 0) atomicAdd(mfix + c[k], (unsigned long long)q[k]);
+    }
+}
+
+// Sort keys for the spatial ordering of particles: the CIC base cell.
+template <class real>
+__global__ __launch_bounds__(256) void k_cell_keys(PState<real> p, Geo<real> g, unsigned* __restrict__ keys,
+                                                   unsigned* __restrict__ idx) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const int64_t n = p.n;
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        real u = (p.x[d * n + i] - g.domain_min) * g.inv_dx - (real)0.5f;
+        u = clampr<real>(u, 0, g.upper);
+        b[d] = max(0, min((int)floor(u), g.nx - 1));
+    }
+    keys[i] = unsigned(b[0]) + unsigned(g.nx) * (unsigned(b[1]) + unsigned(g.nx) * unsigned(b[2]));
+    idx[i] = unsigned(i);
+}
+
+// Gather the particle state through a permutation: dst[i] = src[idx[i]].
+template <class real>
+__global__ __launch_bounds__(256) void k_permute(PState<real> src, PState<real> dst, const unsigned* __restrict__ idx,
+                                                 const int* __restrict__ id_src, int* __restrict__ id_dst) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= src.n) return;
+    const int64_t n = src.n;
+    const int64_t j = idx[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        dst.x[d * n + i] = src.x[d * n + j];
+        dst.v[d * n + i] = src.v[d * n + j];
+        dst.a[d * n + i] = src.a[d * n + j];
+        dst.w[d * n + i] = src.w[d * n + j];
+    }
+    dst.m[i] = src.m[j];
+    dst.g[i] = src.g[j];
+    id_dst[i] = id_src[j];
 }
 
 // Fixed point -> cell mass and source term.  rho = m / (DX*DX*DX) - rho_mean
@@ -162,6 +234,10 @@ __device__ __forceinline__ void xcd_block(int nbx, int nby, int nbz, int& bx, in
     bz = r / (nbx * nby);
 }
 
+// Each thread owns one (x, y) column and marches ZB planes in z, keeping
+// phi at z-1, z, z+1 in registers; x/y neighbours come from the cache.
+constexpr int kWaveZB = 8;
+
 template <class real, bool PERIODIC, bool CHECK>
 __global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real* __restrict__ p,
                                               const real* __restrict__ po, const real* __restrict__ src, int NX,
@@ -170,31 +246,41 @@ __global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real*
     xcd_block(gridDim.x, gridDim.y, gridDim.z, bx, by, bz);
     const int ix = bx * 64 + threadIdx.x;
     const int iy = by * 4 + threadIdx.y;
-    const int iz = bz;
     real dmax = 0, amax = 0;
     if (ix < NX && iy < NX) {
         const size_t sy = size_t(NX), sz = size_t(NX) * NX;
-        const size_t c = size_t(ix) + iy * sy + iz * sz;
-        real out;
-        const bool on_edge = ix == 0 || ix == NX - 1 || iy == 0 || iy == NX - 1 || iz == 0 || iz == NX - 1;
-        if (!PERIODIC && on_edge) {
-            out = 0;  // ZeroDirichlet (wave.cu:140-145)
-        } else {
-            int xm = ix - 1, xp = ix + 1, ym = iy - 1, yp = iy + 1, zm = iz - 1, zp = iz + 1;
-            if (PERIODIC) {
-                xm = xm < 0 ? xm + NX : xm;  xp = xp >= NX ? xp - NX : xp;
-                ym = ym < 0 ? ym + NX : ym;  yp = yp >= NX ? yp - NX : yp;
-                zm = zm < 0 ? zm + NX : zm;  zp = zp >= NX ? zp - NX : zp;
+        // neighbour indices: wrapped (periodic) or clamped (edge cells are not
+        // computed from them, the clamp only keeps the loads in bounds)
+        auto nb = [&](int i) { return PERIODIC ? (i < 0 ? i + NX : (i >= NX ? i - NX : i)) : max(0, min(i, NX - 1)); };
+        const int xm = nb(ix - 1), xp = nb(ix + 1), ym = nb(iy - 1), yp = nb(iy + 1);
+        const bool edge_xy = ix == 0 || ix == NX - 1 || iy == 0 || iy == NX - 1;
+        const size_t col = size_t(ix) + iy * sy;
+        const int z0 = bz * kWaveZB;
+        real below = p[col + nb(z0 - 1) * sz];
+        real cen = p[col + size_t(z0) * sz];
+#pragma unroll 2
+        for (int t = 0; t < kWaveZB; ++t) {
+            const int iz = z0 + t;
+            if (iz >= NX) break;
+            const real above = p[col + nb(iz + 1) * sz];
+            const size_t pl = size_t(iz) * sz;
+            const size_t c = col + pl;
+            real out;
+            if (!PERIODIC && (edge_xy || iz == 0 || iz == NX - 1)) {
+                out = 0;  // ZeroDirichlet (wave.cu:140-145)
+            } else {
+                // phi[xm] + phi[xp] + phi[ym] + phi[yp] + phi[zm] + phi[zp] (wave.cu:183-186)
+                const real lap_sum = p[xm + iy * sy + pl] + p[xp + iy * sy + pl] + p[ix + ym * sy + pl] +
+                                     p[ix + yp * sy + pl] + below + above;
+                out = lap_sum * k.k_main + cen * k.k_actual + po[c] * k.k_old + src[c];
             }
-            const size_t row = iy * sy + iz * sz;
-            const real lap_sum = p[size_t(xm) + row] + p[size_t(xp) + row] + p[ix + ym * sy + iz * sz] +
-                                 p[ix + yp * sy + iz * sz] + p[ix + iy * sy + zm * sz] + p[ix + iy * sy + zp * sz];
-            out = lap_sum * k.k_main + p[c] * k.k_actual + po[c] * k.k_old + src[c];
-        }
-        pn[c] = out;
-        if (CHECK) {
-            dmax = fabs(out - p[c]);
-            amax = fabs(out);
+            pn[c] = out;
+            if (CHECK) {
+                dmax = fmax(dmax, fabs(out - cen));
+                amax = fmax(amax, fabs(out));
+            }
+            below = cen;
+            cen = above;
         }
     }
     if (CHECK) {
@@ -369,6 +455,21 @@ void predict_deposit(hipStream_t s, const PState<real>& p, real dt, bool predict
 }
 
 template <class real>
+void cell_keys(hipStream_t s, const PState<real>& p, const Geo<real>& g, unsigned* keys, unsigned* idx) {
+    if (p.n == 0) return;
+    k_cell_keys<real><<<grid1d(p.n, 256), 256, 0, s>>>(p, g, keys, idx);
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real>
+void permute(hipStream_t s, const PState<real>& src, const PState<real>& dst, const unsigned* idx, const int* id_src,
+             int* id_dst) {
+    if (src.n == 0) return;
+    k_permute<real><<<grid1d(src.n, 256), 256, 0, s>>>(src, dst, idx, id_src, id_dst);
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real>
 void fixed_to_source(hipStream_t s, int nx, unsigned long long* mfix, double inv_scale, real dx, real rho_mean,
                      bool periodic, real k_f, real* mass, real* src) {
     const int64_t ncell = int64_t(nx) * nx * nx;
@@ -392,7 +493,8 @@ void wave_iteration(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, 
                     const real* po, const real* src, void* maxbits) {
     using MB = typename MaxBits<real>::T;
     MB* mb = reinterpret_cast<MB*>(maxbits);
-    const dim3 grid = grid3d(nx), block(64, 4, 1);
+    const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned((nx + kWaveZB - 1) / kWaveZB));
+    const dim3 block(64, 4, 1);
     if (periodic) {
         if (mb) k_wave<real, true, true><<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb);
         else k_wave<real, true, false><<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb);
@@ -446,6 +548,9 @@ void fill_u64(hipStream_t s, unsigned long long* p, size_t count, unsigned long 
                                         unsigned long long*);                                                   \
     template void fixed_to_source<real>(hipStream_t, int, unsigned long long*, double, real, real, bool, real,  \
                                         real*, real*);                                                          \
+    template void cell_keys<real>(hipStream_t, const PState<real>&, const Geo<real>&, unsigned*, unsigned*);     \
+    template void permute<real>(hipStream_t, const PState<real>&, const PState<real>&, const unsigned*,          \
+                                const int*, int*);                                                              \
     template void mass_to_source<real>(hipStream_t, int, const real*, real, real, bool, real, real*);           \
     template void wave_iteration<real>(hipStream_t, int, bool, const WaveK<real>&, real*, const real*,          \
                                        const real*, const real*, void*);                                        \

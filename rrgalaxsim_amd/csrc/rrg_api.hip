@@ -88,7 +88,21 @@ template <class real> struct Particles {
     PState<real> view() const {
         return PState<real>{X.p, V.p, A.p, W.p, M.p, S2.p, G.p, n};
     }
+    void swap(Particles& o) {
+        std::swap(X, o.X); std::swap(V, o.V); std::swap(A, o.A); std::swap(W, o.W);
+        std::swap(M, o.M); std::swap(S2, o.S2); std::swap(G, o.G);
+    }
 };
+
+// out[id[i]] = in[i] for AoS triples (id: internal slot -> caller's index).
+void unpermute_vec(int64_t n, const std::vector<int>& id, const double* in, double* out) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) out[3 * int64_t(id[i]) + c] = in[3 * i + c];
+}
+void permute_vec_in(int64_t n, const std::vector<int>& id, const double* in, double* out) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) out[3 * i + c] = in[3 * int64_t(id[i]) + c];
+}
 
 template <class real> void aos_to_soa(int64_t n, const double* aos, std::vector<real>& soa) {
     soa.resize(3 * n);
@@ -169,6 +183,7 @@ public:
             pool_.push_back(r.b);
         }
         pending_.clear();
+        std::memset(out, 0, sizeof(*out));
         for (int k = 0; k < RRG_NSTAGES; ++k) {
             out->ms[k] = ms_[k];
             out->launches[k] = launches_[k];
@@ -416,6 +431,21 @@ public:
         upload_vec<real>(s_, n, pos, P_.X.p);
         upload_vec<real>(s_, n, vel, P_.V.p);
         upload_scalar<real>(s_, n, mass, P_.M.p);
+        // spatial ordering buffers (particles are kept sorted by CIC cell)
+        P2_.alloc(n);
+        id_.alloc(n);
+        id2_.alloc(n);
+        keys_.alloc(n); keys2_.alloc(n); idx_.alloc(n); idx2_.alloc(n);
+        {
+            std::vector<int> ident(n);
+            for (int64_t i = 0; i < n; ++i) ident[i] = int(i);
+            RRG_HIP(hipMemcpyAsync(id_.p, ident.data(), sizeof(int) * n, hipMemcpyHostToDevice, s_));
+            RRG_HIP(hipStreamSynchronize(s_));
+        }
+        end_bit_ = 1;
+        while ((int64_t(1) << end_bit_) < ncell_) ++end_bit_;
+        pm::radix_sort_pairs(s_, nullptr, sort_bytes_, keys_.p, keys2_.p, idx_.p, idx2_.p, n, end_bit_);
+        sort_temp_.alloc(std::max<size_t>(sort_bytes_, 1));
         zero_async(s_, P_.S2.p, P_.S2.bytes());
         zero_async(s_, P_.A.p, P_.A.bytes());
         zero_async(s_, P_.W.p, P_.W.bytes());
@@ -435,6 +465,7 @@ public:
     }
 
     void precalc() override {  // rrgsim_integration_rect.cuh:53-83
+        sort_particles();
         deposit(false);
         last_iters_ = solve(prm_.wave.setup_iterations);
         pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
@@ -447,6 +478,7 @@ public:
         if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_step before rrg_precalc");
         const real dt = (real)prm_.sim.dt_dynamics;
         for (int k = 0; k < nsteps; ++k) {
+            if (++since_sort_ >= kSortInterval) sort_particles();
             deposit(true);
             int t = timer.start(s_, RRG_STAGE_WAVE);
             last_iters_ = solve(prm_.wave.iterations);
@@ -472,17 +504,27 @@ public:
         grav_actual_ = acc_actual_ = true;
     }
 
+    // The device arrays hold the particles in cell order; id_ maps each slot
+    // back to the caller's index.
     void get_state(double* pos, double* vel, double* acc, double* gr) override {
         const int64_t n = P_.n;
-        if (pos) download_vec<real>(s_, n, P_.X.p, pos);
-        if (vel) download_vec<real>(s_, n, P_.V.p, vel);
-        if (acc) download_vec<real>(s_, n, P_.A.p, acc);
-        if (gr) download_scalar<real>(s_, n, P_.G.p, gr);
+        const std::vector<int> id = ids();
+        std::vector<double> tmp(3 * n);
+        if (pos) { download_vec<real>(s_, n, P_.X.p, tmp.data()); unpermute_vec(n, id, tmp.data(), pos); }
+        if (vel) { download_vec<real>(s_, n, P_.V.p, tmp.data()); unpermute_vec(n, id, tmp.data(), vel); }
+        if (acc) { download_vec<real>(s_, n, P_.A.p, tmp.data()); unpermute_vec(n, id, tmp.data(), acc); }
+        if (gr) {
+            download_scalar<real>(s_, n, P_.G.p, tmp.data());
+            for (int64_t i = 0; i < n; ++i) gr[id[i]] = tmp[i];
+        }
     }
 
     void set_state(const double* pos, const double* vel) override {
-        if (pos) upload_vec<real>(s_, P_.n, pos, P_.X.p);
-        if (vel) upload_vec<real>(s_, P_.n, vel, P_.V.p);
+        const int64_t n = P_.n;
+        const std::vector<int> id = ids();
+        std::vector<double> tmp(3 * n);
+        if (pos) { permute_vec_in(n, id, pos, tmp.data()); upload_vec<real>(s_, n, tmp.data(), P_.X.p); }
+        if (vel) { permute_vec_in(n, id, vel, tmp.data()); upload_vec<real>(s_, n, tmp.data(), P_.V.p); }
         grav_actual_ = acc_actual_ = false;
     }
 
@@ -513,7 +555,30 @@ public:
     }
 
 private:
+    static constexpr int kSortInterval = 8;  // steps between spatial re-sorts
+
     real* cur() { return phi_[ic_].p; }
+
+    std::vector<int> ids() {
+        std::vector<int> id(P_.n);
+        RRG_HIP(hipMemcpyAsync(id.data(), id_.p, sizeof(int) * P_.n, hipMemcpyDeviceToHost, s_));
+        RRG_HIP(hipStreamSynchronize(s_));
+        return id;
+    }
+
+    // Reorder the particle arrays by CIC base cell (radix sort + gather), so
+    // the deposit's wave-level run merging sees long runs of equal cells.
+    void sort_particles() {
+        int t = timer.start(s_, RRG_STAGE_SORT);
+        pm::cell_keys<real>(s_, P_.view(), geo_, keys_.p, idx_.p);
+        size_t bytes = sort_bytes_;
+        pm::radix_sort_pairs(s_, sort_temp_.p, bytes, keys_.p, keys2_.p, idx_.p, idx2_.p, P_.n, end_bit_);
+        pm::permute<real>(s_, P_.view(), P2_.view(), idx2_.p, id_.p, id2_.p);
+        P_.swap(P2_);
+        std::swap(id_, id2_);
+        timer.stop(s_, t, 3);
+        since_sort_ = 0;
+    }
 
     void deposit(bool predict) {
         int t = timer.start(s_, RRG_STAGE_DEPOSIT);
@@ -574,7 +639,13 @@ private:
     real rho_mean_ = 0, tol_ = 0;
     int e_ = 61;
     double scale_ = 0, inv_scale_ = 0;
-    Particles<real> P_;
+    Particles<real> P_, P2_;
+    DBuf<int> id_, id2_;
+    DBuf<unsigned> keys_, keys2_, idx_, idx2_;
+    DBuf<char> sort_temp_;
+    size_t sort_bytes_ = 0;
+    int end_bit_ = 32;
+    int since_sort_ = 0;
     DBuf<unsigned long long> mfix_;
     DBuf<real> mass_, src_, phi_[3], gacc_;
     DBuf<unsigned long long> maxbits_;

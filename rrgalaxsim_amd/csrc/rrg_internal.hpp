@@ -138,6 +138,17 @@ template <class real>
 void predict_deposit(hipStream_t s, const PState<real>& p, real dt, bool predict, const Geo<real>& g,
                      double scale, unsigned long long* mfix);
 
+// Spatial ordering: keys[i] = CIC base cell of particle i, idx[i] = i.
+template <class real>
+void cell_keys(hipStream_t s, const PState<real>& p, const Geo<real>& g, unsigned* keys, unsigned* idx);
+// dst[i] = src[idx[i]] for x, v, a, v*, m, phi_p and the particle ids.
+template <class real>
+void permute(hipStream_t s, const PState<real>& src, const PState<real>& dst, const unsigned* idx, const int* id_src,
+             int* id_dst);
+// Key/value radix sort (hipCUB); with temp == nullptr only sets temp_bytes.
+void radix_sort_pairs(hipStream_t s, void* temp, size_t& temp_bytes, const unsigned* keys_in, unsigned* keys_out,
+                      const unsigned* vals_in, unsigned* vals_out, int64_t n, int end_bit);
+
 // Fixed-point grid -> cell mass (real) and stencil source term
 // K_F * 4 pi G (m/dx^3 - rho_mean); clears the fixed-point grid for the next deposit.
 template <class real>
