@@ -95,7 +95,7 @@ def direct_bench(rrg, ics, steps=3):
     n = len(m)
     s2 = np.full(n, 0.005 ** 2)
     out = {}
-    for force, k in (("mixed", steps), ("fp64", 1)):
+    for force, k in (("mixed", steps), ("fp32", steps), ("fp64", 1), ("fp64_exact", 1)):
         p = rrg.make_params(dt=1e-3, force=force)
         sim = rrg.Simulation(p, pos, vel, m, s2)
         sim.precalc()
@@ -116,9 +116,15 @@ def direct_bench(rrg, ics, steps=3):
             "kernel_ms": kern_s * 1e3,
             "tflops_20_per_interaction": 20 * inter / kern_s / 1e12,
         }
-    out["mixed_over_fp64"] = out["mixed"]["kernel_ginteractions_per_s"] / out["fp64"]["kernel_ginteractions_per_s"]
+    for f32 in ("mixed", "fp32"):
+        for f64 in ("fp64", "fp64_exact"):
+            out[f"{f32}_over_{f64}"] = (out[f32]["kernel_ginteractions_per_s"] /
+                                        out[f64]["kernel_ginteractions_per_s"])
     out["config"] = {"workload": "disk N=196608 + Plummer bulge N=65536 (M=0.2, a=0.2), eps=0.005, dt=1e-3",
-                     "n": n, "timed_steps": {"mixed": steps, "fp64": 1}}
+                     "n": n, "timed_steps": {"mixed": steps, "fp32": steps, "fp64": 1, "fp64_exact": 1},
+                     "kinds": {"fp64": "FP64 pairs, device-library rsqrt", "fp64_exact": "FP64 pairs, IEEE 1./sqrt "
+                               "(the reference kernel's arithmetic)", "mixed": "FP32 pairs on hi/lo positions + "
+                               "two-sum accumulation", "fp32": "FP32 pairs on FP32 positions + two-sum accumulation"}}
     return out
 
 
@@ -188,8 +194,10 @@ def main():
     }
     if not args.no_direct:
         d = direct_bench(rrg, ics)
-        d["mixed"]["fp32_vector_frac"] = d["mixed"]["tflops_20_per_interaction"] / FP32_PEAK_TFLOPS
-        d["fp64"]["fp64_vector_frac"] = d["fp64"]["tflops_20_per_interaction"] / FP64_PEAK_TFLOPS
+        for k in ("mixed", "fp32"):
+            d[k]["fp32_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP32_PEAK_TFLOPS
+        for k in ("fp64", "fp64_exact"):
+            d[k]["fp64_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP64_PEAK_TFLOPS
         line["direct_sum"] = d
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ics)

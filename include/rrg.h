@@ -45,12 +45,18 @@ extern "C" {
 enum { RRG_REAL_F64 = 0, RRG_REAL_F32 = 1 };
 
 /* Direct-sum force arithmetic (N-body model).
- *   RRG_FORCE_F64   : FP64 pair math, the reference kernel's arithmetic
- *                     (nbody.cu:19-65); the "plain FP64" kernel.
- *   RRG_FORCE_MIXED : FP32 pair math on hi/lo-split positions with
- *                     compensated (two-sum) per-particle accumulation, FP64
- *                     integrator state.  No counterpart in the reference. */
-enum { RRG_FORCE_F64 = 0, RRG_FORCE_MIXED = 1 };
+ *   RRG_FORCE_F64       : FP64 pair math with the device library's FP64
+ *                         rsqrt (default; the fastest FP64 variant).
+ *   RRG_FORCE_MIXED     : FP32 pair math on hi/lo-split positions
+ *                         (d = (hi_j - hi_i) + (lo_j - lo_i)) with compensated
+ *                         (two-sum) per-particle accumulation, FP64 state.
+ *   RRG_FORCE_FP32      : FP32 pair math on FP32-rounded positions with the
+ *                         same compensated accumulation; cheaper, larger
+ *                         error on close pairs.
+ *   RRG_FORCE_F64_EXACT : FP64 pair math with IEEE 1./sqrt, the reference
+ *                         kernel's arithmetic (nbody.cu:50-54).
+ * The FP32 kinds have no counterpart in the reference. */
+enum { RRG_FORCE_F64 = 0, RRG_FORCE_MIXED = 1, RRG_FORCE_FP32 = 2, RRG_FORCE_F64_EXACT = 3 };
 
 /* Wave-equation boundary types (co_boundary.cuh:4-9).  The reference's live
  * configuration is periodic (rrgsim_integration_rect.cuh:65-69). */

@@ -25,6 +25,9 @@ REAL_F64 = 0
 REAL_F32 = 1
 FORCE_F64 = 0
 FORCE_MIXED = 1
+FORCE_FP32 = 2
+FORCE_F64_EXACT = 3
+FORCE_KINDS = {"fp64": 0, "mixed": 1, "fp32": 2, "fp64_exact": 3}
 BC_ZERO_DIRICHLET = 0
 BC_PERIODIC = 1
 

@@ -36,10 +36,13 @@ template <int RSQ> __device__ __forceinline__ double inv_sqrt_f64(double r2) {
     }
 }
 
-template <int IB, int RSQ>
-__global__ __launch_bounds__(TPB) void k_accel_f64(PState<double> p, int64_t slice_len, double* __restrict__ out) {
+// UNI: every particle has the same softening, so h_i + h_j is the constant
+// `hh` and the per-source softening is neither staged nor added.
+template <int IB, int RSQ, bool UNI>
+__global__ __launch_bounds__(TPB) void k_accel_f64(PState<double> p, int64_t slice_len, double hh,
+                                                   double* __restrict__ out) {
     __shared__ double4 sp[TPB];  // (x, y, z, m)
-    __shared__ double sh[TPB];   // 0.5 * soft2
+    __shared__ double sh[UNI ? 1 : TPB];  // 0.5 * soft2
     const int t = threadIdx.x;
     const int64_t n = p.n;
     const int64_t base = int64_t(blockIdx.x) * TPB * IB;
@@ -51,7 +54,7 @@ __global__ __launch_bounds__(TPB) void k_accel_f64(PState<double> p, int64_t sli
         xi[k] = X[i];
         yi[k] = X[n + i];
         zi[k] = X[2 * n + i];
-        hi[k] = 0.5 * p.s2[i];
+        hi[k] = UNI ? hh : 0.5 * p.s2[i];
         ax[k] = ay[k] = az[k] = 0.0;
     }
     const int64_t j0 = int64_t(blockIdx.y) * slice_len;
@@ -62,22 +65,22 @@ __global__ __launch_bounds__(TPB) void k_accel_f64(PState<double> p, int64_t sli
         double h = 1.0;
         if (j < j1) {
             q = make_double4(X[j], X[n + j], X[2 * n + j], p.m[j]);
-            h = 0.5 * p.s2[j];
+            if (!UNI) h = 0.5 * p.s2[j];
         }
         __syncthreads();
         sp[t] = q;
-        sh[t] = h;
+        if (!UNI) sh[t] = h;
         __syncthreads();
 #pragma unroll 4
         for (int jj = 0; jj < TPB; ++jj) {
             const double4 s = sp[jj];
-            const double hj = sh[jj];
+            const double hj = UNI ? 0.0 : sh[jj];
 #pragma unroll
             for (int k = 0; k < IB; ++k) {
                 const double dx = s.x - xi[k];
                 const double dy = s.y - yi[k];
                 const double dz = s.z - zi[k];
-                const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, hi[k] + hj)));
+                const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, UNI ? hi[k] : hi[k] + hj)));
                 const double inv = inv_sqrt_f64<RSQ>(r2);
                 const double c = s.w * (inv * inv * inv);
                 ax[k] = fma(dx, c, ax[k]);
@@ -165,11 +168,15 @@ __device__ __forceinline__ void two_sum_acc(float& s, float& c, float t) {
     c += e;
 }
 
-template <int IB>
+// HILO = false is the plain FP32 variant: d = x_j - x_i on the FP32 values
+// (the lo parts are ignored), cheaper and less exact for close pairs; the
+// accumulation is compensated either way.
+template <int IB, bool HILO, bool UNI>
 __global__ __launch_bounds__(TPB) void k_accel_mixed(const float4* __restrict__ A, const float4* __restrict__ B,
-                                                     int64_t n, int64_t slice_len, double* __restrict__ out) {
+                                                     int64_t n, int64_t slice_len, float hh, double* __restrict__ out) {
+    constexpr bool NEED_B = HILO || !UNI;
     __shared__ float4 sa[TPB];
-    __shared__ float4 sb[TPB];
+    __shared__ float4 sb[NEED_B ? TPB : 1];
     const int t = threadIdx.x;
     const int64_t base = int64_t(blockIdx.x) * TPB * IB;
     float xh[IB], yh[IB], zh[IB], xl[IB], yl[IB], zl[IB], hs[IB];
@@ -179,7 +186,8 @@ __global__ __launch_bounds__(TPB) void k_accel_mixed(const float4* __restrict__ 
         const int64_t i = min(base + k * TPB + t, n - 1);
         const float4 a = A[i], b = B[i];
         xh[k] = a.x; yh[k] = a.y; zh[k] = a.z;
-        xl[k] = b.x; yl[k] = b.y; zl[k] = b.z; hs[k] = b.w;
+        xl[k] = b.x; yl[k] = b.y; zl[k] = b.z;
+        hs[k] = UNI ? hh : b.w;
         sx[k] = sy[k] = sz[k] = cx[k] = cy[k] = cz[k] = 0.f;
     }
     const int64_t j0 = int64_t(blockIdx.y) * slice_len;
@@ -190,11 +198,11 @@ __global__ __launch_bounds__(TPB) void k_accel_mixed(const float4* __restrict__ 
         float4 qb = make_float4(0.f, 0.f, 0.f, 1.f);
         if (j < j1) {
             qa = A[j];
-            qb = B[j];
+            if (NEED_B) qb = B[j];
         }
         __syncthreads();
         sa[t] = qa;
-        sb[t] = qb;
+        if (NEED_B) sb[t] = qb;
         __syncthreads();
         float tx[IB], ty[IB], tz[IB];
 #pragma unroll
@@ -202,13 +210,20 @@ __global__ __launch_bounds__(TPB) void k_accel_mixed(const float4* __restrict__ 
 #pragma unroll 4
         for (int jj = 0; jj < TPB; ++jj) {
             const float4 a = sa[jj];
-            const float4 b = sb[jj];
+            const float4 b = NEED_B ? sb[jj] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int k = 0; k < IB; ++k) {
-                const float dx = (a.x - xh[k]) + (b.x - xl[k]);
-                const float dy = (a.y - yh[k]) + (b.y - yl[k]);
-                const float dz = (a.z - zh[k]) + (b.z - zl[k]);
-                const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, hs[k] + b.w)));
+                float dx, dy, dz;
+                if (HILO) {
+                    dx = (a.x - xh[k]) + (b.x - xl[k]);
+                    dy = (a.y - yh[k]) + (b.y - yl[k]);
+                    dz = (a.z - zh[k]) + (b.z - zl[k]);
+                } else {
+                    dx = a.x - xh[k];
+                    dy = a.y - yh[k];
+                    dz = a.z - zh[k];
+                }
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, UNI ? hs[k] : hs[k] + b.w)));
                 const float inv = __builtin_amdgcn_rsqf(r2);
                 const float c = (a.w * inv) * (inv * inv);
                 tx[k] = fmaf(dx, c, tx[k]);
@@ -404,13 +419,16 @@ inline unsigned grid1d(int64_t n, int tpb) { return unsigned((n + tpb - 1) / tpb
 
 }  // namespace
 
+bool is_fp32_kind(int kind) { return kind == RRG_FORCE_MIXED || kind == RRG_FORCE_FP32; }
+
 Plan plan_direct(int64_t n, int force_kind, int target_workgroups) {
     Plan pl;
-    pl.ib = (force_kind == RRG_FORCE_MIXED) ? 2 : 1;
-    if (n >= 65536) pl.ib = (force_kind == RRG_FORCE_MIXED) ? 4 : 2;
+    const bool f32 = is_fp32_kind(force_kind);
+    pl.ib = f32 ? 2 : 1;
+    if (n >= 65536) pl.ib = f32 ? 4 : 2;
     const int64_t per_tile = int64_t(TPB) * pl.ib;
     pl.tiles = (n + per_tile - 1) / per_tile;
-    const int64_t target = target_workgroups > 0 ? target_workgroups : 2048;
+    const int64_t target = target_workgroups > 0 ? target_workgroups : 4096;
     int64_t slices = (target + pl.tiles - 1) / pl.tiles;
     const int64_t max_slices = std::max<int64_t>(1, n / TPB);
     slices = std::min(std::max<int64_t>(slices, 1), std::min<int64_t>(max_slices, 256));
@@ -427,50 +445,53 @@ void pack_mixed(hipStream_t s, const PState<double>& p, float4* A, float4* B) {
     RRG_LAUNCH_CHECK();
 }
 
-void accel_f64(hipStream_t s, const Plan& pl, const PState<double>& p, double* out, int rsqrt_kind) {
+void accel(hipStream_t s, const Plan& pl, const Force& f, const PState<double>& p, const float4* A, const float4* B,
+           double* out) {
     const dim3 grid(unsigned(pl.tiles), unsigned(pl.slices));
-#define RRG_CASE(IB, RSQ)                                                                      \
-    if (pl.ib == IB && rsqrt_kind == RSQ) {                                                    \
-        k_accel_f64<IB, RSQ><<<grid, TPB, 0, s>>>(p, pl.slice_len, out);                       \
-        RRG_LAUNCH_CHECK();                                                                    \
-        return;                                                                                \
+    const int64_t n = p.n;
+    const int rsq = f.kind == RRG_FORCE_F64_EXACT ? 0 : 1;
+    bool done = false;
+#define RRG_F64(IB, RSQ, UNI)                                                                          \
+    if (!done && !is_fp32_kind(f.kind) && pl.ib == IB && rsq == RSQ && f.uniform == UNI) {             \
+        k_accel_f64<IB, RSQ, UNI><<<grid, TPB, 0, s>>>(p, pl.slice_len, f.hh, out);                   \
+        done = true;                                                                                   \
     }
-    RRG_CASE(1, 0) RRG_CASE(2, 0) RRG_CASE(1, 1) RRG_CASE(2, 1)
-#undef RRG_CASE
-    fail(RRG_ERR_INVALID, "accel_f64: unsupported launch shape");
-}
-
-void grav_f64(hipStream_t s, const Plan& pl, const PState<double>& p, double* out, int rsqrt_kind) {
-    const dim3 grid(unsigned(pl.tiles), unsigned(pl.slices));
-#define RRG_CASE(IB, RSQ)                                                                      \
-    if (pl.ib == IB && rsqrt_kind == RSQ) {                                                    \
-        k_grav_f64<IB, RSQ><<<grid, TPB, 0, s>>>(p, pl.slice_len, out);                        \
-        RRG_LAUNCH_CHECK();                                                                    \
-        return;                                                                                \
+    RRG_F64(1, 0, false) RRG_F64(2, 0, false) RRG_F64(1, 1, false) RRG_F64(2, 1, false)
+    RRG_F64(1, 0, true) RRG_F64(2, 0, true) RRG_F64(1, 1, true) RRG_F64(2, 1, true)
+#undef RRG_F64
+    const bool hilo = f.kind == RRG_FORCE_MIXED;
+#define RRG_F32(IB, HILO, UNI)                                                                         \
+    if (!done && is_fp32_kind(f.kind) && pl.ib == IB && hilo == HILO && f.uniform == UNI) {            \
+        k_accel_mixed<IB, HILO, UNI><<<grid, TPB, 0, s>>>(A, B, n, pl.slice_len, float(f.hh), out);   \
+        done = true;                                                                                   \
     }
-    RRG_CASE(1, 0) RRG_CASE(2, 0) RRG_CASE(1, 1) RRG_CASE(2, 1)
-#undef RRG_CASE
-    fail(RRG_ERR_INVALID, "grav_f64: unsupported launch shape");
-}
-
-void accel_mixed(hipStream_t s, const Plan& pl, int64_t n, const float4* A, const float4* B, double* out) {
-    const dim3 grid(unsigned(pl.tiles), unsigned(pl.slices));
-    switch (pl.ib) {
-        case 1: k_accel_mixed<1><<<grid, TPB, 0, s>>>(A, B, n, pl.slice_len, out); break;
-        case 2: k_accel_mixed<2><<<grid, TPB, 0, s>>>(A, B, n, pl.slice_len, out); break;
-        case 4: k_accel_mixed<4><<<grid, TPB, 0, s>>>(A, B, n, pl.slice_len, out); break;
-        default: fail(RRG_ERR_INVALID, "accel_mixed: unsupported launch shape");
-    }
+    RRG_F32(1, true, false) RRG_F32(2, true, false) RRG_F32(4, true, false)
+    RRG_F32(1, true, true) RRG_F32(2, true, true) RRG_F32(4, true, true)
+    RRG_F32(1, false, false) RRG_F32(2, false, false) RRG_F32(4, false, false)
+    RRG_F32(1, false, true) RRG_F32(2, false, true) RRG_F32(4, false, true)
+#undef RRG_F32
+    if (!done) fail(RRG_ERR_INVALID, "direct-sum acceleration: unsupported launch shape");
     RRG_LAUNCH_CHECK();
 }
 
-void grav_mixed(hipStream_t s, const Plan& pl, int64_t n, const float4* A, const float4* B, double* out) {
+void grav(hipStream_t s, const Plan& pl, const Force& f, const PState<double>& p, const float4* A, const float4* B,
+          double* out) {
     const dim3 grid(unsigned(pl.tiles), unsigned(pl.slices));
-    switch (pl.ib) {
-        case 1: k_grav_mixed<1><<<grid, TPB, 0, s>>>(A, B, n, pl.slice_len, out); break;
-        case 2: k_grav_mixed<2><<<grid, TPB, 0, s>>>(A, B, n, pl.slice_len, out); break;
-        case 4: k_grav_mixed<4><<<grid, TPB, 0, s>>>(A, B, n, pl.slice_len, out); break;
-        default: fail(RRG_ERR_INVALID, "grav_mixed: unsupported launch shape");
+    const int64_t n = p.n;
+    if (is_fp32_kind(f.kind)) {
+        switch (pl.ib) {
+            case 1: k_grav_mixed<1><<<grid, TPB, 0, s>>>(A, B, n, pl.slice_len, out); break;
+            case 2: k_grav_mixed<2><<<grid, TPB, 0, s>>>(A, B, n, pl.slice_len, out); break;
+            case 4: k_grav_mixed<4><<<grid, TPB, 0, s>>>(A, B, n, pl.slice_len, out); break;
+            default: fail(RRG_ERR_INVALID, "direct-sum potential: unsupported launch shape");
+        }
+    } else {
+        const int rsq = f.kind == RRG_FORCE_F64_EXACT ? 0 : 1;
+        if (pl.ib == 1 && rsq == 0) k_grav_f64<1, 0><<<grid, TPB, 0, s>>>(p, pl.slice_len, out);
+        else if (pl.ib == 2 && rsq == 0) k_grav_f64<2, 0><<<grid, TPB, 0, s>>>(p, pl.slice_len, out);
+        else if (pl.ib == 1 && rsq == 1) k_grav_f64<1, 1><<<grid, TPB, 0, s>>>(p, pl.slice_len, out);
+        else if (pl.ib == 2 && rsq == 1) k_grav_f64<2, 1><<<grid, TPB, 0, s>>>(p, pl.slice_len, out);
+        else fail(RRG_ERR_INVALID, "direct-sum potential: unsupported launch shape");
     }
     RRG_LAUNCH_CHECK();
 }

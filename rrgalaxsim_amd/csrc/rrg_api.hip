@@ -141,6 +141,30 @@ void zero_async(hipStream_t s, void* p, size_t bytes) {
     if (bytes) RRG_HIP(hipMemsetAsync(p, 0, bytes, s));
 }
 
+// Force arithmetic for a direct-sum evaluation; detects uniform softening
+// (then h_i + h_j is folded into one constant, formed in the kernel's
+// precision exactly as the per-pair sum would be).
+nbody::Force make_force(int kind, int64_t n, const double* soft2) {
+    nbody::Force f;
+    f.kind = kind;
+    f.uniform = n > 0;
+    for (int64_t i = 1; i < n && f.uniform; ++i) f.uniform = soft2[i] == soft2[0];
+    if (f.uniform) {
+        if (nbody::is_fp32_kind(kind)) {
+            const float h = float(0.5 * soft2[0]);
+            f.hh = double(h + h);
+        } else {
+            const double h = 0.5 * soft2[0];
+            f.hh = h + h;
+        }
+    }
+    return f;
+}
+
+bool valid_force_kind(int k) {
+    return k == RRG_FORCE_F64 || k == RRG_FORCE_MIXED || k == RRG_FORCE_FP32 || k == RRG_FORCE_F64_EXACT;
+}
+
 // ------------------------------------------------------------ stage timing
 // HIP events recorded on the context stream around each timed stage.  The
 // event pairs are resolved (and returned to the pool) by collect().
@@ -266,7 +290,8 @@ public:
                 const double* mass, const double* soft2)
         : prm_(p), s_(s), kind_(p.force_kind) {
         require(soft2 != nullptr, "rrg_create: soft2 is required by the direct-sum model");
-        require(kind_ == RRG_FORCE_F64 || kind_ == RRG_FORCE_MIXED, "rrg_create: unknown force_kind");
+        require(valid_force_kind(kind_), "rrg_create: unknown force_kind");
+        force_ = make_force(kind_, n, soft2);
         P_.alloc(n);
         upload_vec<double>(s_, n, pos, P_.X.p);
         upload_vec<double>(s_, n, vel, P_.V.p);
@@ -277,7 +302,7 @@ public:
         zero_async(s_, P_.G.p, P_.G.bytes());
         plan_ = nbody::plan_direct(n, kind_);
         if (plan_.slices > 1) part_.alloc(size_t(plan_.slices) * 3 * n);
-        if (kind_ == RRG_FORCE_MIXED) {
+        if (nbody::is_fp32_kind(kind_)) {
             A_.alloc(n);
             B_.alloc(n);
         }
@@ -285,7 +310,7 @@ public:
     }
 
     void precalc() override {  // rrgsim_integration_nbody.cuh:26-30
-        if (kind_ == RRG_FORCE_MIXED) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
+        if (nbody::is_fp32_kind(kind_)) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
         grav();
         accel(true);
         grav_actual_ = true;
@@ -329,7 +354,7 @@ public:
     void set_state(const double* pos, const double* vel) override {
         if (pos) upload_vec<double>(s_, P_.n, pos, P_.X.p);
         if (vel) upload_vec<double>(s_, P_.n, vel, P_.V.p);
-        if (pos && kind_ == RRG_FORCE_MIXED) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
+        if (pos && nbody::is_fp32_kind(kind_)) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
         grav_actual_ = false;
     }
 
@@ -353,8 +378,7 @@ private:
     void accel(bool reduce_now) {
         const int64_t n = P_.n;
         double* out = plan_.slices > 1 ? part_.p : P_.A.p;
-        if (kind_ == RRG_FORCE_MIXED) nbody::accel_mixed(s_, plan_, n, A_.p, B_.p, out);
-        else nbody::accel_f64(s_, plan_, P_.view(), out, rsqrt_kind_);
+        nbody::accel(s_, plan_, force_, P_.view(), A_.p, B_.p, out);
         if (reduce_now && plan_.slices > 1) nbody::reduce_slices(s_, n, 3, plan_.slices, part_.p, P_.A.p);
     }
     void grav() {
@@ -367,15 +391,14 @@ private:
             tmp = &gpart_;
             out = gpart_.p;
         }
-        if (kind_ == RRG_FORCE_MIXED) nbody::grav_mixed(s_, pl, n, A_.p, B_.p, out);
-        else nbody::grav_f64(s_, pl, P_.view(), out, rsqrt_kind_);
+        nbody::grav(s_, pl, force_, P_.view(), A_.p, B_.p, out);
         if (tmp) nbody::reduce_slices(s_, n, 1, pl.slices, tmp->p, P_.G.p);
     }
 
     rrg_params prm_;
     hipStream_t s_;
     int kind_;
-    int rsqrt_kind_ = 0;
+    nbody::Force force_;
     Particles<double> P_;
     nbody::Plan plan_;
     DBuf<double> part_, gpart_, cpart_;
@@ -871,7 +894,7 @@ template <class T> void d2h(hipStream_t s, void* dst, const T* src, size_t bytes
 void direct_op(int32_t kind, int64_t n, const double* pos, const double* mass, const double* soft2, double* out,
                bool potential) {
     require(n > 0 && pos && mass && soft2 && out, "direct-sum operator: bad arguments");
-    require(kind == RRG_FORCE_F64 || kind == RRG_FORCE_MIXED, "direct-sum operator: unknown force_kind");
+    require(valid_force_kind(kind), "direct-sum operator: unknown force_kind");
     OpStream os;
     Particles<double> P;
     P.alloc(n);
@@ -883,16 +906,14 @@ void direct_op(int32_t kind, int64_t n, const double* pos, const double* mass, c
     DBuf<double> part(size_t(pl.slices) * comps * n);
     DBuf<double> res(size_t(comps) * n);
     DBuf<float4> A, B;
-    if (kind == RRG_FORCE_MIXED) {
+    const nbody::Force f = make_force(kind, n, soft2);
+    if (nbody::is_fp32_kind(kind)) {
         A.alloc(n);
         B.alloc(n);
         nbody::pack_mixed(os.s, P.view(), A.p, B.p);
-        if (potential) nbody::grav_mixed(os.s, pl, n, A.p, B.p, part.p);
-        else nbody::accel_mixed(os.s, pl, n, A.p, B.p, part.p);
-    } else {
-        if (potential) nbody::grav_f64(os.s, pl, P.view(), part.p, 0);
-        else nbody::accel_f64(os.s, pl, P.view(), part.p, 0);
     }
+    if (potential) nbody::grav(os.s, pl, f, P.view(), A.p, B.p, part.p);
+    else nbody::accel(os.s, pl, f, P.view(), A.p, B.p, part.p);
     nbody::reduce_slices(os.s, n, comps, pl.slices, part.p, res.p);
     if (potential) {
         d2h(os.s, out, res.p, sizeof(double) * n);

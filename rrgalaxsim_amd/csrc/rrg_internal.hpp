@@ -80,17 +80,27 @@ struct Plan {
 };
 
 Plan plan_direct(int64_t n, int force_kind, int target_workgroups = 0);
+bool is_fp32_kind(int force_kind);
 
-// Packed FP32 source records for the mixed kernel:
+// Force arithmetic of one evaluation: kind (RRG_FORCE_*) and, when every
+// particle has the same softening, hh = h_i + h_j (h = 0.5 * soft2).
+struct Force {
+    int kind = RRG_FORCE_F64;
+    bool uniform = false;
+    double hh = 0.0;
+};
+
+// Packed FP32 source records for the FP32 kinds:
 //   A = (x_hi, y_hi, z_hi, m), B = (x_lo, y_lo, z_lo, 0.5*soft2).
 void pack_mixed(hipStream_t s, const PState<double>& p, float4* A, float4* B);
 
-// Acceleration partials: out[(slice*3 + c)*n + i].
-void accel_f64(hipStream_t s, const Plan& pl, const PState<double>& p, double* out, int rsqrt_kind);
-void accel_mixed(hipStream_t s, const Plan& pl, int64_t n, const float4* A, const float4* B, double* out);
+// Acceleration partials: out[(slice*3 + c)*n + i].  A/B are used by the
+// FP32 kinds only.
+void accel(hipStream_t s, const Plan& pl, const Force& f, const PState<double>& p, const float4* A, const float4* B,
+           double* out);
 // Potential partials: out[slice*n + i].
-void grav_f64(hipStream_t s, const Plan& pl, const PState<double>& p, double* out, int rsqrt_kind);
-void grav_mixed(hipStream_t s, const Plan& pl, int64_t n, const float4* A, const float4* B, double* out);
+void grav(hipStream_t s, const Plan& pl, const Force& f, const PState<double>& p, const float4* A, const float4* B,
+          double* out);
 
 // Sum `slices` partial blocks of `comps` components into dst (comps*n).
 void reduce_slices(hipStream_t s, int64_t n, int comps, int slices, const double* part, double* dst);

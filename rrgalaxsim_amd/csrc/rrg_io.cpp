@@ -284,9 +284,11 @@ void params_parse_json(const std::string& path, rrg_params* p, rrg_ini_params* i
             p->real_type = v->str == "fp32" ? RRG_REAL_F32 : RRG_REAL_F64;
         }
         if (const JValue* v = m->find("force")) {
-            if (v->kind != JValue::String || (v->str != "fp64" && v->str != "mixed"))
-                fail(RRG_ERR_IO, "mi355x.force must be \"fp64\" or \"mixed\"");
-            p->force_kind = v->str == "mixed" ? RRG_FORCE_MIXED : RRG_FORCE_F64;
+            const std::map<std::string, int> kinds = {{"fp64", RRG_FORCE_F64}, {"mixed", RRG_FORCE_MIXED},
+                                                      {"fp32", RRG_FORCE_FP32}, {"fp64_exact", RRG_FORCE_F64_EXACT}};
+            if (v->kind != JValue::String || !kinds.count(v->str))
+                fail(RRG_ERR_IO, "mi355x.force must be \"fp64\", \"mixed\", \"fp32\" or \"fp64_exact\"");
+            p->force_kind = kinds.at(v->str);
         }
         if (const JValue* v = m->find("boundary")) {
             if (v->kind != JValue::String || (v->str != "periodic" && v->str != "zero_dirichlet"))

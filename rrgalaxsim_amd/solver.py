@@ -42,7 +42,7 @@ def make_params(*, dt, time_max=None, dt_save=None, grid=None, wave=None, real="
         for k, v in wave.items():
             setattr(p.wave, k, v)
     p.real_type = {"fp64": L.REAL_F64, "fp32": L.REAL_F32}[real]
-    p.force_kind = {"fp64": L.FORCE_F64, "mixed": L.FORCE_MIXED}[force]
+    p.force_kind = L.FORCE_KINDS[force]
     p.boundary = {"periodic": L.BC_PERIODIC, "zero_dirichlet": L.BC_ZERO_DIRICHLET}[boundary]
     p.fixed_exponent = fixed_exponent
     p.device = device
@@ -162,7 +162,7 @@ def direct_accel(pos, mass, soft2, force="fp64"):
     n = len(mass)
     pos = L.f64(pos, (n, 3)); mass = L.f64(mass); soft2 = L.f64(soft2)
     out = np.empty((n, 3))
-    kind = {"fp64": L.FORCE_F64, "mixed": L.FORCE_MIXED}[force]
+    kind = L.FORCE_KINDS[force]
     L.check(L.lib().rrg_op_direct_accel(kind, n, L.ptr(pos), L.ptr(mass), L.ptr(soft2), L.ptr(out)))
     return out
 
@@ -172,7 +172,7 @@ def direct_grav(pos, mass, soft2, force="fp64"):
     n = len(mass)
     pos = L.f64(pos, (n, 3)); mass = L.f64(mass); soft2 = L.f64(soft2)
     out = np.empty(n)
-    kind = {"fp64": L.FORCE_F64, "mixed": L.FORCE_MIXED}[force]
+    kind = L.FORCE_KINDS[force]
     L.check(L.lib().rrg_op_direct_grav(kind, n, L.ptr(pos), L.ptr(mass), L.ptr(soft2), L.ptr(out)))
     return out
 

@@ -19,27 +19,32 @@ def _rel(a, ref):
     return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
 
 
+@pytest.mark.parametrize("force", ["fp64", "fp64_exact"])
 @pytest.mark.parametrize("n", [1, 2, 255, 1000, 4096])
-def test_fp64_accel_matches_oracle(rrg, orc, n):
+def test_fp64_accel_matches_oracle(rrg, orc, n, force):
     pos, _, m = ics.plummer(max(n, 2), seed=0)
     pos, m = pos[:n], m[:n]
     s2 = np.full(n, 1e-4)
     ref = orc.accel(pos, m, s2)
-    got = rrg.direct_accel(pos, m, s2, force="fp64")
+    got = rrg.direct_accel(pos, m, s2, force=force)
     scale = np.sqrt(np.mean(np.sum(ref ** 2, axis=1))) + 1e-300
     err = np.max(np.linalg.norm(got - ref, axis=1)) / scale
     print(f"n={n} fp64 max err / rms|a| = {err:.3e}")
     assert err < 1e-13
 
 
-def test_fp64_accel_unequal_softening(rrg, orc):
+@pytest.mark.parametrize("force,tol", [("fp64", 1e-14), ("fp64_exact", 1e-14), ("mixed", 1e-6), ("fp32", 1e-6)])
+def test_accel_unequal_softening(rrg, orc, force, tol):
+    """Non-uniform softening takes the per-pair 0.5*(s_i + s_j) path."""
     pos, _, m = ics.plummer(777, seed=5)
     rng = np.random.default_rng(5)
     s2 = rng.uniform(1e-5, 1e-3, len(m))
     m = m * rng.uniform(0.5, 1.5, len(m))
     ref = orc.accel(pos, m, s2)
-    got = rrg.direct_accel(pos, m, s2, force="fp64")
-    assert np.median(_rel(got, ref)) < 1e-14
+    got = rrg.direct_accel(pos, m, s2, force=force)
+    assert np.median(_rel(got, ref)) < tol
+    if force != "fp64":
+        return
     grav_ref = orc.grav(pos, m, s2)
     grav = rrg.direct_grav(pos, m, s2, force="fp64")
     assert np.max(np.abs(grav - grav_ref) / np.abs(grav_ref)) < 1e-13
@@ -66,9 +71,13 @@ def test_mixed_accel_error_c3_sample(rrg, orc):
     rel = _rel(got, ref)
     med, p99, mx = np.median(rel), np.quantile(rel, 0.99), np.max(rel)
     print(f"mixed C3 sample: median {med:.3e} p99 {p99:.3e} max {mx:.3e}")
-    assert med <= 1e-6
-    got64 = rrg.direct_accel(pos, m, s2, force="fp64")[idx]
-    assert np.median(_rel(got64, ref)) < 1e-13
+    assert med <= 1e-6 and p99 <= 1e-6
+    rel32 = _rel(rrg.direct_accel(pos, m, s2, force="fp32")[idx], ref)
+    print(f"fp32 C3 sample: median {np.median(rel32):.3e} p99 {np.quantile(rel32, 0.99):.3e} max {rel32.max():.3e}")
+    assert np.median(rel32) <= 1e-6
+    for force in ("fp64", "fp64_exact"):
+        got64 = rrg.direct_accel(pos, m, s2, force=force)[idx]
+        assert np.median(_rel(got64, ref)) < 1e-13
 
 
 @pytest.mark.parametrize("n", [1, 100, 4096])
