@@ -128,6 +128,53 @@ def direct_bench(rrg, ics, steps=3):
     return out
 
 
+def ensemble_bench(rrg, ics, steps=100):
+    """64 systems, N_g = round(2^U(12,16)) (seed 2), Plummer for even g and
+    exponential disk for odd g, eps = 0.01, dt = 1e-3, FP32 forces (hi/lo) on
+    FP64 state; one persistent work-queue launch per force evaluation,
+    hipGraph-replayed steps, autotuned launch shape."""
+    sizes = ics.ensemble_sizes(64, seed=2)
+    rng = np.random.default_rng(2)
+    parts = [ics.plummer(int(nn), rng=rng) if g % 2 == 0 else ics.exp_disk(int(nn), rng=rng)
+             for g, nn in enumerate(sizes)]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    pos = np.concatenate([q[0] for q in parts]); vel = np.concatenate([q[1] for q in parts])
+    m = np.concatenate([q[2] for q in parts])
+    s2 = np.full(len(m), 0.01 ** 2)
+    p = rrg.make_params(dt=1e-3, force="mixed", autotune=True)
+    sim = rrg.Simulation(p, pos, vel, m, s2, offsets=off)
+    sim.precalc()
+    sim.calc_step(2)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.calc_step(steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    stats = sim.stats()
+    # kernel time of the force launch, from events on the context stream
+    sim.set_timing(True)
+    sim.calc_step(3)
+    tm = sim.timing()
+    sim.close()
+    inter = float(np.sum(sizes.astype(np.float64) ** 2))
+    kern_s = tm["direct"][0] / 1e3 / 3
+    return {
+        "ginteractions_per_s": inter * steps / (t1 - t0) / 1e9,
+        "kernel_ginteractions_per_s": inter / kern_s / 1e9,
+        "ms_per_step": (t1 - t0) / steps * 1e3,
+        "force_kernel_ms": kern_s * 1e3,
+        "fp32_vector_frac": 20 * inter / kern_s / 1e12 / FP32_PEAK_TFLOPS,
+        "launches_per_step": 3,
+        "graph_launches": stats["graph_launches"],
+        "tuned_shape": {"ib": stats["tile_ib"], "sources_per_item": stats["slice_len"],
+                        "workgroups": stats["workgroups"], "items": stats["work_items"],
+                        "candidates": stats["tune_candidates"]},
+        "config": {"workload": "64 galaxies, N log-uniform in [2^12, 2^16] (seed 2), Plummer/disk alternating, "
+                               "eps=0.01, dt=1e-3, mixed FP32 forces / FP64 state", "systems": 64,
+                   "n_total": int(off[-1]), "sum_n2": inter, "timed_steps": steps},
+    }
+
+
 def cpu_baseline(ics):
     """The CPU oracle (FP32 restatement) on a bounded sample of the headline
     workload: the full 1e6-particle / 128^3 problem, 2 PM steps (each with the
@@ -199,6 +246,7 @@ def main():
         for k in ("fp64", "fp64_exact"):
             d[k]["fp64_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP64_PEAK_TFLOPS
         line["direct_sum"] = d
+        line["ensemble"] = ensemble_bench(rrg, ics)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ics)
     print(json.dumps(line), flush=True)

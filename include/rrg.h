@@ -117,6 +117,9 @@ typedef struct {
     int32_t boundary;       /* RRG_BC_*; default RRG_BC_PERIODIC */
     int32_t fixed_exponent; /* CIC fixed-point resolution 2^-e; 0 = derive from total mass */
     int32_t device;         /* HIP device ordinal; default 0 */
+    int32_t use_graphs;     /* 1 (default): replay captured hipGraphs for steps */
+    int32_t autotune;       /* 1: time candidate launch shapes at precalc and keep
+                               the fastest (cached per process); default 0 */
 } rrg_params;
 
 typedef struct {
@@ -132,6 +135,14 @@ typedef struct {
     int32_t last_wave_iterations; /* stencil iterations run by the last wave solve */
     int32_t fixed_exponent;    /* CIC fixed-point exponent in use */
     double time;               /* sum of dt_dynamics over completed steps */
+    int32_t systems;           /* independent systems (1, or k of an ensemble) */
+    int32_t tile_ib;           /* direct sum: targets per lane */
+    int64_t slice_len;         /* direct sum: sources per slice / work item */
+    int32_t workgroups;        /* direct sum: workgroups of the force launch */
+    int32_t work_items;        /* direct sum, ensemble: items in the work list */
+    int32_t tuned;             /* 1 when the launch shape came from the autotuner */
+    int32_t tune_candidates;   /* candidates timed by the autotuner */
+    int64_t graph_launches;    /* steps replayed from a captured hipGraph */
 } rrg_stats;
 
 /* Stages timed by rrg_set_timing / rrg_get_timing. */
@@ -188,6 +199,14 @@ int rrg_ini_read(const rrg_ini_params* ini, const char* base_dir, int64_t* n, do
  * mode. */
 int rrg_create(const rrg_params* p, int64_t n, const double* pos, const double* vel,
                const double* mass, const double* soft2, rrg_ctx** out);
+/* Ensemble of k independent systems of uneven size in one context (no
+ * counterpart in the reference): system g holds particles
+ * [offsets[g], offsets[g+1]) of the concatenated arrays (offsets[0] = 0,
+ * offsets[k] = n).  Direct-sum model only; every force evaluation is one
+ * persistent work-queue launch over (system, target tile, source slice)
+ * items, and steps replay a captured hipGraph. */
+int rrg_create_ensemble(const rrg_params* p, int32_t k, const int64_t* offsets, const double* pos,
+                        const double* vel, const double* mass, const double* soft2, rrg_ctx** out);
 int rrg_destroy(rrg_ctx* ctx);
 /* precalc(): initial potential and acceleration
  * (rrgsim_integration_nbody.cuh:26-30; rrgsim_integration_rect.cuh:53-83). */
@@ -210,6 +229,9 @@ int rrg_get_conservation(rrg_ctx* ctx, rrg_conservation* out);
 /* PM grids (any pointer may be NULL): cell mass (nx^3), phi (nx^3), -grad phi
  * (nx^3,3), x-fastest (co_device_utils.cuh:61). */
 int rrg_get_grid(rrg_ctx* ctx, double* mass, double* phi, double* acc);
+/* Per-system conservation sums of an ensemble (k entries; k = 1 for a
+ * single system gives the same as rrg_get_conservation). */
+int rrg_get_conservation_systems(rrg_ctx* ctx, int32_t k, rrg_conservation* out);
 int rrg_get_stats(rrg_ctx* ctx, rrg_stats* out);
 /* Wait for all work queued on the context stream. */
 int rrg_synchronize(rrg_ctx* ctx);

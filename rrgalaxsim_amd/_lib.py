@@ -60,7 +60,8 @@ class IniParams(ctypes.Structure):
 class Params(ctypes.Structure):
     _fields_ = [("sim", SimParams), ("has_grid", _i32), ("grid", GridParams), ("has_wave", _i32),
                 ("wave", WaveParams), ("use_wave_model", _i32), ("real_type", _i32), ("force_kind", _i32),
-                ("boundary", _i32), ("fixed_exponent", _i32), ("device", _i32)]
+                ("boundary", _i32), ("fixed_exponent", _i32), ("device", _i32), ("use_graphs", _i32),
+                ("autotune", _i32)]
 
 
 class Conservation(ctypes.Structure):
@@ -69,7 +70,8 @@ class Conservation(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("n", _i64), ("steps", _i64), ("last_wave_iterations", _i32), ("fixed_exponent", _i32),
-                ("time", _f64)]
+                ("time", _f64), ("systems", _i32), ("tile_ib", _i32), ("slice_len", _i64), ("workgroups", _i32),
+                ("work_items", _i32), ("tuned", _i32), ("tune_candidates", _i32), ("graph_launches", _i64)]
 
 
 class Timing(ctypes.Structure):
@@ -90,7 +92,10 @@ SIGNATURES = {
     "rrg_ini_read": (ctypes.c_int, [ctypes.POINTER(IniParams), ctypes.c_char_p, ctypes.POINTER(_i64),
                                     _vp, _vp, _vp, _vp]),
     "rrg_create": (ctypes.c_int, [ctypes.POINTER(Params), _i64, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "rrg_create_ensemble": (ctypes.c_int, [ctypes.POINTER(Params), _i32, _vp, _vp, _vp, _vp, _vp,
+                                           ctypes.POINTER(_vp)]),
     "rrg_destroy": (ctypes.c_int, [_vp]),
+    "rrg_get_conservation_systems": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(Conservation)]),
     "rrg_precalc": (ctypes.c_int, [_vp]),
     "rrg_step": (ctypes.c_int, [_vp, _i32]),
     "rrg_sync": (ctypes.c_int, [_vp]),

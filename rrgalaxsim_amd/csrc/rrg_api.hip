@@ -11,7 +11,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -258,9 +260,20 @@ struct Solver {
     virtual void get_state(double* pos, double* vel, double* acc, double* grav) = 0;
     virtual void set_state(const double* pos, const double* vel) = 0;
     virtual void conservation(rrg_conservation* out) = 0;
+    virtual void conservation_systems(int k, rrg_conservation* out) = 0;
     virtual void get_grid(double* mass, double* phi, double* acc) = 0;
     virtual void stats(rrg_stats* out) = 0;
 };
+
+void fill_conservation(const double* q, double time, rrg_conservation* out) {
+    out->time = time;
+    out->Ek = 0.5 * q[0];
+    out->Ep = 0.5 * q[1];
+    for (int c = 0; c < 3; ++c) {
+        out->momentum[c] = q[2 + c];
+        out->angular[c] = q[5 + c];
+    }
+}
 
 template <class real> void conservation_common(hipStream_t s, const PState<real>& v, DBuf<double>& part,
                                                PinnedBuf& host, double time, rrg_conservation* out) {
@@ -274,23 +287,67 @@ template <class real> void conservation_common(hipStream_t s, const PState<real>
     double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t b = 0; b < nb; ++b)
         for (int k = 0; k < 8; ++k) q[k] += h[b * 8 + k];
-    out->time = time;
-    out->Ek = 0.5 * q[0];
-    out->Ep = 0.5 * q[1];
-    for (int c = 0; c < 3; ++c) {
-        out->momentum[c] = q[2 + c];
-        out->angular[c] = q[5 + c];
-    }
+    fill_conservation(q, time, out);
 }
 
-// Direct-sum strategy (NBodyIntegrationSolver), FP64 particle state.
+// Captured step graphs, keyed by the state that changes the launches'
+// arguments (buffer rotation, particle buffers).  A graph holds kernel nodes
+// only: the captured work records no memset.
+class StepGraphs {
+public:
+    ~StepGraphs() { clear(); }
+    void clear() {
+        for (auto& kv : execs_) (void)hipGraphExecDestroy(kv.second);
+        execs_.clear();
+    }
+    template <class F> void run(hipStream_t s, uint64_t key, F&& enqueue) {
+        auto it = execs_.find(key);
+        if (it == execs_.end()) {
+            hipGraph_t g = nullptr;
+            RRG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue();
+            } catch (...) {
+                if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            RRG_HIP(hipStreamEndCapture(s, &g));
+            hipGraphExec_t e = nullptr;
+            const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            RRG_HIP(err);
+            it = execs_.emplace(key, e).first;
+        }
+        RRG_HIP(hipGraphLaunch(it->second, s));
+        ++launches_;
+    }
+    int64_t launches() const { return launches_; }
+
+private:
+    std::map<uint64_t, hipGraphExec_t> execs_;
+    int64_t launches_ = 0;
+};
+
+// Process-wide cache of autotuned direct-sum launch shapes.
+struct TunedShape {
+    int ib = 0;
+    int64_t len = 0;  // grid form: target workgroups; queue form: sources per item
+};
+std::mutex g_tune_mutex;
+std::map<std::string, TunedShape> g_tune_cache;
+
+// Direct-sum strategy (NBodyIntegrationSolver), FP64 particle state, for one
+// system (grid-form launches) or an ensemble of independent systems
+// (queue-form launches over a work list).
 class NBodySolver final : public Solver {
 public:
     NBodySolver(const rrg_params& p, hipStream_t s, int64_t n, const double* pos, const double* vel,
-                const double* mass, const double* soft2)
-        : prm_(p), s_(s), kind_(p.force_kind) {
+                const double* mass, const double* soft2, const std::vector<int64_t>& offsets)
+        : prm_(p), s_(s), kind_(p.force_kind), offsets_(offsets) {
         require(soft2 != nullptr, "rrg_create: soft2 is required by the direct-sum model");
         require(valid_force_kind(kind_), "rrg_create: unknown force_kind");
+        k_ = int(offsets_.size()) - 1;
+        ensemble_ = k_ > 1;
         force_ = make_force(kind_, n, soft2);
         P_.alloc(n);
         upload_vec<double>(s_, n, pos, P_.X.p);
@@ -300,17 +357,22 @@ public:
         zero_async(s_, P_.A.p, P_.A.bytes());
         zero_async(s_, P_.W.p, P_.W.bytes());
         zero_async(s_, P_.G.p, P_.G.bytes());
-        plan_ = nbody::plan_direct(n, kind_);
-        if (plan_.slices > 1) part_.alloc(size_t(plan_.slices) * 3 * n);
+        d_offsets_.alloc(offsets_.size());
+        RRG_HIP(hipMemcpyAsync(d_offsets_.p, offsets_.data(), sizeof(int64_t) * offsets_.size(),
+                               hipMemcpyHostToDevice, s_));
+        counters_.alloc(2);
+        zero_async(s_, counters_.p, counters_.bytes());
         if (nbody::is_fp32_kind(kind_)) {
             A_.alloc(n);
             B_.alloc(n);
         }
+        configure(0, 0);
         RRG_HIP(hipStreamSynchronize(s_));
     }
 
     void precalc() override {  // rrgsim_integration_nbody.cuh:26-30
         if (nbody::is_fp32_kind(kind_)) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
+        if (prm_.autotune) tune();
         grav();
         accel(true);
         grav_actual_ = true;
@@ -319,21 +381,13 @@ public:
 
     void step(int nsteps) override {  // rrgsim_integration_nbody.cuh:50-56
         if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_step before rrg_precalc");
-        const double dt = prm_.sim.dt_dynamics;
         for (int k = 0; k < nsteps; ++k) {
-            int t = timer.start(s_, RRG_STAGE_PREDICT);
-            nbody::predict<double>(s_, P_.view(), dt, A_.p, B_.p);
-            timer.stop(s_, t, 1);
-            t = timer.start(s_, RRG_STAGE_DIRECT);
-            accel(false);
-            timer.stop(s_, t, 1);
-            t = timer.start(s_, RRG_STAGE_CORRECT);
-            nbody::correct_reduce(s_, P_.view(), dt, plan_.slices, plan_.slices > 1 ? part_.p : nullptr);
-            timer.stop(s_, t, 1);
+            if (prm_.use_graphs && !timer.on()) graphs_.run(s_, 0, [&] { enqueue_step(); });
+            else enqueue_step();
             ++steps_;
-            time_ += dt;
+            time_ += prm_.sim.dt_dynamics;
         }
-        grav_actual_ = false;
+        if (nsteps > 0) grav_actual_ = false;
     }
 
     // The reference returns early without a grid and leaves the potential
@@ -363,6 +417,18 @@ public:
         conservation_common<double>(s_, P_.view(), cpart_, host_, time_, out);
     }
 
+    void conservation_systems(int k, rrg_conservation* out) override {
+        require(k == k_, "rrg_get_conservation_systems: k does not match the context");
+        sync();
+        if (cpart_.n < size_t(8 * k_)) cpart_.alloc(8 * k_);
+        nbody::conservation_segments(s_, P_.view(), k_, d_offsets_.p, cpart_.p);
+        host_.ensure(sizeof(double) * 8 * k_);
+        RRG_HIP(hipMemcpyAsync(host_.p, cpart_.p, sizeof(double) * 8 * k_, hipMemcpyDeviceToHost, s_));
+        RRG_HIP(hipStreamSynchronize(s_));
+        const double* h = static_cast<const double*>(host_.p);
+        for (int g = 0; g < k_; ++g) fill_conservation(h + 8 * g, time_, out + g);
+    }
+
     void get_grid(double*, double*, double*) override {
         fail(RRG_ERR_STATE, "rrg_get_grid: the direct-sum model has no grid");
     }
@@ -372,40 +438,188 @@ public:
         out->n = P_.n;
         out->steps = steps_;
         out->time = time_;
+        out->systems = k_;
+        out->tile_ib = ensemble_ ? qplan_.ib : plan_.ib;
+        out->slice_len = ensemble_ ? item_len_ : plan_.slice_len;
+        out->workgroups = ensemble_ ? qplan_.workgroups : int(plan_.tiles * plan_.slices);
+        out->work_items = ensemble_ ? qplan_.nitems : 0;
+        out->tuned = tuned_ ? 1 : 0;
+        out->tune_candidates = tune_candidates_;
+        out->graph_launches = graphs_.launches();
     }
 
 private:
+    // Launch shape.  Grid form: ib and target workgroup count (0 = default).
+    // Queue form: ib and sources per item (0 = default).
+    void configure(int ib, int64_t len) {
+        const int64_t n = P_.n;
+        graphs_.clear();
+        if (!ensemble_) {
+            plan_ = nbody::plan_direct(n, kind_, int(len), ib);
+            if (plan_.slices > 1 && part_.n < size_t(plan_.slices) * 3 * n) part_.alloc(size_t(plan_.slices) * 3 * n);
+            if (plan_.slices > 1 && gpart_.n < size_t(plan_.slices) * n) gpart_.alloc(size_t(plan_.slices) * n);
+            return;
+        }
+        if (ib <= 0 || !nbody::valid_ib(kind_, ib)) ib = nbody::is_fp32_kind(kind_) ? 2 : 1;
+        if (len <= 0) len = 4096;
+        const int tile = nbody::targets_per_tile(ib);
+        std::vector<nbody::WorkItem> items;
+        std::vector<unsigned char> ns(n);
+        int smax = 1;
+        for (int g = 0; g < k_; ++g) {
+            const int64_t o = offsets_[g], ng = offsets_[g + 1] - o;
+            if (ng <= 0) continue;
+            int64_t sg = std::min<int64_t>(255, std::max<int64_t>(1, (ng + len - 1) / len));
+            const int64_t lg = (ng + sg - 1) / sg;
+            sg = (ng + lg - 1) / lg;
+            smax = std::max<int>(smax, int(sg));
+            for (int64_t i = 0; i < ng; ++i) ns[o + i] = (unsigned char)sg;
+            for (int64_t t0 = 0; t0 < ng; t0 += tile)
+                for (int64_t q = 0; q < sg; ++q) {
+                    nbody::WorkItem w{};
+                    w.i0 = int(o + t0);
+                    w.i1 = int(o + std::min(ng, t0 + tile));
+                    w.j0 = int(o + q * lg);
+                    w.j1 = int(o + std::min(ng, (q + 1) * lg));
+                    w.slice = int(q);
+                    items.push_back(w);
+                }
+        }
+        // largest items first, so the queue's tail is made of small items
+        std::stable_sort(items.begin(), items.end(), [](const nbody::WorkItem& a, const nbody::WorkItem& b) {
+            return int64_t(a.i1 - a.i0) * (a.j1 - a.j0) > int64_t(b.i1 - b.i0) * (b.j1 - b.j0);
+        });
+        qplan_.ib = ib;
+        qplan_.nitems = int(items.size());
+        qplan_.workgroups = std::max(1, std::min<int>(qplan_.nitems, 2048));
+        item_len_ = len;
+        smax_ = smax;
+        items_.alloc(items.size());
+        RRG_HIP(hipMemcpyAsync(items_.p, items.data(), sizeof(nbody::WorkItem) * items.size(), hipMemcpyHostToDevice, s_));
+        ns_.alloc(n);
+        RRG_HIP(hipMemcpyAsync(ns_.p, ns.data(), n, hipMemcpyHostToDevice, s_));
+        if (part_.n < size_t(smax) * 3 * n) part_.alloc(size_t(smax) * 3 * n);
+        if (gpart_.n < size_t(smax) * n) gpart_.alloc(size_t(smax) * n);
+        RRG_HIP(hipStreamSynchronize(s_));
+    }
+
+    // Autotuner: time every candidate shape on the live data (one warm and
+    // one timed force launch each) and keep the fastest; results are cached
+    // per (kind, softening, system sizes) for the process.
+    void tune() {
+        std::string key = std::to_string(kind_) + (force_.uniform ? "u" : "v");
+        for (int64_t o : offsets_) key += ":" + std::to_string(o);
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mutex);
+            auto it = g_tune_cache.find(key);
+            if (it != g_tune_cache.end()) {
+                configure(it->second.ib, it->second.len);
+                tuned_ = true;
+                return;
+            }
+        }
+        const std::vector<int> ibs = nbody::is_fp32_kind(kind_) ? std::vector<int>{1, 2, 4} : std::vector<int>{1, 2};
+        const std::vector<int64_t> lens = ensemble_ ? std::vector<int64_t>{1024, 2048, 4096, 8192}
+                                                    : std::vector<int64_t>{1024, 2048, 4096, 8192};
+        hipEvent_t e0, e1;
+        RRG_HIP(hipEventCreate(&e0));
+        RRG_HIP(hipEventCreate(&e1));
+        TunedShape best;
+        float best_ms = 1e30f;
+        tune_candidates_ = 0;
+        try {
+            for (int ib : ibs)
+                for (int64_t len : lens) {
+                    configure(ib, len);
+                    accel(false);
+                    RRG_HIP(hipEventRecord(e0, s_));
+                    accel(false);
+                    RRG_HIP(hipEventRecord(e1, s_));
+                    RRG_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    RRG_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    ++tune_candidates_;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best.ib = ib;
+                        best.len = len;
+                    }
+                }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        configure(best.ib, best.len);
+        tuned_ = true;
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        g_tune_cache[key] = best;
+    }
+
+    void enqueue_step() {
+        const double dt = prm_.sim.dt_dynamics;
+        int t = timer.start(s_, RRG_STAGE_PREDICT);
+        nbody::predict<double>(s_, P_.view(), dt, A_.p, B_.p);
+        timer.stop(s_, t, 1);
+        t = timer.start(s_, RRG_STAGE_DIRECT);
+        accel(false);
+        timer.stop(s_, t, 1);
+        t = timer.start(s_, RRG_STAGE_CORRECT);
+        if (ensemble_) nbody::correct_reduce(s_, P_.view(), dt, 0, ns_.p, part_.p);
+        else nbody::correct_reduce(s_, P_.view(), dt, plan_.slices, nullptr, plan_.slices > 1 ? part_.p : nullptr);
+        timer.stop(s_, t, 1);
+    }
+
     void accel(bool reduce_now) {
         const int64_t n = P_.n;
+        if (ensemble_) {
+            nbody::queue(s_, qplan_, force_, false, P_.view(), A_.p, B_.p, items_.p, counters_.p, part_.p);
+            if (reduce_now) nbody::reduce_slices(s_, n, 3, 0, ns_.p, part_.p, P_.A.p);
+            return;
+        }
         double* out = plan_.slices > 1 ? part_.p : P_.A.p;
         nbody::accel(s_, plan_, force_, P_.view(), A_.p, B_.p, out);
-        if (reduce_now && plan_.slices > 1) nbody::reduce_slices(s_, n, 3, plan_.slices, part_.p, P_.A.p);
+        if (reduce_now && plan_.slices > 1) nbody::reduce_slices(s_, n, 3, plan_.slices, nullptr, part_.p, P_.A.p);
     }
+
     void grav() {
         const int64_t n = P_.n;
-        nbody::Plan pl = plan_;
-        DBuf<double>* tmp = nullptr;
-        double* out = P_.G.p;
-        if (pl.slices > 1) {
-            if (gpart_.n < size_t(pl.slices) * n) gpart_.alloc(size_t(pl.slices) * n);
-            tmp = &gpart_;
-            out = gpart_.p;
+        if (ensemble_) {
+            nbody::queue(s_, qplan_, force_, true, P_.view(), A_.p, B_.p, items_.p, counters_.p, gpart_.p);
+            nbody::reduce_slices(s_, n, 1, 0, ns_.p, gpart_.p, P_.G.p);
+            return;
         }
-        nbody::grav(s_, pl, force_, P_.view(), A_.p, B_.p, out);
-        if (tmp) nbody::reduce_slices(s_, n, 1, pl.slices, tmp->p, P_.G.p);
+        double* out = plan_.slices > 1 ? gpart_.p : P_.G.p;
+        nbody::grav(s_, plan_, force_, P_.view(), A_.p, B_.p, out);
+        if (plan_.slices > 1) nbody::reduce_slices(s_, n, 1, plan_.slices, nullptr, gpart_.p, P_.G.p);
     }
 
     rrg_params prm_;
     hipStream_t s_;
     int kind_;
     nbody::Force force_;
+    std::vector<int64_t> offsets_;
+    int k_ = 1;
+    bool ensemble_ = false;
     Particles<double> P_;
+    DBuf<int64_t> d_offsets_;
     nbody::Plan plan_;
+    nbody::QueuePlan qplan_;
+    int64_t item_len_ = 0;
+    int smax_ = 1;
+    DBuf<nbody::WorkItem> items_;
+    DBuf<unsigned char> ns_;
+    DBuf<int> counters_;
     DBuf<double> part_, gpart_, cpart_;
     DBuf<float4> A_, B_;
     PinnedBuf host_;
+    StepGraphs graphs_;
     bool precalc_done_ = false;
     bool grav_actual_ = false;
+    bool tuned_ = false;
+    int tune_candidates_ = 0;
     int64_t steps_ = 0;
     double time_ = 0.0;
 };
@@ -455,6 +669,7 @@ public:
         upload_vec<real>(s_, n, vel, P_.V.p);
         upload_scalar<real>(s_, n, mass, P_.M.p);
         // spatial ordering buffers (particles are kept sorted by CIC cell)
+        x_first_ = P_.X.p;
         P2_.alloc(n);
         id_.alloc(n);
         id2_.alloc(n);
@@ -500,18 +715,24 @@ public:
     void step(int nsteps) override {  // rrgsim_integration_rect.cuh:99-112
         if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_step before rrg_precalc");
         const real dt = (real)prm_.sim.dt_dynamics;
+        // Without a steady-state check the step makes no host decision, so
+        // it is replayed from a captured graph (one per buffer arrangement).
+        const bool graph = prm_.use_graphs && !timer.on() && !(tol_ > (real)0);
         for (int k = 0; k < nsteps; ++k) {
             if (++since_sort_ >= kSortInterval) sort_particles();
-            deposit(true);
-            int t = timer.start(s_, RRG_STAGE_WAVE);
-            last_iters_ = solve(prm_.wave.iterations);
-            timer.stop(s_, t, last_iters_);
-            t = timer.start(s_, RRG_STAGE_GRADIENT);
-            pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
-            timer.stop(s_, t, 1);
-            t = timer.start(s_, RRG_STAGE_GATHER);
-            pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true);
-            timer.stop(s_, t, 1);
+            if (graph) {
+                const uint64_t key = uint64_t(ip_) * 3 + uint64_t(ic_) + (P_.X.p == x_first_ ? 0 : 16);
+                const int ip0 = ip_, ic0 = ic_;
+                graphs_.run(s_, key, [&] {
+                    enqueue_step(dt);
+                    ip_ = ip0;
+                    ic_ = ic0;
+                });
+                rotate(prm_.wave.iterations);
+                last_iters_ = prm_.wave.iterations;
+            } else {
+                enqueue_step(dt);
+            }
             ++steps_;
             time_ += prm_.sim.dt_dynamics;
         }
@@ -556,6 +777,11 @@ public:
         conservation_common<real>(s_, P_.view(), cpart_, host_, time_, out);
     }
 
+    void conservation_systems(int k, rrg_conservation* out) override {
+        require(k == 1, "rrg_get_conservation_systems: a PM context holds one system");
+        conservation(out);
+    }
+
     void get_grid(double* mass, double* phi, double* acc) override {
         if (mass) download_scalar<real>(s_, ncell_, mass_.p, mass);
         if (phi) download_scalar<real>(s_, ncell_, cur(), phi);
@@ -575,10 +801,34 @@ public:
         out->time = time_;
         out->last_wave_iterations = last_iters_;
         out->fixed_exponent = e_;
+        out->systems = 1;
+        out->graph_launches = graphs_.launches();
     }
 
 private:
     static constexpr int kSortInterval = 8;  // steps between spatial re-sorts
+
+    void enqueue_step(real dt) {
+        deposit(true);
+        int t = timer.start(s_, RRG_STAGE_WAVE);
+        last_iters_ = solve(prm_.wave.iterations);
+        timer.stop(s_, t, last_iters_);
+        t = timer.start(s_, RRG_STAGE_GRADIENT);
+        pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
+        timer.stop(s_, t, 1);
+        t = timer.start(s_, RRG_STAGE_GATHER);
+        pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true);
+        timer.stop(s_, t, 1);
+    }
+
+    // The three-buffer rotation of `iters` iterations (wave.cu:271-272).
+    void rotate(int iters) {
+        for (int i = 0; i < iters % 3; ++i) {
+            const int in = 3 - ic_ - ip_;
+            ip_ = ic_;
+            ic_ = in;
+        }
+    }
 
     real* cur() { return phi_[ic_].p; }
 
@@ -669,6 +919,8 @@ private:
     size_t sort_bytes_ = 0;
     int end_bit_ = 32;
     int since_sort_ = 0;
+    real* x_first_ = nullptr;  // P_.X at creation (the sort swaps buffers)
+    StepGraphs graphs_;
     DBuf<unsigned long long> mfix_;
     DBuf<real> mass_, src_, phi_[3], gacc_;
     DBuf<unsigned long long> maxbits_;
@@ -778,8 +1030,49 @@ RRG_API int rrg_create(const rrg_params* p, int64_t n, const double* pos, const 
             } else {
                 require(ctx->p.real_type == RRG_REAL_F64,
                         "rrg_create: the direct-sum model keeps FP64 state (use force_kind for FP32 pair math)");
-                ctx->solver = std::make_unique<NBodySolver>(ctx->p, ctx->stream, n, pos, vel, mass, soft2);
+                ctx->solver = std::make_unique<NBodySolver>(ctx->p, ctx->stream, n, pos, vel, mass, soft2,
+                                                            std::vector<int64_t>{0, n});
             }
+        } catch (...) {
+            ctx->solver.reset();
+            (void)hipStreamDestroy(ctx->stream);
+            throw;
+        }
+        *out = ctx.release();
+    });
+}
+
+RRG_API int rrg_create_ensemble(const rrg_params* p, int32_t k, const int64_t* offsets, const double* pos,
+                                const double* vel, const double* mass, const double* soft2, rrg_ctx** out) {
+    return guarded([&] {
+        require(p && out && offsets, "rrg_create_ensemble: null argument");
+        *out = nullptr;
+        require(k >= 1, "rrg_create_ensemble: k must be positive");
+        require(offsets[0] == 0, "rrg_create_ensemble: offsets[0] must be 0");
+        for (int g = 0; g < k; ++g)
+            require(offsets[g + 1] > offsets[g], "rrg_create_ensemble: offsets must be strictly increasing");
+        const int64_t n = offsets[k];
+        require(n <= (int64_t(1) << 31) - 1, "rrg_create_ensemble: n exceeds 2^31-1");
+        require(pos && vel && mass && soft2, "rrg_create_ensemble: pos, vel, mass and soft2 are required");
+        check_finite(3 * n, pos, "pos");
+        check_finite(3 * n, vel, "vel");
+        check_finite(n, mass, "mass");
+        check_finite(n, soft2, "soft2");
+        int ndev = 0;
+        RRG_HIP(hipGetDeviceCount(&ndev));
+        require(p->device >= 0 && p->device < ndev, "rrg_create_ensemble: invalid device ordinal");
+        require(p->real_type == RRG_REAL_F64, "rrg_create_ensemble: the direct-sum model keeps FP64 state");
+        require(std::isfinite(p->sim.dt_dynamics), "rrg_create_ensemble: sim.dt_dynamics is not set");
+        auto ctx = std::make_unique<rrg_ctx>();
+        ctx->p = *p;
+        io::params_finalize(&ctx->p);
+        ctx->p.use_wave_model = 0;
+        ctx->device = p->device;
+        DeviceGuard dg(ctx->device);
+        RRG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        try {
+            ctx->solver = std::make_unique<NBodySolver>(ctx->p, ctx->stream, n, pos, vel, mass, soft2,
+                                                        std::vector<int64_t>(offsets, offsets + k + 1));
         } catch (...) {
             ctx->solver.reset();
             (void)hipStreamDestroy(ctx->stream);
@@ -832,6 +1125,13 @@ RRG_API int rrg_get_conservation(rrg_ctx* ctx, rrg_conservation* out) {
     RRG_CTX_CALL(ctx, {
         require(out != nullptr, "rrg_get_conservation: null output");
         ctx->solver->conservation(out);
+    });
+}
+
+RRG_API int rrg_get_conservation_systems(rrg_ctx* ctx, int32_t k, rrg_conservation* out) {
+    RRG_CTX_CALL(ctx, {
+        require(out != nullptr && k >= 1, "rrg_get_conservation_systems: bad arguments");
+        ctx->solver->conservation_systems(k, out);
     });
 }
 
@@ -914,7 +1214,7 @@ void direct_op(int32_t kind, int64_t n, const double* pos, const double* mass, c
     }
     if (potential) nbody::grav(os.s, pl, f, P.view(), A.p, B.p, part.p);
     else nbody::accel(os.s, pl, f, P.view(), A.p, B.p, part.p);
-    nbody::reduce_slices(os.s, n, comps, pl.slices, part.p, res.p);
+    nbody::reduce_slices(os.s, n, comps, pl.slices, nullptr, part.p, res.p);
     if (potential) {
         d2h(os.s, out, res.p, sizeof(double) * n);
     } else {

@@ -69,9 +69,9 @@ template <class real> struct PState {
 // ----------------------------------------------------------------- direct sum
 namespace nbody {
 
-// Launch shape of a direct-sum evaluation: `ib` targets per thread, the j
-// range split into `slices` contiguous slices of `slice_len` sources, each
-// handled by its own workgroup row; partial results are summed in slice order.
+// Grid-form launch shape of a direct-sum evaluation: `ib` targets per lane
+// (tiles of 256*ib targets), the source range split into `slices` contiguous
+// slices of `slice_len` sources; partial results are summed in slice order.
 struct Plan {
     int ib = 1;
     int slices = 1;
@@ -79,8 +79,11 @@ struct Plan {
     int64_t tiles = 0;  // workgroups along the target axis
 };
 
-Plan plan_direct(int64_t n, int force_kind, int target_workgroups = 0);
+// ib <= 0 selects the default for (n, kind); target_workgroups <= 0 the default.
+Plan plan_direct(int64_t n, int force_kind, int target_workgroups = 0, int ib = 0);
 bool is_fp32_kind(int force_kind);
+bool valid_ib(int force_kind, int ib);
+int targets_per_tile(int ib);
 
 // Force arithmetic of one evaluation: kind (RRG_FORCE_*) and, when every
 // particle has the same softening, hh = h_i + h_j (h = 0.5 * soft2).
@@ -88,6 +91,19 @@ struct Force {
     int kind = RRG_FORCE_F64;
     bool uniform = false;
     double hh = 0.0;
+};
+
+// One item of the queue form: targets [i0, i1) against sources [j0, j1),
+// written to partial slice `slice`.
+struct WorkItem {
+    int i0, i1, j0, j1, slice, pad0, pad1, pad2;
+};
+
+// Queue-form launch: `workgroups` persistent workgroups over `nitems` items.
+struct QueuePlan {
+    int ib = 1;
+    int nitems = 0;
+    int workgroups = 0;
 };
 
 // Packed FP32 source records for the FP32 kinds:
@@ -101,24 +117,34 @@ void accel(hipStream_t s, const Plan& pl, const Force& f, const PState<double>& 
 // Potential partials: out[slice*n + i].
 void grav(hipStream_t s, const Plan& pl, const Force& f, const PState<double>& p, const float4* A, const float4* B,
           double* out);
+// Queue form (acceleration or potential).  counters = 2 ints, zero before
+// the first launch; the launch leaves them at zero.
+void queue(hipStream_t s, const QueuePlan& q, const Force& f, bool potential, const PState<double>& p,
+           const float4* A, const float4* B, const WorkItem* items, int* counters, double* out);
+void reset_counters(hipStream_t s, int* counters);
 
-// Sum `slices` partial blocks of `comps` components into dst (comps*n).
-void reduce_slices(hipStream_t s, int64_t n, int comps, int slices, const double* part, double* dst);
+// Sum the slice partials of `comps` components into dst (comps*n); the slice
+// count is `slices`, or ns[i] per particle when ns is non-null.
+void reduce_slices(hipStream_t s, int64_t n, int comps, int slices, const unsigned char* ns, const double* part,
+                   double* dst);
 
 // predict_step_ (nbody.cu:122-152); with A/B non-null also repacks the
-// mixed-kernel source records from the new positions.
+// FP32 source records from the new positions.
 template <class real>
 void predict(hipStream_t s, const PState<real>& p, real dt, float4* A, float4* B);
-// correct_step_ (nbody.cu:159-181) on a[] (after summing `slices` partials
+// correct_step_ (nbody.cu:159-181) on a[] (after summing the slice partials
 // from `part` into a[] when part != nullptr).
-void correct_reduce(hipStream_t s, const PState<double>& p, double dt, int slices, const double* part);
+void correct_reduce(hipStream_t s, const PState<double>& p, double dt, int slices, const unsigned char* ns,
+                    const double* part);
 template <class real> void correct(hipStream_t s, const PState<real>& p, real dt);
 
 // Conservation sums (nbody.cu:246-291) as FP64 per-block partials:
-// partial[b*8 + q], q = {Ek*2, Ep*2, Px, Py, Pz, Lx, Ly, Lz}.  Returns the
+// partial[b*8 + q], q = {2 Ek, 2 Ep, Px, Py, Pz, Lx, Ly, Lz}.  Returns the
 // number of blocks written (at most conservation_blocks(n)).
 int64_t conservation_blocks(int64_t n);
 template <class real> int64_t conservation(hipStream_t s, const PState<real>& p, double* partial);
+// Same sums per system of an ensemble: out[g*8 + q], offsets on the device.
+void conservation_segments(hipStream_t s, const PState<double>& p, int k, const int64_t* offsets, double* out);
 
 }  // namespace nbody
 

@@ -207,6 +207,8 @@ void params_default(rrg_params* p) {
     p->boundary = RRG_BC_PERIODIC;
     p->fixed_exponent = 0;
     p->device = 0;
+    p->use_graphs = 1;
+    p->autotune = 0;
 }
 
 // validate_dependent_fields (io_json.cpp:170-186).  The CFL step is formed
@@ -297,6 +299,14 @@ void params_parse_json(const std::string& path, rrg_params* p, rrg_ini_params* i
         }
         opt_num(*m, "fixed_exponent", p->fixed_exponent);
         opt_num(*m, "device", p->device);
+        if (const JValue* v = m->find("graphs")) {
+            if (v->kind != JValue::Bool) fail(RRG_ERR_IO, "mi355x.graphs must be true or false");
+            p->use_graphs = v->b ? 1 : 0;
+        }
+        if (const JValue* v = m->find("autotune")) {
+            if (v->kind != JValue::Bool) fail(RRG_ERR_IO, "mi355x.autotune must be true or false");
+            p->autotune = v->b ? 1 : 0;
+        }
     }
     params_finalize(p);
 }

@@ -24,7 +24,7 @@ def default_params() -> L.Params:
 
 
 def make_params(*, dt, time_max=None, dt_save=None, grid=None, wave=None, real="fp64", force="fp64",
-                boundary="periodic", fixed_exponent=0, device=0) -> L.Params:
+                boundary="periodic", fixed_exponent=0, device=0, graphs=True, autotune=False) -> L.Params:
     """Build an ``rrg_params``.  ``grid`` = dict(nx, dx, bc_frac); ``wave`` =
     dict(diss_base, diss_extra, wave_speed[, setup_iterations, iterations,
     tolerance, steady_check_step, steady_hold_steps])."""
@@ -46,6 +46,8 @@ def make_params(*, dt, time_max=None, dt_save=None, grid=None, wave=None, real="
     p.boundary = {"periodic": L.BC_PERIODIC, "zero_dirichlet": L.BC_ZERO_DIRICHLET}[boundary]
     p.fixed_exponent = fixed_exponent
     p.device = device
+    p.use_graphs = 1 if graphs else 0
+    p.autotune = 1 if autotune else 0
     L.check(L.lib().rrg_params_finalize(ctypes.byref(p)))
     return p
 
@@ -69,16 +71,26 @@ def read_ini(ini: L.IniParams, base_dir: str | None = None):
 
 
 class Simulation:
-    """One solver context on one GPU."""
+    """One solver context on one GPU.
 
-    def __init__(self, params: L.Params, pos, vel, mass, soft2=None):
+    ``offsets`` (k+1 entries) makes it an ensemble of k independent systems
+    (direct-sum model): system g is particles offsets[g]:offsets[g+1]."""
+
+    def __init__(self, params: L.Params, pos, vel, mass, soft2=None, offsets=None):
         self.n = len(mass)
         self.params = params
         pos = L.f64(pos, (self.n, 3)); vel = L.f64(vel, (self.n, 3)); mass = L.f64(mass)
         s2 = None if soft2 is None else L.f64(soft2)
         h = L._vp()
-        L.check(L.lib().rrg_create(ctypes.byref(params), self.n, L.ptr(pos), L.ptr(vel), L.ptr(mass), L.ptr(s2),
-                                   ctypes.byref(h)))
+        if offsets is None:
+            self.k = 1
+            L.check(L.lib().rrg_create(ctypes.byref(params), self.n, L.ptr(pos), L.ptr(vel), L.ptr(mass), L.ptr(s2),
+                                       ctypes.byref(h)))
+        else:
+            off = np.ascontiguousarray(offsets, dtype=np.int64)
+            self.k = len(off) - 1
+            L.check(L.lib().rrg_create_ensemble(ctypes.byref(params), self.k, L.ptr(off), L.ptr(pos), L.ptr(vel),
+                                                L.ptr(mass), L.ptr(s2), ctypes.byref(h)))
         self._h = h
 
     # IntegrationSolver interface -----------------------------------------
@@ -109,6 +121,13 @@ class Simulation:
         return {"time": c.time, "Ek": c.Ek, "Ep": c.Ep, "E": c.Ek + c.Ep,
                 "P": np.array(c.momentum[:]), "L": np.array(c.angular[:])}
 
+    def conservation_systems(self) -> list:
+        """Per-system conservation sums (one dict per system)."""
+        arr = (L.Conservation * self.k)()
+        L.check(L.lib().rrg_get_conservation_systems(self._h, self.k, arr))
+        return [{"time": c.time, "Ek": c.Ek, "Ep": c.Ep, "E": c.Ek + c.Ep, "P": np.array(c.momentum[:]),
+                 "L": np.array(c.angular[:])} for c in arr]
+
     def grid(self) -> dict:
         nx = self.params.grid.nx
         m = np.empty(nx ** 3); phi = np.empty(nx ** 3); acc = np.empty((nx ** 3, 3))
@@ -118,8 +137,7 @@ class Simulation:
     def stats(self) -> dict:
         s = L.Stats()
         L.check(L.lib().rrg_get_stats(self._h, ctypes.byref(s)))
-        return {"n": s.n, "steps": s.steps, "last_wave_iterations": s.last_wave_iterations,
-                "fixed_exponent": s.fixed_exponent, "time": s.time}
+        return {k: getattr(s, k) for k, _ in L.Stats._fields_}
 
     def synchronize(self) -> None:
         L.check(L.lib().rrg_synchronize(self._h))

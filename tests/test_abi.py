@@ -149,3 +149,12 @@ def test_bad_arguments_are_rejected_before_device_work(rrg):
     with pytest.raises(rrg.RRGError):
         rrg.Simulation(p, np.full((2, 3), np.nan), np.zeros((2, 3)), np.ones(2), np.ones(2))
     assert rrg.lib().rrg_destroy(None) == 0
+
+
+def test_ensemble_offsets_validated_before_device_work(rrg):
+    p = rrg.make_params(dt=1e-3)
+    pos = np.zeros((6, 3)); vel = np.zeros((6, 3)); m = np.ones(6); s2 = np.ones(6)
+    for off in ([1, 3, 6], [0, 3, 3, 6], [0, 4, 2, 6]):
+        with pytest.raises(rrg.RRGError) as ei:
+            rrg.Simulation(p, pos, vel, m, s2, offsets=off)
+        assert ei.value.code == rrg._lib.RRG_ERR_INVALID
