@@ -175,6 +175,51 @@ def ensemble_bench(rrg, ics, steps=100):
     }
 
 
+def adjoint_bench(rrg, ics, n=65_536, steps=200, every=20):
+    """Plummer N=65,536 (seed 0), eps=0.01, dt=1e-3, 200 steps; target = the
+    same ICs with masses x exp(0.05 N(0,1)) (seed 3) run forward; loss = mean
+    |x_T - x_T^target|^2; gradients w.r.t. x0, v0 and m with checkpoints every
+    20 steps.  t_forward = the recorded forward run, t_adjoint = rrg_adjoint.
+    Gradient error: the mixed-kind gradients against the FP64-kind gradients
+    of the same run (the FP64 path is pinned to the CPU FP64 reverse-mode
+    gradient in tests/test_gpu_adjoint.py)."""
+    pos, vel, m = ics.plummer(n, seed=0)
+    s2 = np.full(n, 0.01 ** 2)
+    mt = m * np.exp(0.05 * np.random.default_rng(3).standard_normal(n))
+    p = rrg.make_params(dt=1e-3, force="mixed")
+    with rrg.Simulation(p, pos, vel, mt, s2) as sim:
+        sim.precalc()
+        sim.calc_step(steps)
+        target = sim.state()["pos"]
+    out = {}
+    for force in ("mixed", "fp64"):
+        p = rrg.make_params(dt=1e-3, force=force)
+        with rrg.Simulation(p, pos, vel, m, s2) as sim:
+            sim.precalc()
+            sim.adjoint_begin(every)
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.calc_step(steps)
+            sim.synchronize()
+            t1 = time.perf_counter()
+            xT = sim.state()["pos"]
+            loss = float(np.mean(np.sum((xT - target) ** 2, axis=1)))
+            t2 = time.perf_counter()
+            g = sim.adjoint(2 * (xT - target) / n)
+            t3 = time.perf_counter()
+        out[force] = {"t_forward_s": t1 - t0, "t_adjoint_s": t3 - t2, "adjoint_over_forward": (t3 - t2) / (t1 - t0),
+                      "loss": loss, "grads": g}
+    ref = out["fp64"]["grads"]
+    errs = {}
+    for name, a, b in zip(("x0", "v0", "m"), out["mixed"]["grads"], ref):
+        errs[name] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    for f in out.values():
+        del f["grads"]
+    return {"mixed": out["mixed"], "fp64": out["fp64"], "mixed_grad_rel_err_vs_fp64": errs,
+            "config": {"workload": "Plummer N=65536, eps=0.01, dt=1e-3, 200 steps, checkpoint every 20, "
+                                   "loss = mean |x_T - x_T(target masses)|^2", "n": n, "steps": steps}}
+
+
 def cpu_baseline(ics):
     """The CPU oracle (FP32 restatement) on a bounded sample of the headline
     workload: the full 1e6-particle / 128^3 problem, 2 PM steps (each with the
@@ -247,6 +292,7 @@ def main():
             d[k]["fp64_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP64_PEAK_TFLOPS
         line["direct_sum"] = d
         line["ensemble"] = ensemble_bench(rrg, ics)
+        line["adjoint"] = adjoint_bench(rrg, ics)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ics)
     print(json.dumps(line), flush=True)

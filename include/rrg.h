@@ -240,6 +240,20 @@ int rrg_set_timing(rrg_ctx* ctx, int32_t enabled);
 /* Synchronize the context stream and return the accumulated totals. */
 int rrg_get_timing(rrg_ctx* ctx, rrg_timing* out);
 
+/* ---- adjoint of the direct-sum run (no counterpart in the reference) ----
+ * rrg_adjoint_begin marks the current state (after rrg_precalc) as step 0 of
+ * a recorded run; the following rrg_step calls store (x, v, a) checkpoints
+ * every `checkpoint_every` steps on the device.  rrg_adjoint then returns the
+ * gradients of a loss L(x_T, v_T) of the state after the recorded steps with
+ * respect to the initial positions, velocities and the masses, given
+ * dL/dx_T and dL/dv_T (either may be NULL = zero).  It recomputes each
+ * checkpoint segment forward and runs the transposed (reverse) Verlet step
+ * with one vector-Jacobian product of the pair forces per step.  Any output
+ * pointer may be NULL.  Single-system direct-sum contexts only. */
+int rrg_adjoint_begin(rrg_ctx* ctx, int32_t checkpoint_every);
+int rrg_adjoint(rrg_ctx* ctx, const double* gx_T, const double* gv_T, double* gx_0, double* gv_0,
+                double* gm);
+
 /* ---- stateless operators (one call = one kernel family on device 0) ----
  * Arrays of `real` are double for RRG_REAL_F64 and float for RRG_REAL_F32.
  * They are the reference's L2 free functions without the contexts. */

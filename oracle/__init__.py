@@ -264,3 +264,25 @@ def pm_run(pos, vel, mass, nsteps, *, nx, dx, dt, diss_base, wave_speed, setup_i
                                                  _int(nsteps), _p(acc), _p(phi2), _p(gacc), _p(gr), _p(iters))
     return {"pos": pos, "vel": vel, "acc": acc, "phi": phi2[:ncell], "phi_prev": phi2[ncell:],
             "grid_acc": gacc, "grav": gr, "iterations": iters}
+
+
+def nbody_adjoint(pos, vel, mass, soft2, dt, nsteps, gx_T, gv_T=None):
+    """Reverse-mode gradient of nbody_run: returns (dL/dx0, dL/dv0, dL/dm)
+    for a loss with dL/dx_T = gx_T and dL/dv_T = gv_T.  FP64."""
+    pos = _c(pos, np.float64); vel = _c(vel, np.float64)
+    mass = _c(mass, np.float64); soft2 = _c(soft2, np.float64)
+    n = len(mass)
+    gx = _c(gx_T, np.float64)
+    gv = np.zeros((n, 3)) if gv_T is None else _c(gv_T, np.float64)
+    gx0 = np.empty((n, 3)); gv0 = np.empty((n, 3)); gm = np.empty(n)
+    lib().orc_nbody_adjoint_f64(_i64(n), _p(pos), _p(vel), _p(mass), _p(soft2), ctypes.c_double(dt),
+                                _int(nsteps), _p(gx), _p(gv), _p(gx0), _p(gv0), _p(gm))
+    return gx0, gv0, gm
+
+
+def vjp(pos, mass, soft2, u):
+    """(J^T u, dm) of the direct-sum acceleration at pos.  FP64."""
+    pos = _c(pos, np.float64); mass = _c(mass, np.float64); soft2 = _c(soft2, np.float64); u = _c(u, np.float64)
+    g = np.empty_like(pos); gm = np.empty(len(mass))
+    lib().orc_vjp_f64(_i64(len(mass)), _p(pos), _p(mass), _p(soft2), _p(u), _p(g), _p(gm))
+    return g, gm

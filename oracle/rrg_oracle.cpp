@@ -516,6 +516,76 @@ void nbody_run(int64_t n, real* pos, real* vel, const real* mass, const real* so
     if (acc_out) std::memcpy(acc_out, acc.data(), sizeof(real) * 3 * n);
 }
 
+// ---------------------------------------------------------------------------
+// Reverse-mode gradient of the N-body run above (no reference counterpart):
+// the transpose of predict/accel/correct, FP64, all positions kept.  Pinned
+// by central finite differences of nbody_run in the tests.
+// VJP at x: g_k = sum_j T(x_j - x_k)(m_k u_j - m_j u_k) with
+// T(d) = r^-3 I - 3 r^-5 d d^T (r^2 = |d|^2 + h_kj), and the mass gradient
+// gm_k = sum_i u_i . K(x_k - x_i), K(d) = d r^-3.
+// ---------------------------------------------------------------------------
+void vjp(int64_t n, const double* x, const double* m, const double* s2, const double* u, double* g, double* gm) {
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        double gk[3] = {0, 0, 0}, gmk = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == k) continue;
+            double d[3], w[3];
+            for (int c = 0; c < 3; ++c) {
+                d[c] = x[3 * j + c] - x[3 * k + c];
+                w[c] = m[k] * u[3 * j + c] - m[j] * u[3 * k + c];
+            }
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + 0.5 * (s2[k] + s2[j]);
+            const double r = std::sqrt(r2);
+            const double inv3 = 1.0 / (r2 * r);
+            const double inv5 = inv3 / r2;
+            const double dw = d[0] * w[0] + d[1] * w[1] + d[2] * w[2];
+            for (int c = 0; c < 3; ++c) gk[c] += inv3 * w[c] - 3.0 * inv5 * dw * d[c];
+            // gm_k collects u_j . K(x_k - x_j) = -(u_j . d) r^-3
+            gmk -= (u[3 * j] * d[0] + u[3 * j + 1] * d[1] + u[3 * j + 2] * d[2]) * inv3;
+        }
+        for (int c = 0; c < 3; ++c) g[3 * k + c] = gk[c];
+        gm[k] = gmk;
+    }
+}
+
+void nbody_adjoint(int64_t n, const double* x0, const double* v0, const double* m, const double* s2, double dt,
+                   int nsteps, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gmo) {
+    std::vector<double> xs(size_t(nsteps + 1) * 3 * n), x(3 * n), v(3 * n), a(3 * n), vp(3 * n);
+    std::memcpy(x.data(), x0, sizeof(double) * 3 * n);
+    std::memcpy(v.data(), v0, sizeof(double) * 3 * n);
+    std::memcpy(xs.data(), x0, sizeof(double) * 3 * n);
+    accel<double>(n, x.data(), m, s2, a.data());
+    for (int s = 0; s < nsteps; ++s) {
+        predict<double>(n, a.data(), v.data(), x.data(), vp.data(), dt);
+        accel<double>(n, x.data(), m, s2, a.data());
+        correct<double>(n, a.data(), v.data(), vp.data(), dt);
+        std::memcpy(xs.data() + size_t(s + 1) * 3 * n, x.data(), sizeof(double) * 3 * n);
+    }
+    std::vector<double> X(gxT, gxT + 3 * n), V(gvT, gvT + 3 * n), A(3 * n, 0.0), U(3 * n), g(3 * n), gm(n, 0.0),
+        gmt(n);
+    for (int s = nsteps - 1; s >= 0; --s) {
+        // step s: (x_s, v_s) -> (x_{s+1}, v_{s+1}); adjoints (X, V, A) of (x_{s+1}, v_{s+1}, a_{s+1})
+        for (int64_t q = 0; q < 3 * n; ++q) U[q] = A[q] + 0.5 * dt * V[q];       // correct_step_
+        vjp(n, xs.data() + size_t(s + 1) * 3 * n, m, s2, U.data(), g.data(), gmt.data());  // a_{s+1} = F(x_{s+1})
+        for (int64_t i = 0; i < n; ++i) gm[i] += gmt[i];
+        for (int64_t q = 0; q < 3 * n; ++q) {
+            const double Xt = X[q] + g[q];
+            const double vstar = 0.5 * V[q] + 0.5 * dt * Xt;  // adjoint of v*
+            const double vbar = 0.5 * V[q] + 0.5 * dt * Xt + vstar;
+            A[q] = dt * vstar;                                 // adjoint of a_s
+            V[q] = vbar;
+            X[q] = Xt;
+        }
+    }
+    vjp(n, x0, m, s2, A.data(), g.data(), gmt.data());  // a_0 = F(x_0)
+    for (int64_t q = 0; q < 3 * n; ++q) X[q] += g[q];
+    for (int64_t i = 0; i < n; ++i) gm[i] += gmt[i];
+    std::memcpy(gx0, X.data(), sizeof(double) * 3 * n);
+    std::memcpy(gv0, V.data(), sizeof(double) * 3 * n);
+    std::memcpy(gmo, gm.data(), sizeof(double) * n);
+}
+
 struct PmConfig {
     int nx;
     double dx, bc_frac;
@@ -645,6 +715,16 @@ void pm_run(const PmConfig& cf, int64_t n, real* pos, real* vel, const real* mas
                                      const real* m, int nsteps, real* acc, real* phi2,             \
                                      real* gacc, real* gr, int* iters) {                           \
         pm_run<real>(*cf, n, pos, vel, m, nsteps, acc, phi2, gacc, gr, iters); }
+
+ORC_EXPORT void orc_nbody_adjoint_f64(int64_t n, const double* x0, const double* v0, const double* m,
+                                      const double* s2, double dt, int nsteps, const double* gxT,
+                                      const double* gvT, double* gx0, double* gv0, double* gm) {
+    nbody_adjoint(n, x0, v0, m, s2, dt, nsteps, gxT, gvT, gx0, gv0, gm);
+}
+ORC_EXPORT void orc_vjp_f64(int64_t n, const double* x, const double* m, const double* s2, const double* u,
+                            double* g, double* gm) {
+    vjp(n, x, m, s2, u, g, gm);
+}
 
 ORC_INSTANTIATE(f64, double)
 ORC_INSTANTIATE(f32, float)

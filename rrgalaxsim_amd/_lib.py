@@ -96,6 +96,8 @@ SIGNATURES = {
                                            ctypes.POINTER(_vp)]),
     "rrg_destroy": (ctypes.c_int, [_vp]),
     "rrg_get_conservation_systems": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(Conservation)]),
+    "rrg_adjoint_begin": (ctypes.c_int, [_vp, _i32]),
+    "rrg_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rrg_precalc": (ctypes.c_int, [_vp]),
     "rrg_step": (ctypes.c_int, [_vp, _i32]),
     "rrg_sync": (ctypes.c_int, [_vp]),

@@ -263,6 +263,10 @@ struct Solver {
     virtual void conservation_systems(int k, rrg_conservation* out) = 0;
     virtual void get_grid(double* mass, double* phi, double* acc) = 0;
     virtual void stats(rrg_stats* out) = 0;
+    virtual void adjoint_begin(int) { fail(RRG_ERR_STATE, "adjoint: supported by direct-sum contexts only"); }
+    virtual void adjoint(const double*, const double*, double*, double*, double*) {
+        fail(RRG_ERR_STATE, "adjoint: supported by direct-sum contexts only");
+    }
 };
 
 void fill_conservation(const double* q, double time, rrg_conservation* out) {
@@ -386,8 +390,66 @@ public:
             else enqueue_step();
             ++steps_;
             time_ += prm_.sim.dt_dynamics;
+            if (recording_) {
+                ++rec_steps_;
+                if (rec_steps_ % ck_every_ == 0) store_checkpoint();
+            }
         }
         if (nsteps > 0) grav_actual_ = false;
+    }
+
+    void adjoint_begin(int every) override {
+        require(!ensemble_, "rrg_adjoint_begin: not supported for ensembles");
+        require(every >= 1, "rrg_adjoint_begin: checkpoint_every must be >= 1");
+        if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_adjoint_begin before rrg_precalc");
+        ck_every_ = every;
+        rec_steps_ = 0;
+        ck_.clear();
+        recording_ = true;
+        store_checkpoint();
+    }
+
+    // Reverse sweep over the recorded steps (see adjoint_kernels.hip).
+    void adjoint(const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) override {
+        if (!recording_) fail(RRG_ERR_STATE, "rrg_adjoint without rrg_adjoint_begin");
+        const int64_t n = P_.n;
+        const double dt = prm_.sim.dt_dynamics;
+        const bool f32 = nbody::is_fp32_kind(kind_);
+        adj_alloc();
+        std::vector<double> zero(3 * n, 0.0);
+        upload_vec<double>(s_, n, gxT ? gxT : zero.data(), Xb_.p);
+        upload_vec<double>(s_, n, gvT ? gvT : zero.data(), Vb_.p);
+        zero_async(s_, Ab_.p, Ab_.bytes());
+        zero_async(s_, gm_.p, gm_.bytes());
+        const int T = int(rec_steps_);
+        const int K = ck_every_;
+        const int nseg = (T + K - 1) / K;
+        const size_t v3 = sizeof(double) * 3 * n;
+        for (int c = nseg - 1; c >= 0; --c) {
+            const int s0 = c * K, len = std::min(T, s0 + K) - s0;
+            // recompute the segment forward on the scratch state
+            const DBuf<double>& ck = ck_[c];
+            RRG_HIP(hipMemcpyAsync(R_.X.p, ck.p, v3, hipMemcpyDeviceToDevice, s_));
+            RRG_HIP(hipMemcpyAsync(R_.V.p, ck.p + 3 * n, v3, hipMemcpyDeviceToDevice, s_));
+            RRG_HIP(hipMemcpyAsync(R_.A.p, ck.p + 6 * n, v3, hipMemcpyDeviceToDevice, s_));
+            P_.swap(R_);
+            RRG_HIP(hipMemcpyAsync(xs_.p, P_.X.p, v3, hipMemcpyDeviceToDevice, s_));
+            if (f32) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
+            for (int q = 1; q <= len; ++q) {
+                enqueue_step();
+                RRG_HIP(hipMemcpyAsync(xs_.p + size_t(q) * 3 * n, P_.X.p, v3, hipMemcpyDeviceToDevice, s_));
+            }
+            P_.swap(R_);
+            // reverse steps n = s0+q, q = len-1 .. 0, each at x_{n+1}
+            for (int q = len - 1; q >= 0; --q) reverse_step(xs_.p + size_t(q + 1) * 3 * n, dt, false);
+        }
+        // a_0 = F(x_0): the closing product at the initial positions
+        reverse_step(ck_[0].p, 0.0, true);
+        if (f32) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
+        if (gx0) download_vec<double>(s_, n, Xb_.p, gx0);
+        if (gv0) download_vec<double>(s_, n, Vb_.p, gv0);
+        if (gm) download_scalar<double>(s_, n, gm_.p, gm);
+        RRG_HIP(hipStreamSynchronize(s_));
     }
 
     // The reference returns early without a grid and leaves the potential
@@ -410,6 +472,7 @@ public:
         if (vel) upload_vec<double>(s_, P_.n, vel, P_.V.p);
         if (pos && nbody::is_fp32_kind(kind_)) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
         grav_actual_ = false;
+        recording_ = false;  // a recorded run no longer describes this state
     }
 
     void conservation(rrg_conservation* out) override {
@@ -558,6 +621,47 @@ private:
         g_tune_cache[key] = best;
     }
 
+    void store_checkpoint() {
+        const int64_t n = P_.n;
+        ck_.emplace_back(9 * n);
+        DBuf<double>& ck = ck_.back();
+        const size_t v3 = sizeof(double) * 3 * n;
+        RRG_HIP(hipMemcpyAsync(ck.p, P_.X.p, v3, hipMemcpyDeviceToDevice, s_));
+        RRG_HIP(hipMemcpyAsync(ck.p + 3 * n, P_.V.p, v3, hipMemcpyDeviceToDevice, s_));
+        RRG_HIP(hipMemcpyAsync(ck.p + 6 * n, P_.A.p, v3, hipMemcpyDeviceToDevice, s_));
+    }
+
+    void adj_alloc() {
+        const int64_t n = P_.n;
+        if (R_.n != n) {
+            R_.alloc(n);
+            RRG_HIP(hipMemcpyAsync(R_.M.p, P_.M.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s_));
+            RRG_HIP(hipMemcpyAsync(R_.S2.p, P_.S2.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s_));
+            zero_async(s_, R_.W.p, R_.W.bytes());
+            zero_async(s_, R_.G.p, R_.G.bytes());
+            Xb_.alloc(3 * n); Vb_.alloc(3 * n); Ab_.alloc(3 * n); At_.alloc(3 * n); gm_.alloc(n);
+            vplan_ = nbody::plan_direct(n, kind_, 0, std::min(plan_.ib, 2));
+            vpart_.alloc(size_t(vplan_.slices) * 4 * n);
+            if (nbody::is_fp32_kind(kind_)) {
+                Uf_.alloc(n);
+                Ap_.alloc(n);
+                Bp_.alloc(n);
+            }
+        }
+        if (xs_.n < size_t(ck_every_ + 1) * 3 * n) xs_.alloc(size_t(ck_every_ + 1) * 3 * n);
+    }
+
+    // One reverse Verlet step at positions x (the state after the forward
+    // step); `last` is the closing product for a_0 = F(x_0).
+    void reverse_step(const double* x, double dt, bool last) {
+        const int64_t n = P_.n;
+        const bool f32 = nbody::is_fp32_kind(kind_);
+        adjoint::pre(s_, n, dt, Vb_.p, Ab_.p, At_.p, f32 ? Uf_.p : nullptr);
+        if (f32) adjoint::pack_pos(s_, n, x, P_.M.p, P_.S2.p, Ap_.p, Bp_.p);
+        adjoint::vjp(s_, vplan_, kind_, n, x, P_.M.p, P_.S2.p, At_.p, Ap_.p, Bp_.p, Uf_.p, vpart_.p);
+        adjoint::post(s_, n, prm_.sim.dt_dynamics, vplan_.slices, vpart_.p, Xb_.p, Vb_.p, Ab_.p, gm_.p, last);
+    }
+
     void enqueue_step() {
         const double dt = prm_.sim.dt_dynamics;
         int t = timer.start(s_, RRG_STAGE_PREDICT);
@@ -622,6 +726,16 @@ private:
     int tune_candidates_ = 0;
     int64_t steps_ = 0;
     double time_ = 0.0;
+    // adjoint
+    bool recording_ = false;
+    int ck_every_ = 1;
+    int64_t rec_steps_ = 0;
+    std::vector<DBuf<double>> ck_;  // (x, v, a) every ck_every_ steps
+    Particles<double> R_;           // scratch state for segment recomputation
+    DBuf<double> xs_;               // positions of one segment
+    DBuf<double> Xb_, Vb_, Ab_, At_, gm_, vpart_;
+    DBuf<float4> Uf_, Ap_, Bp_;
+    nbody::Plan vplan_;
 };
 
 // Particle-mesh strategy (RectWaveIntegrationSolver), state in `real`.
@@ -1126,6 +1240,15 @@ RRG_API int rrg_get_conservation(rrg_ctx* ctx, rrg_conservation* out) {
         require(out != nullptr, "rrg_get_conservation: null output");
         ctx->solver->conservation(out);
     });
+}
+
+RRG_API int rrg_adjoint_begin(rrg_ctx* ctx, int32_t checkpoint_every) {
+    RRG_CTX_CALL(ctx, ctx->solver->adjoint_begin(checkpoint_every));
+}
+
+RRG_API int rrg_adjoint(rrg_ctx* ctx, const double* gx_T, const double* gv_T, double* gx_0, double* gv_0,
+                        double* gm) {
+    RRG_CTX_CALL(ctx, ctx->solver->adjoint(gx_T, gv_T, gx_0, gv_0, gm));
 }
 
 RRG_API int rrg_get_conservation_systems(rrg_ctx* ctx, int32_t k, rrg_conservation* out) {

@@ -148,6 +148,23 @@ void conservation_segments(hipStream_t s, const PState<double>& p, int k, const 
 
 }  // namespace nbody
 
+// ----------------------------------------------------------------- adjoint
+namespace adjoint {
+
+// J^T u and the mass gradient at positions X (SoA) for the force kind `kind`;
+// out[(slice*4 + c)*n + i], c = 0..2 (J^T u), 3 (dL/dm).  FP32 kinds read the
+// packed records A/B (positions) and Uf (u as float4); FP64 kinds X and U.
+void vjp(hipStream_t s, const nbody::Plan& pl, int kind, int64_t n, const double* X, const double* M,
+         const double* S2, const double* U, const float4* A, const float4* B, const float4* Uf, double* out);
+// At = Aa + dt V / 2 (and its packed FP32 copy when Uf is non-null).
+void pre(hipStream_t s, int64_t n, double dt, const double* V, const double* Aa, double* At, float4* Uf);
+// Sum VJP slices; X += J^T A~, gm += dm; unless `last`: V += dt X, Aa = dt V_old/2 + dt^2 X/2.
+void post(hipStream_t s, int64_t n, double dt, int slices, const double* part, double* X, double* V, double* Aa,
+          double* gm, bool last);
+void pack_pos(hipStream_t s, int64_t n, const double* X, const double* M, const double* S2, float4* A, float4* B);
+
+}  // namespace adjoint
+
 // ------------------------------------------------------------ particle-mesh
 namespace pm {
 

@@ -142,6 +142,19 @@ class Simulation:
     def synchronize(self) -> None:
         L.check(L.lib().rrg_synchronize(self._h))
 
+    def adjoint_begin(self, checkpoint_every: int = 20) -> None:
+        """Record the following steps for rrg_adjoint (checkpoints every K steps)."""
+        L.check(L.lib().rrg_adjoint_begin(self._h, int(checkpoint_every)))
+
+    def adjoint(self, gx_T=None, gv_T=None):
+        """Gradients (dL/dx0, dL/dv0, dL/dm) of the recorded run for the given
+        dL/dx_T and dL/dv_T."""
+        gx = None if gx_T is None else L.f64(gx_T, (self.n, 3))
+        gv = None if gv_T is None else L.f64(gv_T, (self.n, 3))
+        gx0 = np.empty((self.n, 3)); gv0 = np.empty((self.n, 3)); gm = np.empty(self.n)
+        L.check(L.lib().rrg_adjoint(self._h, L.ptr(gx), L.ptr(gv), L.ptr(gx0), L.ptr(gv0), L.ptr(gm)))
+        return gx0, gv0, gm
+
     def set_timing(self, enabled: bool) -> None:
         """Per-stage HIP-event timing on the context stream (resets totals)."""
         L.check(L.lib().rrg_set_timing(self._h, 1 if enabled else 0))

@@ -135,3 +135,55 @@ def test_tiny_n(orc, n):
     assert np.all(np.isfinite(a))
     if n == 1:
         assert np.all(a == 0)
+
+
+def _loss_grad_fd(orc, pos, vel, m, s2, dt, T, target, what, idx, c, eps):
+    def loss(p, v, mm):
+        xT, _, _, _ = orc.nbody_run(p, v, mm, s2, dt, T)
+        return np.sum((xT - target) ** 2)
+    p1, p2 = pos.copy(), pos.copy(); v1, v2 = vel.copy(), vel.copy(); m1, m2 = m.copy(), m.copy()
+    if what == "x":
+        p1[idx, c] += eps; p2[idx, c] -= eps
+    elif what == "v":
+        v1[idx, c] += eps; v2[idx, c] -= eps
+    else:
+        m1[idx] += eps; m2[idx] -= eps
+    return (loss(p1, v1, m1) - loss(p2, v2, m2)) / (2 * eps)
+
+
+def test_vjp_matches_finite_difference_of_accel(orc):
+    """J^T u and the mass gradient of the acceleration against central
+    differences of the oracle's acceleration."""
+    rng = np.random.default_rng(0)
+    n = 7
+    pos = rng.standard_normal((n, 3)); m = rng.uniform(0.5, 1.5, n); s2 = rng.uniform(0.01, 0.05, n)
+    u = rng.standard_normal((n, 3))
+    g, gm = orc.vjp(pos, m, s2, u)
+    eps = 1e-6
+    for k in range(n):
+        for c in range(3):
+            p1 = pos.copy(); p2 = pos.copy()
+            p1[k, c] += eps; p2[k, c] -= eps
+            fd = np.sum(u * (orc.accel(p1, m, s2) - orc.accel(p2, m, s2))) / (2 * eps)
+            assert abs(fd - g[k, c]) < 1e-7 * (1 + abs(fd))
+        m1 = m.copy(); m2 = m.copy()
+        m1[k] += eps; m2[k] -= eps
+        fd = np.sum(u * (orc.accel(pos, m1, s2) - orc.accel(pos, m2, s2))) / (2 * eps)
+        assert abs(fd - gm[k]) < 1e-7 * (1 + abs(fd))
+
+
+def test_adjoint_matches_finite_differences(orc):
+    """Gradient of L = sum |x_T - target|^2 w.r.t. x0, v0 and m, 12 steps."""
+    pos, vel, m = ics.plummer(24, seed=5)
+    s2 = np.full(24, 0.05)
+    dt, T = 0.01, 12
+    target = pos + 0.1
+    xT, _, _, _ = orc.nbody_run(pos, vel, m, s2, dt, T)
+    gx0, gv0, gm = orc.nbody_adjoint(pos, vel, m, s2, dt, T, 2 * (xT - target))
+    for what, grad in (("x", gx0), ("v", gv0)):
+        for idx, c in ((0, 0), (5, 2), (17, 1)):
+            fd = _loss_grad_fd(orc, pos, vel, m, s2, dt, T, target, what, idx, c, 1e-6)
+            assert abs(fd - grad[idx, c]) < 1e-6 * (1 + abs(fd)), (what, idx, c, fd, grad[idx, c])
+    for idx in (0, 9, 23):
+        fd = _loss_grad_fd(orc, pos, vel, m, s2, dt, T, target, "m", idx, 0, 1e-7)
+        assert abs(fd - gm[idx]) < 1e-5 * (1 + abs(fd)), (idx, fd, gm[idx])
