@@ -386,8 +386,7 @@ public:
     void step(int nsteps) override {  // rrgsim_integration_nbody.cuh:50-56
         if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_step before rrg_precalc");
         for (int k = 0; k < nsteps; ++k) {
-            if (prm_.use_graphs && !timer.on()) graphs_.run(s_, 0, [&] { enqueue_step(); });
-            else enqueue_step();
+            enqueue_step(prm_.use_graphs != 0);
             ++steps_;
             time_ += prm_.sim.dt_dynamics;
             if (recording_) {
@@ -436,7 +435,7 @@ public:
             RRG_HIP(hipMemcpyAsync(xs_.p, P_.X.p, v3, hipMemcpyDeviceToDevice, s_));
             if (f32) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
             for (int q = 1; q <= len; ++q) {
-                enqueue_step();
+                enqueue_step(false);
                 RRG_HIP(hipMemcpyAsync(xs_.p + size_t(q) * 3 * n, P_.X.p, v3, hipMemcpyDeviceToDevice, s_));
             }
             P_.swap(R_);
@@ -662,18 +661,24 @@ private:
         adjoint::post(s_, n, prm_.sim.dt_dynamics, vplan_.slices, vpart_.p, Xb_.p, Vb_.p, Ab_.p, gm_.p, last);
     }
 
-    void enqueue_step() {
+    // One step as three segments (predictor, force, corrector); with `graph`
+    // each segment is replayed from its captured graph, and stage timing
+    // events sit between the segment launches.
+    template <class F> void segment(int stage, bool graph, uint64_t key, F&& f) {
+        const int t = timer.start(s_, stage);
+        if (graph) graphs_.run(s_, key, f);
+        else f();
+        timer.stop(s_, t, 1);
+    }
+
+    void enqueue_step(bool graph) {
         const double dt = prm_.sim.dt_dynamics;
-        int t = timer.start(s_, RRG_STAGE_PREDICT);
-        nbody::predict<double>(s_, P_.view(), dt, A_.p, B_.p);
-        timer.stop(s_, t, 1);
-        t = timer.start(s_, RRG_STAGE_DIRECT);
-        accel(false);
-        timer.stop(s_, t, 1);
-        t = timer.start(s_, RRG_STAGE_CORRECT);
-        if (ensemble_) nbody::correct_reduce(s_, P_.view(), dt, 0, ns_.p, part_.p);
-        else nbody::correct_reduce(s_, P_.view(), dt, plan_.slices, nullptr, plan_.slices > 1 ? part_.p : nullptr);
-        timer.stop(s_, t, 1);
+        segment(RRG_STAGE_PREDICT, graph, 0, [&] { nbody::predict<double>(s_, P_.view(), dt, A_.p, B_.p); });
+        segment(RRG_STAGE_DIRECT, graph, 1, [&] { accel(false); });
+        segment(RRG_STAGE_CORRECT, graph, 2, [&] {
+            if (ensemble_) nbody::correct_reduce(s_, P_.view(), dt, 0, ns_.p, part_.p);
+            else nbody::correct_reduce(s_, P_.view(), dt, plan_.slices, nullptr, plan_.slices > 1 ? part_.p : nullptr);
+        });
     }
 
     void accel(bool reduce_now) {
@@ -830,23 +835,12 @@ public:
         if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_step before rrg_precalc");
         const real dt = (real)prm_.sim.dt_dynamics;
         // Without a steady-state check the step makes no host decision, so
-        // it is replayed from a captured graph (one per buffer arrangement).
-        const bool graph = prm_.use_graphs && !timer.on() && !(tol_ > (real)0);
+        // its segments are replayed from captured graphs (one set per buffer
+        // arrangement).
+        const bool graph = prm_.use_graphs && !(tol_ > (real)0);
         for (int k = 0; k < nsteps; ++k) {
             if (++since_sort_ >= kSortInterval) sort_particles();
-            if (graph) {
-                const uint64_t key = uint64_t(ip_) * 3 + uint64_t(ic_) + (P_.X.p == x_first_ ? 0 : 16);
-                const int ip0 = ip_, ic0 = ic_;
-                graphs_.run(s_, key, [&] {
-                    enqueue_step(dt);
-                    ip_ = ip0;
-                    ic_ = ic0;
-                });
-                rotate(prm_.wave.iterations);
-                last_iters_ = prm_.wave.iterations;
-            } else {
-                enqueue_step(dt);
-            }
+            enqueue_step(dt, graph);
             ++steps_;
             time_ += prm_.sim.dt_dynamics;
         }
@@ -922,17 +916,56 @@ public:
 private:
     static constexpr int kSortInterval = 8;  // steps between spatial re-sorts
 
-    void enqueue_step(real dt) {
-        deposit(true);
-        int t = timer.start(s_, RRG_STAGE_WAVE);
-        last_iters_ = solve(prm_.wave.iterations);
-        timer.stop(s_, t, last_iters_);
-        t = timer.start(s_, RRG_STAGE_GRADIENT);
-        pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
-        timer.stop(s_, t, 1);
-        t = timer.start(s_, RRG_STAGE_GATHER);
-        pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true);
-        timer.stop(s_, t, 1);
+    template <class F> void segment(int stage, int64_t launches, bool graph, uint64_t key, F&& f) {
+        const int t = timer.start(s_, stage);
+        if (graph) graphs_.run(s_, key, f);
+        else f();
+        timer.stop(s_, t, launches);
+    }
+
+    uint64_t graph_base() const {
+        return (uint64_t(ip_) * 3 + uint64_t(ic_)) * 16 + (P_.X.p == x_first_ ? 0 : 8);
+    }
+
+    // One PM step as segments: predictor + deposit, source, the wave
+    // iterations, gradient, gather + kick.  With `graph` every segment is
+    // replayed from its captured graph and stage timing events sit between
+    // segment launches.
+    void enqueue_step(real dt, bool graph) {
+        const uint64_t b0 = graph_base();
+        segment(RRG_STAGE_DEPOSIT, 1, graph, b0 + 0, [&] {
+            pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p);
+        });
+        segment(RRG_STAGE_SOURCE, 1, graph, b0 + 1, [&] {
+            pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, mass_.p,
+                                      src_.p);
+        });
+        const int iters = prm_.wave.iterations;
+        if (graph) {
+            segment(RRG_STAGE_WAVE, iters, true, b0 + 2, [&] {
+                int ip = ip_, ic = ic_;
+                wave_launches(iters, ip, ic);
+            });
+            rotate(iters);
+            last_iters_ = iters;
+        } else {
+            const int t = timer.start(s_, RRG_STAGE_WAVE);
+            last_iters_ = solve(iters);
+            timer.stop(s_, t, last_iters_);
+        }
+        const uint64_t b1 = graph_base();
+        segment(RRG_STAGE_GRADIENT, 1, graph, b1 + 3, [&] { pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p); });
+        segment(RRG_STAGE_GATHER, 1, graph, b1 + 4, [&] { pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true); });
+    }
+
+    // `iters` stencil launches with the three-buffer rotation, no check.
+    void wave_launches(int iters, int& ip, int& ic) {
+        for (int it = 0; it < iters; ++it) {
+            const int inx = 3 - ic - ip;
+            pm::wave_iteration<real>(s_, nx_, periodic_, k_, phi_[inx].p, phi_[ic].p, phi_[ip].p, src_.p, nullptr);
+            ip = ic;
+            ic = inx;
+        }
     }
 
     // The three-buffer rotation of `iters` iterations (wave.cu:271-272).

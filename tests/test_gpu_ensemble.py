@@ -103,7 +103,7 @@ def test_graph_replay_bitwise_equals_eager(rrg, ensemble, force):
                 sim.conservation()  # eager work and a synchronisation between replays
                 seq.append(sim.state())
             states[graphs] = (seq, sim.stats()["graph_launches"])
-    assert states[False][1] == 0 and states[True][1] == 8
+    assert states[False][1] == 0 and states[True][1] == 24  # 3 segment graphs per step
     for a, b in zip(states[False][0], states[True][0]):
         for k in ("pos", "vel", "acc"):
             assert np.array_equal(a[k], b[k]), k
@@ -124,7 +124,7 @@ def test_pm_graph_replay_bitwise_equals_eager(rrg):
                 sim.conservation()
                 seq.append((sim.state(), sim.grid()["phi"]))
             res[graphs] = (seq, sim.stats()["graph_launches"])
-    assert res[True][1] == 18
+    assert res[True][1] == 90  # 5 segment graphs per step
     for (a, pa), (b, pb) in zip(res[False][0], res[True][0]):
         assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
         assert np.array_equal(pa, pb)
