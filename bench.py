@@ -242,6 +242,28 @@ def cpu_baseline(ics):
                       "workload, 2 PM steps of 200 wave iterations, minus precalc"}
 
 
+def profile_pass(rrg, ics, steps):
+    """Short run for hardware-counter passes: `steps` PM steps of the headline
+    workload (setup iterations cut to 200), then a device copy of exactly
+    2**25 float32 (128 MiB read + 128 MiB written) whose FETCH_SIZE /
+    WRITE_SIZE per dispatch calibrate the counters' units."""
+    import torch
+
+    pos, vel, m = ics.exp_disk(PM["n"], seed=1)
+    wave = dict(diss_base=PM["diss_base"], diss_extra=PM["diss_extra"], wave_speed=C, setup_iterations=200,
+                iterations=PM["iterations"], tolerance=0.0)
+    p = rrg.make_params(dt=PM["dt"], grid=dict(nx=PM["nx"], dx=PM["dx"], bc_frac=0.0), wave=wave, real="fp32")
+    with rrg.Simulation(p, pos, vel, m) as sim:
+        sim.precalc()
+        sim.calc_step(steps)
+        sim.synchronize()
+    a = torch.ones(2 ** 25, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_pass": "done", "pm_steps": steps, "copy_bytes_each_way": 2 ** 27}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,7 +271,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true", help="short PM run + calibration copy for rocprofv3 --pmc")
     args = ap.parse_args()
+    if args.profile_pass:
+        import rrgalaxsim_amd as rrg
+        from rrgalaxsim_amd import ics
+
+        rrg.lib()
+        profile_pass(rrg, ics, args.steps)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the hot path has no "
               "multi-GPU implementation", file=sys.stderr)

@@ -107,6 +107,35 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
         }
         key = cic_fixed<real>(r, p.m[i], g, scale, q, c);
     }
+    // Bitonic sort of the wave's (key, lane) pairs with shuffles, so equal
+    // cells become adjacent lanes even where the particle order has drifted
+    // since the last global sort; each lane then takes the contributions of
+    // the particle it was sorted to.
+    {
+        unsigned sk = key;
+        int src = lane;
+#pragma unroll
+        for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const unsigned ok = __shfl_xor(sk, stride, 64);
+                const int os = __shfl_xor(src, stride, 64);
+                const bool up = (lane & size) == 0;
+                const bool lower = (lane & stride) == 0;
+                const bool other_smaller = ok < sk || (ok == sk && os < src);
+                if (lower == up ? other_smaller : !other_smaller) {
+                    sk = ok;
+                    src = os;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            q[k] = __shfl(q[k], src, 64);
+            c[k] = __shfl(c[k], src, 64);
+        }
+        key = sk;
+    }
     const unsigned kprev = __shfl_up(key, 1, 64);
     const unsigned knext = __shfl_down(key, 1, 64);
     int f = (lane == 0 || kprev != key) ? 1 : 0;  // run head
