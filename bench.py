@@ -244,10 +244,11 @@ def cpu_baseline(ics):
 
 def profile_pass(rrg, ics, steps):
     """Short run for hardware-counter passes: `steps` PM steps of the headline
-    workload (setup iterations cut to 200), then a device copy of exactly
-    2**25 float32 (128 MiB read + 128 MiB written) whose FETCH_SIZE /
-    WRITE_SIZE per dispatch calibrate the counters' units."""
-    import torch
+    workload (setup iterations cut to 200), then one device-to-device copy of
+    exactly 128 MiB through the HIP runtime (its copy kernel reads and writes
+    128 MiB), whose FETCH_SIZE / WRITE_SIZE per dispatch calibrate the
+    counters' units."""
+    import ctypes
 
     pos, vel, m = ics.exp_disk(PM["n"], seed=1)
     wave = dict(diss_base=PM["diss_base"], diss_extra=PM["diss_extra"], wave_speed=C, setup_iterations=200,
@@ -257,11 +258,20 @@ def profile_pass(rrg, ics, steps):
         sim.precalc()
         sim.calc_step(steps)
         sim.synchronize()
-    a = torch.ones(2 ** 25, device="cuda", dtype=torch.float32)
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"profile_pass": "done", "pm_steps": steps, "copy_bytes_each_way": 2 ** 27}), flush=True)
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    nbytes = 2 ** 27
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"profile_pass": "done", "pm_steps": steps, "copy_bytes_each_way": nbytes}), flush=True)
 
 
 def main():
