@@ -32,6 +32,13 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 C = 4574.337022617616
+# HBM bytes per k_wave<float> launch on the 128^3 headline grid, measured with
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE in separate runs of
+# `bench.py --profile-pass`, each scaled by what the same counter reads for a
+# 128 MiB device-to-device copy (FETCH_SIZE x 1.9996, WRITE_SIZE x 1.0):
+# 12824.5 KiB fetched and 8192.0 KiB written per launch.  The summary with the
+# commands is profiles/wave_pm1e6_128_hbm_counters.txt.
+WAVE_TRAFFIC_128 = 34_647_350
 
 PM = dict(n=1_000_000, nx=128, dx=0.125, dt=5e-3, iterations=200, setup_iterations=2000, diss_base=0.785,
           diss_extra=0.01)
@@ -317,7 +324,9 @@ def main():
                                "iterations + gradient + trilinear gather, dt=5e-3, fp32",
                    "n": PM["n"], "nx": PM["nx"], "wave_iterations": PM["iterations"]},
         "roofline": {"bound": "hbm", "kernel": "k_wave<float>", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": WAVE_TRAFFIC_128 if PM["nx"] == 128 else None,
+                     "algorithmic_bytes": pm["wave_bytes_per_launch"]},
         "step_roofline": {"algorithmic_bytes": pm["step_algorithmic_bytes"],
                           "achieved_gbs": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9,
                           "frac": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
