@@ -97,15 +97,33 @@ def pm_bench(rrg, ics, steps, warmup):
     }
 
 
+def _fp64_sample_forces(pos, m, s2, nsample=512, chunk=1):
+    """Host FP64 direct sum a_i = sum_j m_j d_ij (r^2 + (s2_i + s2_j)/2)^(-3/2), d_ij = x_j - x_i,
+    for a seeded sample of targets (the reference kernel's formula, nbody.cu:19-65)."""
+    idx = np.sort(np.random.default_rng(5).choice(len(m), nsample, replace=False))
+    ref = np.empty((nsample, 3))
+    x, y, z = (np.ascontiguousarray(pos[:, c]) for c in range(3))
+    for a in range(0, nsample, chunk):
+        t = idx[a:a + chunk]
+        dx, dy, dz = x - x[t, None], y - y[t, None], z - z[t, None]
+        q = dx * dx + dy * dy + dz * dz + 0.5 * (s2[t, None] + s2)
+        w = m / (q * np.sqrt(q))
+        ref[a:a + chunk] = np.stack([(w * dx).sum(1), (w * dy).sum(1), (w * dz).sum(1)], axis=1)
+    return idx, ref
+
+
 def direct_bench(rrg, ics, steps=3):
     pos, vel, m = ics.disk_bulge()
     n = len(m)
     s2 = np.full(n, 0.005 ** 2)
-    out = {}
+    out, acc0, energy0 = {}, {}, {}
     for force, k in (("mixed", steps), ("fp32", steps), ("fp64", 1), ("fp64_exact", 1)):
         p = rrg.make_params(dt=1e-3, force=force)
         sim = rrg.Simulation(p, pos, vel, m, s2)
         sim.precalc()
+        sim.sync_particles_and_grid()
+        acc0[force] = sim.state()["acc"]
+        energy0[force] = sim.conservation()
         sim.calc_step(1)
         sim.synchronize()
         sim.set_timing(True)
@@ -123,6 +141,14 @@ def direct_bench(rrg, ics, steps=3):
             "kernel_ms": kern_s * 1e3,
             "tflops_20_per_interaction": 20 * inter / kern_s / 1e12,
         }
+    idx, ref = _fp64_sample_forces(pos, m, s2)
+    e_ref = energy0["fp64_exact"]["Ek"] + energy0["fp64_exact"]["Ep"]
+    for force in acc0:
+        err = np.linalg.norm(acc0[force][idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        e = energy0[force]["Ek"] + energy0[force]["Ep"]
+        out[force]["force_rel_err_median"] = float(np.median(err))
+        out[force]["force_rel_err_max"] = float(np.max(err))
+        out[force]["energy_rel_err_vs_fp64_exact"] = abs(e - e_ref) / abs(e_ref)
     for f32 in ("mixed", "fp32"):
         for f64 in ("fp64", "fp64_exact"):
             out[f"{f32}_over_{f64}"] = (out[f32]["kernel_ginteractions_per_s"] /
@@ -131,7 +157,10 @@ def direct_bench(rrg, ics, steps=3):
                      "n": n, "timed_steps": {"mixed": steps, "fp32": steps, "fp64": 1, "fp64_exact": 1},
                      "kinds": {"fp64": "FP64 pairs, device-library rsqrt", "fp64_exact": "FP64 pairs, IEEE 1./sqrt "
                                "(the reference kernel's arithmetic)", "mixed": "FP32 pairs on hi/lo positions + "
-                               "two-sum accumulation", "fp32": "FP32 pairs on FP32 positions + two-sum accumulation"}}
+                               "two-sum accumulation", "fp32": "FP32 pairs on FP32 positions + two-sum accumulation"},
+                     "force_error": f"|a - a_ref| / |a_ref| at t=0 on {len(idx)} sampled particles (seed 5); a_ref = "
+                                    "FP64 direct sum over all N sources on the host (numpy)",
+                     "energy_error": "|E - E_ref| / |E_ref| of Ek + Ep at t=0, E_ref from the fp64_exact kernel"}
     return out
 
 
@@ -225,6 +254,57 @@ def adjoint_bench(rrg, ics, n=65_536, steps=200, every=20):
     return {"mixed": out["mixed"], "fp64": out["fp64"], "mixed_grad_rel_err_vs_fp64": errs,
             "config": {"workload": "Plummer N=65536, eps=0.01, dt=1e-3, 200 steps, checkpoint every 20, "
                                    "loss = mean |x_T - x_T(target masses)|^2", "n": n, "steps": steps}}
+
+
+def pm_adjoint_bench(rrg, ics, n=262_144, steps=200, every=20):
+    """PM variant of the differentiable run: exponential disk N=262,144
+    (seed 1), 64^3 cells over [-8,8]^3, 200 damped-wave iterations per step
+    (2000 at setup), dt=5e-3, 200 steps, checkpoints every 20 steps; target =
+    the same ICs with masses x exp(0.05 N(0,1)) (seed 3) run forward; loss =
+    mean |x_T - x_T^target|^2.  t_forward = precalc + recorded steps (the whole
+    differentiated map), t_adjoint = rrg_adjoint.  Gradient error: FP32 state
+    against the FP64 state's gradients of the same run (the FP64 path is
+    pinned to the CPU FP64 reverse-mode gradient in tests/test_gpu_pm_adjoint.py)."""
+    pos, vel, m = ics.exp_disk(n, seed=1)
+    mt = m * np.exp(0.05 * np.random.default_rng(3).standard_normal(n))
+    wave = dict(diss_base=PM["diss_base"], diss_extra=PM["diss_extra"], wave_speed=C, setup_iterations=2000,
+                iterations=200, tolerance=0.0)
+    grid = dict(nx=64, dx=0.25)
+    p = rrg.make_params(dt=5e-3, grid=grid, wave=wave, real="fp32")
+    with rrg.Simulation(p, pos, vel, mt) as sim:
+        sim.precalc()
+        sim.sync_particles_and_grid()
+        sim.calc_step(steps)
+        target = sim.state()["pos"]
+    out = {}
+    for real in ("fp32", "fp64"):
+        p = rrg.make_params(dt=5e-3, grid=grid, wave=wave, real=real)
+        with rrg.Simulation(p, pos, vel, m) as sim:
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.precalc()
+            sim.adjoint_begin(every)
+            sim.synchronize()
+            t1 = time.perf_counter()
+            sim.calc_step(steps)
+            sim.synchronize()
+            t2 = time.perf_counter()
+            xT = sim.state()["pos"]
+            loss = float(np.mean(np.sum((xT - target) ** 2, axis=1)))
+            t3 = time.perf_counter()
+            g = sim.adjoint(2 * (xT - target) / n)
+            t4 = time.perf_counter()
+        out[real] = {"t_forward_s": t2 - t0, "t_forward_steps_s": t2 - t1, "t_adjoint_s": t4 - t3,
+                     "adjoint_over_forward": (t4 - t3) / (t2 - t0), "loss": loss, "grads": g}
+    errs = {}
+    for name, a, b in zip(("x0", "v0", "m"), out["fp32"]["grads"], out["fp64"]["grads"]):
+        errs[name] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    for f in out.values():
+        del f["grads"]
+    return {"fp32": out["fp32"], "fp64": out["fp64"], "fp32_grad_rel_err_vs_fp64": errs,
+            "config": {"workload": "PM: exponential disk N=262144, 64^3 grid on [-8,8]^3, 200 wave iterations/step "
+                                   "(2000 setup), dt=5e-3, 200 steps, checkpoint every 20, loss = mean |x_T - "
+                                   "x_T(target masses)|^2", "n": n, "steps": steps}}
 
 
 def cpu_baseline(ics):
@@ -342,6 +422,7 @@ def main():
         line["direct_sum"] = d
         line["ensemble"] = ensemble_bench(rrg, ics)
         line["adjoint"] = adjoint_bench(rrg, ics)
+        line["adjoint_pm"] = pm_adjoint_bench(rrg, ics)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ics)
     print(json.dumps(line), flush=True)

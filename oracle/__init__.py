@@ -266,6 +266,31 @@ def pm_run(pos, vel, mass, nsteps, *, nx, dx, dt, diss_base, wave_speed, setup_i
             "grid_acc": gacc, "grav": gr, "iterations": iters}
 
 
+def pm_adjoint(pos, vel, mass, nsteps, gx_T, gv_T=None, *, nx, dx, dt, diss_base, wave_speed, setup_iterations,
+               iterations, tolerance=0.0, check_step=30, hold=3, bc_frac=0.0, periodic=True, fixed_e=61):
+    """Reverse-mode gradient of pm_run (FP64): returns (dL/dx0, dL/dv0, dL/dm)
+    for a loss with dL/dx_T = gx_T and dL/dv_T = gv_T."""
+    cf = PmConfig(nx=nx, dx=dx, bc_frac=bc_frac, diss_base=diss_base, wave_speed=wave_speed,
+                  tolerance=tolerance, setup_iterations=setup_iterations, iterations=iterations,
+                  check_step=check_step, hold=hold, periodic=int(periodic), fixed_e=fixed_e, dt=dt)
+    pos = _c(pos, np.float64); vel = _c(vel, np.float64); mass = _c(mass, np.float64)
+    n = len(mass)
+    gx = np.zeros((n, 3)) if gx_T is None else _c(gx_T, np.float64)
+    gv = np.zeros((n, 3)) if gv_T is None else _c(gv_T, np.float64)
+    gx0 = np.empty((n, 3)); gv0 = np.empty((n, 3)); gm = np.empty(n)
+    lib().orc_pm_adjoint_f64(ctypes.byref(cf), _i64(n), _p(pos), _p(vel), _p(mass), _int(nsteps), _p(gx), _p(gv),
+                             _p(gx0), _p(gv0), _p(gm))
+    return gx0, gv0, gm
+
+
+def gradient_t(gbar, nx, dx):
+    """Transpose of gradient(): phibar = (-1/dx) D^T gbar, gbar of shape (nx^3, 3).  FP64."""
+    gbar = _c(gbar, np.float64)
+    out = np.empty(nx ** 3)
+    lib().orc_gradient_t_f64(_int(nx), ctypes.c_double(dx), _p(gbar), _p(out))
+    return out
+
+
 def nbody_adjoint(pos, vel, mass, soft2, dt, nsteps, gx_T, gv_T=None):
     """Reverse-mode gradient of nbody_run: returns (dL/dx0, dL/dv0, dL/dm)
     for a loss with dL/dx_T = gx_T and dL/dv_T = gv_T.  FP64."""

@@ -596,9 +596,18 @@ struct PmConfig {
     double dt;      // particle time step (sim_params.dt_dynamics)
 };
 
+// What the reverse sweep of a PM run needs per grid build (entry 0 = precalc,
+// entry s = step s): the positions the mass was deposited from (after the
+// drift), the potential the gradient read, and the stencil iteration count.
+template <class real> struct PmTape {
+    std::vector<std::vector<real>> x, phi;
+    std::vector<int> iters;
+};
+
 template <class real>
 void pm_run(const PmConfig& cf, int64_t n, real* pos, real* vel, const real* mass, int nsteps,
-            real* acc_out, real* phi_out, real* grid_acc_out, real* grav_out, int* iters_out) {
+            real* acc_out, real* phi_out, real* grid_acc_out, real* grav_out, int* iters_out,
+            PmTape<real>* tape = nullptr) {
     const int nx = cf.nx;
     const size_t ncell = size_t(nx) * nx * nx;
     const real dx = (real)cf.dx, bcf = (real)cf.bc_frac;
@@ -626,10 +635,18 @@ void pm_run(const PmConfig& cf, int64_t n, real* pos, real* vel, const real* mas
         return done;
     };
     int it_idx = 0;
+    auto record = [&](int done) {
+        if (iters_out) iters_out[it_idx++] = done;
+        if (tape) {
+            tape->x.emplace_back(pos, pos + 3 * n);
+            tape->phi.emplace_back(b[1], b[1] + ncell);
+            tape->iters.push_back(done);
+        }
+    };
     // precalc
     deposit();
     int done = solve(cf.setup_iterations);
-    if (iters_out) iters_out[it_idx++] = done;
+    record(done);
     gradient<real>(nx, dx, b[1], gacc.data());
     // sync: gather acceleration onto the particles
     gather<real>(n, pos, nx, dx, bcf, gacc.data(), acc.data());
@@ -637,7 +654,7 @@ void pm_run(const PmConfig& cf, int64_t n, real* pos, real* vel, const real* mas
         predict<real>(n, acc.data(), vel, pos, vp.data(), dtp);
         deposit();
         done = solve(cf.iterations);
-        if (iters_out) iters_out[it_idx++] = done;
+        record(done);
         gradient<real>(nx, dx, b[1], gacc.data());
         gather<real>(n, pos, nx, dx, bcf, gacc.data(), acc.data());
         correct<real>(n, acc.data(), vel, vp.data(), dtp);
@@ -649,6 +666,191 @@ void pm_run(const PmConfig& cf, int64_t n, real* pos, real* vel, const real* mas
     }
     if (grid_acc_out) std::memcpy(grid_acc_out, gacc.data(), sizeof(real) * 3 * ncell);
     if (grav_out) ngp_phi<real>(n, pos, nx, dx, bcf, b[1], grav_out);
+}
+
+// ---------------------------------------------------------------------------
+// Reverse-mode gradient of pm_run (no reference counterpart), FP64.  Pinned by
+// central finite differences of pm_run in the tests.  The deposit's fixed-point
+// rounding (2^-e) is treated as the identity.  Per grid build s = T .. 0
+// (s = 0 is precalc + sync, which builds a_0):
+//   a' = Interp(x', G),  G = -D phi_I / dx      -> Gbar += Interp^T abar, xbar += dInterp
+//   phibar_I += (-1/dx) D^T Gbar                  (masked on zero-Dirichlet edges)
+//   phi_{k+1} = B(K_m L phi_k + K_a phi_k + K_o phi_{k-1} + s) reversed:
+//       mu_{k-1} = B(K_m L mu_k + K_a mu_k + K_o mu_{k+1}),  sbar = sum mu_k
+//   s = 4 pi k_f (CIC(x', m) / dx^3 - rho_mean)  -> mbar += alpha CIC^T sbar, xbar += alpha m dCIC
+//   Verlet (s > 0): xbar, vbar += dt xbar, abar = dt/2 vbar + dt^2/2 xbar,
+//   where the gather's adjoint input is abar + dt/2 vbar.
+// The potential adjoints (mu_k, mu_{k+1}) carry over between grid builds
+// because the stencil sequence continues across steps.
+// ---------------------------------------------------------------------------
+void gradient_t(int nx, double dx, const double* gbar, double* phibar) {  // transpose of gradient()
+    const int NX = nx;
+    const double coef = -1. / dx;
+    const size_t ncell = size_t(NX) * NX * NX;
+    std::fill(phibar, phibar + ncell, 0.0);
+    const size_t st[3] = {1, size_t(NX), size_t(NX) * NX};
+    for (int iz = 0; iz < NX; ++iz)
+        for (int iy = 0; iy < NX; ++iy)
+            for (int ix = 0; ix < NX; ++ix) {
+                const size_t c = AT(ix, iy, iz, NX);
+                const int id[3] = {ix, iy, iz};
+                for (int d = 0; d < 3; ++d) {
+                    const double g = coef * gbar[3 * c + d];
+                    if (id[d] == 0) { phibar[c + st[d]] += g; phibar[c] -= g; }
+                    else if (id[d] == NX - 1) { phibar[c] += g; phibar[c - st[d]] -= g; }
+                    else { phibar[c + st[d]] += 0.5 * g; phibar[c - st[d]] -= 0.5 * g; }
+                }
+            }
+}
+
+// Interpolation weights of gather() for one particle, with d(weight)/d(pos).
+struct InterpW {
+    size_t cell[8];
+    double w[8], dw[8][3];
+};
+
+InterpW interp_weights(const double* p, int nx, double dx, double domain_min) {
+    InterpW r;
+    double f[3];
+    int lo[3], hi[3];
+    double wt[3];
+    for (int d = 0; d < 3; ++d) {
+        f[d] = (p[d] - domain_min) / dx - 0.5f;
+        const int i0 = (int)std::floor((float)f[d]);
+        wt[d] = f[d] - i0;
+        lo[d] = std::max(0, std::min(i0, nx - 1));
+        hi[d] = std::max(0, std::min(i0 + 1, nx - 1));
+    }
+    int q = 0;
+    for (int k = 0; k <= 1; ++k)
+        for (int j = 0; j <= 1; ++j)
+            for (int i = 0; i <= 1; ++i, ++q) {
+                const int sel[3] = {i, j, k};
+                double w1[3], dw1[3];
+                for (int d = 0; d < 3; ++d) {
+                    w1[d] = sel[d] ? wt[d] : 1.0 - wt[d];
+                    dw1[d] = (sel[d] ? 1.0 : -1.0) / dx;
+                }
+                r.cell[q] = AT(i ? hi[0] : lo[0], j ? hi[1] : lo[1], k ? hi[2] : lo[2], nx);
+                r.w[q] = w1[0] * w1[1] * w1[2];
+                r.dw[q][0] = dw1[0] * w1[1] * w1[2];
+                r.dw[q][1] = w1[0] * dw1[1] * w1[2];
+                r.dw[q][2] = w1[0] * w1[1] * dw1[2];
+            }
+    return r;
+}
+
+// CIC weights of cic_stencil() with their derivatives (zero where clamped).
+InterpW cic_weights(const double* p, const Grid<double>& g) {
+    const int NX = g.nx;
+    const double inv_dx = 1.f / g.dx;
+    const double hi_u = (NX - 1) - 1E-6;
+    double u[3], du[3];
+    int i0[3], i1[3];
+    for (int d = 0; d < 3; ++d) {
+        const double raw = (p[d] - g.domain_min) * inv_dx - 0.5f;
+        u[d] = clampv<double>(raw, 0, hi_u);
+        du[d] = (raw < 0 || raw > hi_u) ? 0.0 : inv_dx;
+        i0[d] = (int)std::floor(u[d]);
+        i1[d] = std::min(i0[d] + 1, NX - 1);
+        u[d] -= i0[d];
+    }
+    InterpW r;
+    int q = 0;
+    for (int k = 0; k <= 1; ++k)
+        for (int j = 0; j <= 1; ++j)
+            for (int i = 0; i <= 1; ++i, ++q) {
+                const int sel[3] = {i, j, k};
+                double w1[3], dw1[3];
+                for (int d = 0; d < 3; ++d) {
+                    w1[d] = sel[d] ? u[d] : 1.0 - u[d];
+                    dw1[d] = (sel[d] ? 1.0 : -1.0) * du[d];
+                }
+                r.cell[q] = AT(i ? i1[0] : i0[0], j ? i1[1] : i0[1], k ? i1[2] : i0[2], NX);
+                r.w[q] = w1[0] * w1[1] * w1[2];
+                r.dw[q][0] = dw1[0] * w1[1] * w1[2];
+                r.dw[q][1] = w1[0] * dw1[1] * w1[2];
+                r.dw[q][2] = w1[0] * w1[1] * dw1[2];
+            }
+    return r;
+}
+
+void pm_adjoint(const PmConfig& cf, int64_t n, const double* x0, const double* v0, const double* mass, int nsteps,
+                const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) {
+    std::vector<double> pos(x0, x0 + 3 * n), vel(v0, v0 + 3 * n);
+    PmTape<double> tape;
+    pm_run<double>(cf, n, pos.data(), vel.data(), mass, nsteps, nullptr, nullptr, nullptr, nullptr, nullptr, &tape);
+    const int nx = cf.nx;
+    const size_t ncell = size_t(nx) * nx * nx;
+    const double dx = cf.dx, dt = cf.dt;
+    const double C = cf.wave_speed;
+    const double DT = wave_dt<double>(dx, C);
+    const WaveCoef<double> k = wave_coef<double>(cf.diss_base, C, DT, dx);
+    const double alpha = 4 * kPi * k.k_f / (dx * dx * dx);
+    const Grid<double> g = make_grid<double>(nx, dx, cf.bc_frac);
+    std::vector<double> X(3 * n, 0.0), V(3 * n, 0.0), A(3 * n, 0.0), GM(n, 0.0);
+    if (gxT) std::memcpy(X.data(), gxT, sizeof(double) * 3 * n);
+    if (gvT) std::memcpy(V.data(), gvT, sizeof(double) * 3 * n);
+    std::vector<double> G(3 * ncell), Gb(3 * ncell), phib(ncell), sbar(ncell), zero(ncell, 0.0);
+    std::vector<double> mu[3] = {std::vector<double>(ncell, 0.0), std::vector<double>(ncell, 0.0),
+                                 std::vector<double>(ncell, 0.0)};
+    int ip = 0, ic = 1;  // mu_{k+1}, mu_k
+    auto edge = [&](size_t c) {
+        const int ix = int(c % nx), iy = int((c / nx) % nx), iz = int(c / (size_t(nx) * nx));
+        return ix == 0 || ix == nx - 1 || iy == 0 || iy == nx - 1 || iz == 0 || iz == nx - 1;
+    };
+    for (int s = nsteps; s >= 0; --s) {
+        const double* x = tape.x[s].data();
+        gradient<double>(nx, dx, tape.phi[s].data(), G.data());
+        const double h = s > 0 ? 0.5 * dt : 0.0;
+        std::fill(Gb.begin(), Gb.end(), 0.0);
+        for (int64_t q = 0; q < n; ++q) {
+            const InterpW W = interp_weights(x + 3 * q, nx, dx, g.domain_min);
+            double at[3];
+            for (int d = 0; d < 3; ++d) at[d] = A[3 * q + d] + h * V[3 * q + d];
+            for (int c = 0; c < 8; ++c) {
+                const double* gc = &G[3 * W.cell[c]];
+                const double ga = gc[0] * at[0] + gc[1] * at[1] + gc[2] * at[2];
+                for (int d = 0; d < 3; ++d) {
+                    Gb[3 * W.cell[c] + d] += W.w[c] * at[d];
+                    X[3 * q + d] += W.dw[c][d] * ga;
+                }
+            }
+        }
+        gradient_t(nx, dx, Gb.data(), phib.data());
+        for (size_t c = 0; c < ncell; ++c)
+            if (cf.periodic || !edge(c)) mu[ic][c] += phib[c];
+        std::fill(sbar.begin(), sbar.end(), 0.0);
+        for (int it = 0; it < tape.iters[s]; ++it) {
+            for (size_t c = 0; c < ncell; ++c) sbar[c] += mu[ic][c];
+            const int in = 3 - ip - ic;
+            // the stencil with a zero source is the transposed iteration
+            wave_iteration<double>(nx, dx, cf.diss_base, C, DT, cf.periodic, 0.0, mu[in].data(), mu[ic].data(),
+                                   mu[ip].data(), zero.data());
+            ip = ic;
+            ic = in;
+        }
+        for (int64_t q = 0; q < n; ++q) {
+            const InterpW W = cic_weights(x + 3 * q, g);
+            double mb = 0, dm[3] = {0, 0, 0};
+            for (int c = 0; c < 8; ++c) {
+                mb += W.w[c] * sbar[W.cell[c]];
+                for (int d = 0; d < 3; ++d) dm[d] += W.dw[c][d] * sbar[W.cell[c]];
+            }
+            GM[q] += alpha * mb;
+            for (int d = 0; d < 3; ++d) X[3 * q + d] += alpha * mass[q] * dm[d];
+        }
+        if (s > 0) {
+            for (int64_t i = 0; i < 3 * n; ++i) {
+                const double vb = V[i];
+                V[i] = vb + dt * X[i];
+                A[i] = h * vb + h * dt * X[i];
+            }
+        }
+    }
+    std::memcpy(gx0, X.data(), sizeof(double) * 3 * n);
+    std::memcpy(gv0, V.data(), sizeof(double) * 3 * n);
+    std::memcpy(gm, GM.data(), sizeof(double) * n);
 }
 
 }  // namespace
@@ -720,6 +922,14 @@ ORC_EXPORT void orc_nbody_adjoint_f64(int64_t n, const double* x0, const double*
                                       const double* s2, double dt, int nsteps, const double* gxT,
                                       const double* gvT, double* gx0, double* gv0, double* gm) {
     nbody_adjoint(n, x0, v0, m, s2, dt, nsteps, gxT, gvT, gx0, gv0, gm);
+}
+ORC_EXPORT void orc_pm_adjoint_f64(const PmConfig* cf, int64_t n, const double* x0, const double* v0,
+                                   const double* m, int nsteps, const double* gxT, const double* gvT,
+                                   double* gx0, double* gv0, double* gm) {
+    pm_adjoint(*cf, n, x0, v0, m, nsteps, gxT, gvT, gx0, gv0, gm);
+}
+ORC_EXPORT void orc_gradient_t_f64(int nx, double dx, const double* gbar, double* phibar) {
+    gradient_t(nx, dx, gbar, phibar);
 }
 ORC_EXPORT void orc_vjp_f64(int64_t n, const double* x, const double* m, const double* s2, const double* u,
                             double* g, double* gm) {

@@ -829,6 +829,8 @@ public:
         grav_actual_ = false;
         acc_actual_ = false;
         precalc_done_ = true;
+        fresh_ = true;
+        recording_ = false;
     }
 
     void step(int nsteps) override {  // rrgsim_integration_rect.cuh:99-112
@@ -839,15 +841,96 @@ public:
         // arrangement).
         const bool graph = prm_.use_graphs && !(tol_ > (real)0);
         for (int k = 0; k < nsteps; ++k) {
-            if (++since_sort_ >= kSortInterval) sort_particles();
+            // a recorded run keeps one particle order so its tapes line up
+            if (++since_sort_ >= kSortInterval && !recording_) sort_particles();
             enqueue_step(dt, graph);
             ++steps_;
             time_ += prm_.sim.dt_dynamics;
+            if (recording_) {
+                rec_iters_.push_back(last_iters_);
+                ++rec_steps_;
+                if (rec_steps_ % ck_every_ == 0) store_checkpoint(ck_.emplace_back());
+            }
         }
         if (nsteps > 0) {
             acc_actual_ = true;
             grav_actual_ = false;
+            fresh_ = false;
         }
+    }
+
+    // Records the run from the state right after precalc: the particles are
+    // sorted once and stay in that order, a_0 is interpolated from the
+    // precalc grid (the integrate loop's sync after precalc), and
+    // (x, v, a, phi_prev, phi_curr) is kept every `every` steps.
+    void adjoint_begin(int every) override {
+        require(every >= 1, "rrg_adjoint_begin: checkpoint_every must be >= 1");
+        if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_adjoint_begin before rrg_precalc");
+        if (!fresh_) fail(RRG_ERR_STATE, "rrg_adjoint_begin: a PM context records from the state right after rrg_precalc");
+        sort_particles();
+        sync();
+        ck_every_ = every;
+        rec_steps_ = 0;
+        ck_.clear();
+        rec_iters_.assign(1, last_iters_);
+        store_checkpoint(ck_.emplace_back());
+        recording_ = true;
+    }
+
+    // Reverse sweep: per checkpoint segment (last first) the steps are
+    // recomputed on the live buffers, keeping x' and phi_I of each step, and
+    // then reversed; the precalc build closes the sweep.  The live state is
+    // restored afterwards.
+    void adjoint(const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) override {
+        if (!recording_) fail(RRG_ERR_STATE, "rrg_adjoint without rrg_adjoint_begin");
+        const int64_t n = P_.n;
+        adj_alloc();
+        const std::vector<int> id = ids();
+        {
+            std::vector<double> tmp(3 * n, 0.0), zero(3 * n, 0.0);
+            if (gxT) permute_vec_in(n, id, gxT, tmp.data());
+            upload_vec<double>(s_, n, gxT ? tmp.data() : zero.data(), Xb_.p);
+            if (gvT) permute_vec_in(n, id, gvT, tmp.data());
+            upload_vec<double>(s_, n, gvT ? tmp.data() : zero.data(), Vb_.p);
+        }
+        zero_async(s_, Ab_.p, Ab_.bytes());
+        zero_async(s_, gm_.p, gm_.bytes());
+        for (auto& b : mu_) zero_async(s_, b.p, b.bytes());
+        aip_ = 0;
+        aic_ = 1;
+        // keep the live state
+        const int ip0 = ip_, ic0 = ic_, it0 = last_iters_;
+        store_checkpoint(backup_);
+        const real dt = (real)prm_.sim.dt_dynamics;
+        const bool graph = prm_.use_graphs && !(tol_ > (real)0);
+        const int T = int(rec_steps_), K = ck_every_;
+        const int nseg = (T + K - 1) / K;
+        const size_t v3 = sizeof(real) * 3 * n, vg = sizeof(real) * ncell_;
+        for (int c = nseg - 1; c >= 0; --c) {
+            const int s0 = c * K, len = std::min(T, s0 + K) - s0;
+            load_checkpoint(ck_[c]);
+            for (int q = 1; q <= len; ++q) {
+                enqueue_step(dt, graph, rec_iters_[s0 + q]);
+                RRG_HIP(hipMemcpyAsync(xs_.p + size_t(q) * 3 * n, P_.X.p, v3, hipMemcpyDeviceToDevice, s_));
+                RRG_HIP(hipMemcpyAsync(phis_.p + size_t(q) * ncell_, cur(), vg, hipMemcpyDeviceToDevice, s_));
+            }
+            for (int q = len; q >= 1; --q)
+                reverse_build(xs_.p + size_t(q) * 3 * n, phis_.p + size_t(q) * ncell_, rec_iters_[s0 + q], false);
+        }
+        // precalc build at x_0 (a_0 interpolated from it, phi from the setup solve)
+        reverse_build(ck_[0].p, ck_[0].p + 9 * n + ncell_, rec_iters_[0], true);
+        ip_ = ip0;
+        ic_ = ic0;
+        load_checkpoint(backup_);
+        last_iters_ = it0;
+        std::vector<double> tmp(3 * n);
+        if (gx0) { download_vec<double>(s_, n, Xb_.p, tmp.data()); unpermute_vec(n, id, tmp.data(), gx0); }
+        if (gv0) { download_vec<double>(s_, n, Vb_.p, tmp.data()); unpermute_vec(n, id, tmp.data(), gv0); }
+        if (gm) {
+            download_scalar<double>(s_, n, gm_.p, tmp.data());
+            for (int64_t i = 0; i < n; ++i) gm[id[i]] = tmp[i];
+        }
+        RRG_HIP(hipStreamSynchronize(s_));
     }
 
     void sync() override {  // rrgsim_integration_rect.cuh:85-97
@@ -878,6 +961,7 @@ public:
         if (pos) { permute_vec_in(n, id, pos, tmp.data()); upload_vec<real>(s_, n, tmp.data(), P_.X.p); }
         if (vel) { permute_vec_in(n, id, vel, tmp.data()); upload_vec<real>(s_, n, tmp.data(), P_.V.p); }
         grav_actual_ = acc_actual_ = false;
+        fresh_ = recording_ = false;
     }
 
     void conservation(rrg_conservation* out) override {
@@ -931,7 +1015,7 @@ private:
     // iterations, gradient, gather + kick.  With `graph` every segment is
     // replayed from its captured graph and stage timing events sit between
     // segment launches.
-    void enqueue_step(real dt, bool graph) {
+    void enqueue_step(real dt, bool graph, int fixed_iters = -1) {
         const uint64_t b0 = graph_base();
         segment(RRG_STAGE_DEPOSIT, 1, graph, b0 + 0, [&] {
             pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p);
@@ -948,6 +1032,11 @@ private:
             });
             rotate(iters);
             last_iters_ = iters;
+        } else if (fixed_iters >= 0) {  // a recorded count, no checks (recomputation)
+            const int t = timer.start(s_, RRG_STAGE_WAVE);
+            wave_launches(fixed_iters, ip_, ic_);
+            last_iters_ = fixed_iters;
+            timer.stop(s_, t, fixed_iters);
         } else {
             const int t = timer.start(s_, RRG_STAGE_WAVE);
             last_iters_ = solve(iters);
@@ -984,6 +1073,86 @@ private:
         RRG_HIP(hipMemcpyAsync(id.data(), id_.p, sizeof(int) * P_.n, hipMemcpyDeviceToHost, s_));
         RRG_HIP(hipStreamSynchronize(s_));
         return id;
+    }
+
+    // (x, v, a, phi_prev, phi_curr) of the live state into / out of `ck`.
+    void store_checkpoint(DBuf<real>& ck) {
+        const int64_t n = P_.n;
+        if (ck.n != size_t(9 * n + 2 * ncell_)) ck.alloc(9 * n + 2 * ncell_);
+        const size_t v3 = sizeof(real) * 3 * n, vg = sizeof(real) * ncell_;
+        RRG_HIP(hipMemcpyAsync(ck.p, P_.X.p, v3, hipMemcpyDeviceToDevice, s_));
+        RRG_HIP(hipMemcpyAsync(ck.p + 3 * n, P_.V.p, v3, hipMemcpyDeviceToDevice, s_));
+        RRG_HIP(hipMemcpyAsync(ck.p + 6 * n, P_.A.p, v3, hipMemcpyDeviceToDevice, s_));
+        RRG_HIP(hipMemcpyAsync(ck.p + 9 * n, phi_[ip_].p, vg, hipMemcpyDeviceToDevice, s_));
+        RRG_HIP(hipMemcpyAsync(ck.p + 9 * n + ncell_, phi_[ic_].p, vg, hipMemcpyDeviceToDevice, s_));
+    }
+
+    void load_checkpoint(const DBuf<real>& ck) {
+        const int64_t n = P_.n;
+        const size_t v3 = sizeof(real) * 3 * n, vg = sizeof(real) * ncell_;
+        RRG_HIP(hipMemcpyAsync(P_.X.p, ck.p, v3, hipMemcpyDeviceToDevice, s_));
+        RRG_HIP(hipMemcpyAsync(P_.V.p, ck.p + 3 * n, v3, hipMemcpyDeviceToDevice, s_));
+        RRG_HIP(hipMemcpyAsync(P_.A.p, ck.p + 6 * n, v3, hipMemcpyDeviceToDevice, s_));
+        RRG_HIP(hipMemcpyAsync(phi_[ip_].p, ck.p + 9 * n, vg, hipMemcpyDeviceToDevice, s_));
+        RRG_HIP(hipMemcpyAsync(phi_[ic_].p, ck.p + 9 * n + ncell_, vg, hipMemcpyDeviceToDevice, s_));
+    }
+
+    void adj_alloc() {
+        const int64_t n = P_.n;
+        if (Xb_.n != size_t(3 * n)) {
+            Xb_.alloc(3 * n); Vb_.alloc(3 * n); Ab_.alloc(3 * n); gm_.alloc(n);
+            Gb_.alloc(3 * ncell_);
+            for (auto& b : mu_) b.alloc(ncell_);
+            sbar_.alloc(ncell_);
+            gtmp_.alloc(4 * ncell_);
+            zero_async(s_, Gb_.p, Gb_.bytes());
+        }
+        const size_t K1 = size_t(ck_every_) + 1;
+        if (xs_.n < K1 * 3 * n) xs_.alloc(K1 * 3 * n);
+        if (phis_.n < K1 * ncell_) phis_.alloc(K1 * ncell_);
+    }
+
+    // Reverse of one grid build (a step, or with `closing` the precalc build)
+    // given the positions x' it deposited from and the potential phi_I its
+    // gradient read.  Gb is zero on entry and left zero.
+    void reverse_build(const real* x, const real* phi, int iters, bool closing) {
+        const int64_t n = P_.n;
+        const double dt = prm_.sim.dt_dynamics;
+        pm::gradient<real>(s_, nx_, geo_.dx, phi, gtmp_.p);
+        pm::adj_interp<real>(s_, n, x, geo_, gtmp_.p, closing ? 0.0 : 0.5 * dt, Xb_.p, Vb_.p, Ab_.p, Gb_.p);
+        pm::adj_gradient_t<real>(s_, nx_, geo_.dx, periodic_, Gb_.p, mu_[aic_].p);
+        pm::fill_u64(s_, reinterpret_cast<unsigned long long*>(Gb_.p), 3 * size_t(ncell_), 0ull);
+        if (iters == 0) {
+            zero_async(s_, sbar_.p, sbar_.bytes());
+        } else {
+            auto run = [&](int& ip, int& ic) {
+                for (int it = 0; it < iters; ++it) {
+                    const int in = 3 - ip - ic;
+                    pm::adj_wave<real>(s_, nx_, periodic_, k_, mu_[in].p, mu_[ic].p, mu_[ip].p, sbar_.p, it == 0);
+                    ip = ic;
+                    ic = in;
+                }
+            };
+            const int t = timer.start(s_, RRG_STAGE_WAVE);
+            if (prm_.use_graphs) {
+                const uint64_t key = (uint64_t(1) << 40) | (uint64_t(iters) << 4) | uint64_t(aip_ * 3 + aic_);
+                graphs_.run(s_, key, [&] {
+                    int ip = aip_, ic = aic_;
+                    run(ip, ic);
+                });
+                for (int i = 0; i < iters % 3; ++i) {
+                    const int in = 3 - aip_ - aic_;
+                    aip_ = aic_;
+                    aic_ = in;
+                }
+            } else {
+                run(aip_, aic_);
+            }
+            timer.stop(s_, t, iters);
+        }
+        const real dx3 = geo_.dx * geo_.dx * geo_.dx;
+        const double alpha = 4 * 3.14159265358979 * (double)k_.k_f / (double)dx3;  // d source / d cell mass
+        pm::adj_cic<real>(s_, n, x, P_.M.p, geo_, sbar_.p, alpha, dt, closing, Xb_.p, Vb_.p, Ab_.p, gm_.p);
     }
 
     // Reorder the particle arrays by CIC base cell (radix sort + gather), so
@@ -1075,9 +1244,20 @@ private:
     PinnedBuf host_;
     int ip_ = 0, ic_ = 1;  // indices of phi^{n-1} (prev) and phi^n (curr)
     bool precalc_done_ = false, grav_actual_ = false, acc_actual_ = false;
+    bool fresh_ = false;  // state is the one precalc left (no step, no set_state since)
     int last_iters_ = 0;
     int64_t steps_ = 0;
     double time_ = 0.0;
+    // adjoint
+    bool recording_ = false;
+    int ck_every_ = 1;
+    int64_t rec_steps_ = 0;
+    std::vector<int> rec_iters_;        // stencil iterations per grid build (0 = precalc)
+    std::vector<DBuf<real>> ck_;        // (x, v, a, phi_prev, phi_curr) every ck_every_ steps
+    DBuf<real> backup_, xs_, phis_;     // live state; x' and phi_I of one segment
+    DBuf<double> Xb_, Vb_, Ab_, gm_, Gb_;
+    DBuf<real> mu_[3], sbar_, gtmp_;
+    int aip_ = 0, aic_ = 1;             // potential adjoints mu_{k+1}, mu_k
 };
 
 }  // namespace

@@ -237,6 +237,25 @@ template <class real> void max_diff(hipStream_t s, int64_t ncell, const real* a,
 // Fill kernel (used instead of hipMemsetAsync so the work stays capturable).
 void fill_u64(hipStream_t s, unsigned long long* p, size_t count, unsigned long long value);
 
+// Reverse sweep of the PM step (pm_adjoint_kernels.hip).  Particle arrays SoA
+// (n per component), particle adjoints FP64; Gb = three FP64 grids (x, y, z).
+// Interpolation adjoint at x': Xb += d(Interp)/dx' . G, Gb += Interp^T (Ab + h Vb).
+template <class real>
+void adj_interp(hipStream_t s, int64_t n, const real* x, const Geo<real>& g, const real* G4, double h, double* Xb,
+                const double* Vb, const double* Ab, double* Gb);
+// mu += B(-1/dx D^T Gb) (B masks the faces of zero-Dirichlet grids).
+template <class real> void adj_gradient_t(hipStream_t s, int nx, real dx, bool periodic, const double* Gb, real* mu);
+// One transposed stencil iteration pn = B(K_m L p + K_a p + K_o po) with
+// sbar = p (first) or sbar += p.
+template <class real>
+void adj_wave(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* pn, const real* p, const real* po,
+              real* sbar, bool first);
+// CIC adjoint at x' (gm += alpha CIC^T sbar, Xb += alpha m dCIC^T sbar) and,
+// unless `closing`, the transposed Verlet update of (Xb, Vb, Ab).
+template <class real>
+void adj_cic(hipStream_t s, int64_t n, const real* x, const real* m, const Geo<real>& g, const real* sbar,
+             double alpha, double dt, bool closing, double* Xb, double* Vb, double* Ab, double* gm);
+
 }  // namespace pm
 
 }  // namespace rrg
