@@ -277,7 +277,8 @@ def pm_adjoint_bench(rrg, ics, n=262_144, steps=200, every=20):
         sim.calc_step(steps)
         target = sim.state()["pos"]
     out = {}
-    for real in ("fp32", "fp64"):
+    seed64 = None
+    for real in ("fp64", "fp32"):
         p = rrg.make_params(dt=5e-3, grid=grid, wave=wave, real=real)
         with rrg.Simulation(p, pos, vel, m) as sim:
             sim.synchronize()
@@ -291,17 +292,37 @@ def pm_adjoint_bench(rrg, ics, n=262_144, steps=200, every=20):
             t2 = time.perf_counter()
             xT = sim.state()["pos"]
             loss = float(np.mean(np.sum((xT - target) ** 2, axis=1)))
+            seed = 2 * (xT - target) / n
             t3 = time.perf_counter()
-            g = sim.adjoint(2 * (xT - target) / n)
+            g = sim.adjoint(seed)
             t4 = time.perf_counter()
-        out[real] = {"t_forward_s": t2 - t0, "t_forward_steps_s": t2 - t1, "t_adjoint_s": t4 - t3,
-                     "adjoint_over_forward": (t4 - t3) / (t2 - t0), "loss": loss, "grads": g}
-    errs = {}
-    for name, a, b in zip(("x0", "v0", "m"), out["fp32"]["grads"], out["fp64"]["grads"]):
-        errs[name] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+            r = {"t_forward_s": t2 - t0, "t_forward_steps_s": t2 - t1, "t_adjoint_s": t4 - t3,
+                 "adjoint_over_forward": (t4 - t3) / (t2 - t0), "loss": loss, "grads": g}
+            if real == "fp64":
+                seed64 = seed
+            else:
+                r["grads_fp64_seed"] = sim.adjoint(seed64)
+                sim.set_timing(True)  # stage breakdown from a second, event-timed sweep
+                sim.adjoint(seed)
+                r["adjoint_stage_ms"] = {k: round(v[0], 3) for k, v in sim.timing().items() if v[1]}
+                sim.set_timing(False)
+        out[real] = r
+
+    def rel(a, b):
+        return {name: float(np.max(np.abs(x - y)) / np.max(np.abs(y))) for name, x, y in zip(("x0", "v0", "m"), a, b)}
+
+    ref = out["fp64"]["grads"]
+    same_seed = rel(out["fp32"].pop("grads_fp64_seed"), ref)
+    own_seed = rel(out["fp32"]["grads"], ref)
+    seed_diff = float(np.max(np.abs(seed - seed64)) / np.max(np.abs(seed64)))  # `seed` is the FP32 run's
     for f in out.values():
         del f["grads"]
-    return {"fp32": out["fp32"], "fp64": out["fp64"], "fp32_grad_rel_err_vs_fp64": errs,
+    return {"fp32": out["fp32"], "fp64": out["fp64"],
+            "fp32_grad_rel_err_vs_fp64": same_seed,
+            "fp32_grad_rel_err_vs_fp64_own_loss": own_seed,
+            "fp32_loss_seed_rel_diff": seed_diff,
+            "grad_error_note": "same dL/dx_T seed (the FP64 run's) through both sweeps; the own-loss figure also "
+                               "carries the FP32 forward's divergence from the FP64 trajectory in dL/dx_T",
             "config": {"workload": "PM: exponential disk N=262144, 64^3 grid on [-8,8]^3, 200 wave iterations/step "
                                    "(2000 setup), dt=5e-3, 200 steps, checkpoint every 20, loss = mean |x_T - "
                                    "x_T(target masses)|^2", "n": n, "steps": steps}}

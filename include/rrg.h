@@ -156,7 +156,10 @@ enum {
     RRG_STAGE_PREDICT = 6,  /* predictor (N-body) */
     RRG_STAGE_CORRECT = 7,  /* corrector (N-body) */
     RRG_STAGE_SORT = 8,     /* spatial re-sort of the particles (PM) */
-    RRG_NSTAGES = 9
+    RRG_STAGE_ADJ_FORCE = 9,     /* adjoint: pair-force VJP (N-body); interpolation adjoint + gradient transpose (PM) */
+    RRG_STAGE_ADJ_WAVE = 10,     /* adjoint: transposed stencil iterations (PM) */
+    RRG_STAGE_ADJ_PARTICLE = 11, /* adjoint: CIC adjoint and transposed Verlet update */
+    RRG_NSTAGES = 12
 };
 
 /* GPU time per stage from HIP events recorded on the context stream around

@@ -78,7 +78,8 @@ class Timing(ctypes.Structure):
     _fields_ = [("ms", _f64 * 12), ("launches", _i64 * 12)]
 
 
-STAGES = ["deposit", "source", "wave", "gradient", "gather", "direct", "predict", "correct", "sort"]
+STAGES = ["deposit", "source", "wave", "gradient", "gather", "direct", "predict", "correct", "sort", "adj_force",
+          "adj_wave", "adj_particle"]
 
 
 # Every symbol the header declares, with its ctypes signature.

@@ -52,56 +52,147 @@ __device__ __forceinline__ void xcd_block(int nbx, int nby, int nbz, int& bx, in
 // choice and weights repeat k_gather's arithmetic in `real`; the products are
 // formed in FP64.  at = abar + h vbar is the adjoint of the interpolated
 // acceleration (h = dt/2 for a step, 0 for the precalc build).
+//
+// The deposit of at into Gb merges lanes first: the wave's particles are
+// sorted by their 8-cell stencil (a 64-bit key of the clamped corner
+// indices) with a shuffle bitonic sort, a segmented inclusive scan sums each
+// run of equal stencils, and the run's last lane issues the 8 x 3 FP64
+// atomics.  The particle arrays are kept in cell order during a recorded run,
+// so dense regions form long runs.
+constexpr unsigned long long kNoStencil = ~0ull;
+
 template <class real>
 __global__ __launch_bounds__(256) void k_adj_interp(int64_t n, const real* __restrict__ x, Geo<real> g,
                                                     const real* __restrict__ G4, double h, double* __restrict__ Xb,
                                                     const double* __restrict__ Vb, const double* __restrict__ Ab,
                                                     double* __restrict__ Gb) {
     const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (q >= n) return;
+    const int lane = threadIdx.x & 63;
     const int nx = g.nx;
     const size_t ncell = size_t(nx) * nx * nx;
-    int lo[3], hi[3];
-    double w[3][2];
+    const unsigned long long K = 2ull * unsigned(nx);
+    unsigned long long key = kNoStencil;
+    double at[3] = {0, 0, 0};
+    double w[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    if (q < n) {
+        int lo[3], hi[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const real f = (x[d * n + q] - g.domain_min) / g.dx - (real)0.5f;
-        const int i0 = (int)floorf((float)f);
-        const real wt = f - (real)i0;
-        w[d][0] = (double)((real)1.0f - wt);
-        w[d][1] = (double)wt;
-        lo[d] = max(0, min(i0, nx - 1));
-        hi[d] = max(0, min(i0 + 1, nx - 1));
+        for (int d = 0; d < 3; ++d) {
+            const real f = (x[d * n + q] - g.domain_min) / g.dx - (real)0.5f;
+            const int i0 = (int)floorf((float)f);
+            const real wt = f - (real)i0;
+            w[d][0] = (double)((real)1.0f - wt);
+            w[d][1] = (double)wt;
+            lo[d] = max(0, min(i0, nx - 1));
+            hi[d] = max(0, min(i0 + 1, nx - 1));
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) at[d] = Ab[d * n + q] + h * Vb[d * n + q];
+        const double inv = 1.0 / (double)g.dx;
+        double gx = 0, gy = 0, gz = 0;
+#pragma unroll
+        for (int k = 0; k <= 1; ++k) {
+#pragma unroll
+            for (int j = 0; j <= 1; ++j) {
+#pragma unroll
+                for (int i = 0; i <= 1; ++i) {
+                    const size_t c = size_t(i ? hi[0] : lo[0]) + size_t(j ? hi[1] : lo[1]) * nx +
+                                     size_t(k ? hi[2] : lo[2]) * nx * nx;
+                    const real* f = G4 + 4 * c;
+                    const double ga = (double)f[0] * at[0] + (double)f[1] * at[1] + (double)f[2] * at[2];
+                    const double wi = w[0][i], wj = w[1][j], wk = w[2][k];
+                    gx += (i ? inv : -inv) * wj * wk * ga;
+                    gy += wi * (j ? inv : -inv) * wk * ga;
+                    gz += wi * wj * (k ? inv : -inv) * ga;
+                }
+            }
+        }
+        Xb[q] += gx;
+        Xb[n + q] += gy;
+        Xb[2 * n + q] += gz;
+        // stencil key: per axis 2*lo + (hi != lo)
+        key = 0;
+#pragma unroll
+        for (int d = 2; d >= 0; --d) key = key * K + (unsigned long long)(2 * lo[d] + (hi[d] != lo[d] ? 1 : 0));
     }
-    double at[3];
+    // bitonic sort of (key, lane) over the wave
+    int src = lane;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) at[d] = Ab[d * n + q] + h * Vb[d * n + q];
-    const double inv = 1.0 / (double)g.dx;
-    double gx = 0, gy = 0, gz = 0;
+    for (int size = 2; size <= 64; size <<= 1) {
 #pragma unroll
-    for (int k = 0; k <= 1; ++k) {
-#pragma unroll
-        for (int j = 0; j <= 1; ++j) {
-#pragma unroll
-            for (int i = 0; i <= 1; ++i) {
-                const size_t c = size_t(i ? hi[0] : lo[0]) + size_t(j ? hi[1] : lo[1]) * nx +
-                                 size_t(k ? hi[2] : lo[2]) * nx * nx;
-                const real* f = G4 + 4 * c;
-                const double ga = (double)f[0] * at[0] + (double)f[1] * at[1] + (double)f[2] * at[2];
-                const double wi = w[0][i], wj = w[1][j], wk = w[2][k];
-                gx += (i ? inv : -inv) * wj * wk * ga;
-                gy += wi * (j ? inv : -inv) * wk * ga;
-                gz += wi * wj * (k ? inv : -inv) * ga;
-                const double wc = wi * wj * wk;
-                unsafeAtomicAdd(Gb + c, wc * at[0]);
-                unsafeAtomicAdd(Gb + ncell + c, wc * at[1]);
-                unsafeAtomicAdd(Gb + 2 * ncell + c, wc * at[2]);
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const unsigned long long ok = __shfl_xor(key, stride, 64);
+            const int os = __shfl_xor(src, stride, 64);
+            const bool up = (lane & size) == 0;
+            const bool lower = (lane & stride) == 0;
+            const bool other_smaller = ok < key || (ok == key && os < src);
+            if (lower == up ? other_smaller : !other_smaller) {
+                key = ok;
+                src = os;
             }
         }
     }
-    Xb[q] += gx;
-    Xb[n + q] += gy;
-    Xb[2 * n + q] += gz;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        at[d] = __shfl(at[d], src, 64);
+        w[d][0] = __shfl(w[d][0], src, 64);
+        w[d][1] = __shfl(w[d][1], src, 64);
+    }
+    const unsigned long long kprev = __shfl_up(key, 1, 64);
+    const unsigned long long knext = __shfl_down(key, 1, 64);
+    const int head = (lane == 0 || kprev != key) ? 1 : 0;
+    const bool tail = (lane == 63 || knext != key) && key != kNoStencil;
+    size_t cell[8];
+    if (tail) {
+        unsigned long long r = key;
+        int lo[3], hi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const unsigned c = unsigned(r % K);
+            r /= K;
+            lo[d] = int(c >> 1);
+            hi[d] = lo[d] + int(c & 1);
+        }
+#pragma unroll
+        for (int k = 0; k <= 1; ++k)
+#pragma unroll
+            for (int j = 0; j <= 1; ++j)
+#pragma unroll
+                for (int i = 0; i <= 1; ++i)
+                    cell[i + 2 * j + 4 * k] = size_t(i ? hi[0] : lo[0]) + size_t(j ? hi[1] : lo[1]) * nx +
+                                              size_t(k ? hi[2] : lo[2]) * nx * nx;
+    }
+    // one component at a time: corner sums, segmented inclusive scan, atomics
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k <= 1; ++k)
+#pragma unroll
+            for (int j = 0; j <= 1; ++j)
+#pragma unroll
+                for (int i = 0; i <= 1; ++i) v[i + 2 * j + 4 * k] = w[0][i] * w[1][j] * w[2][k] * at[d];
+        int f = head;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int fu = __shfl_up(f, off, 64);
+            double vu[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) vu[c] = __shfl_up(v[c], off, 64);
+            if (lane >= off) {
+                if (!f) {
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) v[c] += vu[c];
+                }
+                f |= fu;
+            }
+        }
+        if (tail) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (v[c] != 0.0) unsafeAtomicAdd(Gb + d * ncell + cell[c], v[c]);
+        }
+    }
 }
 
 // mu += B(coef * D^T Gb): each cell gathers the terms of the (up to three)

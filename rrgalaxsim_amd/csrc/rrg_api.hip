@@ -655,10 +655,16 @@ private:
     void reverse_step(const double* x, double dt, bool last) {
         const int64_t n = P_.n;
         const bool f32 = nbody::is_fp32_kind(kind_);
+        int t = timer.start(s_, RRG_STAGE_ADJ_PARTICLE);
         adjoint::pre(s_, n, dt, Vb_.p, Ab_.p, At_.p, f32 ? Uf_.p : nullptr);
         if (f32) adjoint::pack_pos(s_, n, x, P_.M.p, P_.S2.p, Ap_.p, Bp_.p);
+        timer.stop(s_, t, f32 ? 2 : 1);
+        t = timer.start(s_, RRG_STAGE_ADJ_FORCE);
         adjoint::vjp(s_, vplan_, kind_, n, x, P_.M.p, P_.S2.p, At_.p, Ap_.p, Bp_.p, Uf_.p, vpart_.p);
+        timer.stop(s_, t, 1);
+        t = timer.start(s_, RRG_STAGE_ADJ_PARTICLE);
         adjoint::post(s_, n, prm_.sim.dt_dynamics, vplan_.slices, vpart_.p, Xb_.p, Vb_.p, Ab_.p, gm_.p, last);
+        timer.stop(s_, t, 1);
     }
 
     // One step as three segments (predictor, force, corrector); with `graph`
@@ -1118,10 +1124,12 @@ private:
     void reverse_build(const real* x, const real* phi, int iters, bool closing) {
         const int64_t n = P_.n;
         const double dt = prm_.sim.dt_dynamics;
+        int t = timer.start(s_, RRG_STAGE_ADJ_FORCE);
         pm::gradient<real>(s_, nx_, geo_.dx, phi, gtmp_.p);
         pm::adj_interp<real>(s_, n, x, geo_, gtmp_.p, closing ? 0.0 : 0.5 * dt, Xb_.p, Vb_.p, Ab_.p, Gb_.p);
         pm::adj_gradient_t<real>(s_, nx_, geo_.dx, periodic_, Gb_.p, mu_[aic_].p);
         pm::fill_u64(s_, reinterpret_cast<unsigned long long*>(Gb_.p), 3 * size_t(ncell_), 0ull);
+        timer.stop(s_, t, 4);
         if (iters == 0) {
             zero_async(s_, sbar_.p, sbar_.bytes());
         } else {
@@ -1133,7 +1141,7 @@ private:
                     ic = in;
                 }
             };
-            const int t = timer.start(s_, RRG_STAGE_WAVE);
+            t = timer.start(s_, RRG_STAGE_ADJ_WAVE);
             if (prm_.use_graphs) {
                 const uint64_t key = (uint64_t(1) << 40) | (uint64_t(iters) << 4) | uint64_t(aip_ * 3 + aic_);
                 graphs_.run(s_, key, [&] {
@@ -1152,7 +1160,9 @@ private:
         }
         const real dx3 = geo_.dx * geo_.dx * geo_.dx;
         const double alpha = 4 * 3.14159265358979 * (double)k_.k_f / (double)dx3;  // d source / d cell mass
+        t = timer.start(s_, RRG_STAGE_ADJ_PARTICLE);
         pm::adj_cic<real>(s_, n, x, P_.M.p, geo_, sbar_.p, alpha, dt, closing, Xb_.p, Vb_.p, Ab_.p, gm_.p);
+        timer.stop(s_, t, 1);
     }
 
     // Reorder the particle arrays by CIC base cell (radix sort + gather), so
