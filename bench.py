@@ -38,7 +38,7 @@ C = 4574.337022617616
 # 128 MiB device-to-device copy (FETCH_SIZE x 1.9996, WRITE_SIZE x 1.0):
 # 12824.5 KiB fetched and 8192.0 KiB written per launch.  The summary with the
 # commands is profiles/wave_pm1e6_128_hbm_counters.txt.
-WAVE_TRAFFIC_128 = 34_647_350
+WAVE_TRAFFIC_128 = 34_647_301
 
 PM = dict(n=1_000_000, nx=128, dx=0.125, dt=5e-3, iterations=200, setup_iterations=2000, diss_base=0.785,
           diss_extra=0.01)
@@ -62,6 +62,7 @@ def pm_bench(rrg, ics, steps, warmup):
     sim = rrg.Simulation(p, pos, vel, m)
     sim.precalc()
     sim.sync_particles_and_grid()
+    acc0 = sim.state()["acc"]
     if warmup:
         sim.calc_step(warmup)
     sim.synchronize()
@@ -86,7 +87,13 @@ def pm_bench(rrg, ics, steps, warmup):
     step_bytes = (PM["iterations"] * wave_bytes + 20 * ncell + 24 * ncell + n * (40 + 24 + 64) +
                   n * (36 + 24 + 128))
     secs = t1 - t0
+    # PM force vs an FP64 direct sum over all particles (softening = dx): the
+    # method error of the PM model (periodic source, cell-size resolution)
+    idx, ref = _fp64_sample_forces(pos, m, np.full(n, PM["dx"] ** 2), nsample=256)
+    err = np.linalg.norm(acc0[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
     return {
+        "pm_force_err_vs_fp64_direct": {"median": float(np.median(err)), "p99": float(np.percentile(err, 99)),
+                                        "sample": "256 particles (seed 5), t=0, direct sum softened by dx"},
         "value": n * steps / secs / 1e6,
         "ms_per_step": secs / steps * 1e3,
         "stage_ms_per_step": {k: v[0] / steps for k, v in tm.items() if v[1]},
@@ -147,6 +154,7 @@ def direct_bench(rrg, ics, steps=3):
         err = np.linalg.norm(acc0[force][idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
         e = energy0[force]["Ek"] + energy0[force]["Ep"]
         out[force]["force_rel_err_median"] = float(np.median(err))
+        out[force]["force_rel_err_p99"] = float(np.percentile(err, 99))
         out[force]["force_rel_err_max"] = float(np.max(err))
         out[force]["energy_rel_err_vs_fp64_exact"] = abs(e - e_ref) / abs(e_ref)
     for f32 in ("mixed", "fp32"):
@@ -254,6 +262,56 @@ def adjoint_bench(rrg, ics, n=65_536, steps=200, every=20):
     return {"mixed": out["mixed"], "fp64": out["fp64"], "mixed_grad_rel_err_vs_fp64": errs,
             "config": {"workload": "Plummer N=65536, eps=0.01, dt=1e-3, 200 steps, checkpoint every 20, "
                                    "loss = mean |x_T - x_T(target masses)|^2", "n": n, "steps": steps}}
+
+
+def wave_scaling(rrg, ics, sizes=(256, 512), steps=2):
+    """The stencil kernel beyond the Infinity Cache: the headline disk on
+    256^3 and 512^3 grids over the same [-8,8]^3 box (20 setup iterations,
+    200 per step, fp32), per-launch time from the stage events."""
+    pos, vel, m = ics.exp_disk(PM["n"], seed=1)
+    out = {}
+    for nx in sizes:
+        wave = dict(diss_base=PM["diss_base"], diss_extra=PM["diss_extra"], wave_speed=C, setup_iterations=20,
+                    iterations=PM["iterations"], tolerance=0.0)
+        p = rrg.make_params(dt=PM["dt"], grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real="fp32")
+        with rrg.Simulation(p, pos, vel, m) as sim:
+            sim.precalc()
+            sim.calc_step(1)
+            sim.synchronize()
+            sim.set_timing(True)
+            sim.calc_step(steps)
+            tm = sim.timing()
+        ms, launches = tm["wave"]
+        us = ms * 1e3 / launches
+        gbs = 16.0 * nx ** 3 / (us * 1e-6) / 1e9
+        out[str(nx)] = {"wave_us_per_launch": us, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                        "cells": nx ** 3, "working_set_mib": 16 * nx ** 3 / 2 ** 20}
+    return out
+
+
+def c1_bench(rrg, ics, n=4096, steps=100):
+    """Plain FP64 direct sum: Plummer N=4096 (seed 0), eps=0.01, dt=1e-3, 100
+    steps; interactions/s over the steps and the relative energy drift."""
+    pos, vel, m = ics.plummer(n, seed=0)
+    s2 = np.full(n, 0.01 ** 2)
+    out = {}
+    for force in ("fp64_exact", "fp64"):
+        p = rrg.make_params(dt=1e-3, force=force)
+        with rrg.Simulation(p, pos, vel, m, s2) as sim:
+            sim.precalc()
+            c0 = sim.conservation()
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.calc_step(steps)
+            sim.synchronize()
+            t1 = time.perf_counter()
+            c1 = sim.conservation()
+        e0, e1 = c0["Ek"] + c0["Ep"], c1["Ek"] + c1["Ep"]
+        out[force] = {"ginteractions_per_s": float(n) * n * steps / (t1 - t0) / 1e9, "ms_per_step": (t1 - t0) / steps * 1e3,
+                      "energy_drift": abs(e1 / e0 - 1.0)}
+    out["config"] = {"workload": "Plummer N=4096 (seed 0), eps=0.01, dt=1e-3, 100 steps, FP64", "n": n,
+                     "steps": steps}
+    return out
 
 
 def pm_adjoint_bench(rrg, ics, n=262_144, steps=200, every=20):
@@ -433,6 +491,7 @@ def main():
                           "frac": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "stage_ms_per_step": pm["stage_ms_per_step"],
         "wave_avg_launch_us": pm["wave_avg_launch_us"],
+        "pm_force_err_vs_fp64_direct": pm["pm_force_err_vs_fp64_direct"],
     }
     if not args.no_direct:
         d = direct_bench(rrg, ics)
@@ -441,6 +500,8 @@ def main():
         for k in ("fp64", "fp64_exact"):
             d[k]["fp64_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP64_PEAK_TFLOPS
         line["direct_sum"] = d
+        line["direct_sum_c1"] = c1_bench(rrg, ics)
+        line["wave_scaling"] = wave_scaling(rrg, ics)
         line["ensemble"] = ensemble_bench(rrg, ics)
         line["adjoint"] = adjoint_bench(rrg, ics)
         line["adjoint_pm"] = pm_adjoint_bench(rrg, ics)

@@ -226,10 +226,10 @@ __global__ __launch_bounds__(256) void k_adj_gradient_t(int NX, double coef, con
 
 // One transposed stencil iteration (k_wave's z-march with no source), and
 // the source adjoint sbar (= cur on the first iteration of a grid build,
-// += cur afterwards).
-constexpr int kZB = 8;
+// += cur afterwards).  Same z-march depth rule as the forward stencil.
+inline int adj_wave_zb(int nx) { return nx <= 96 ? 2 : 4; }
 
-template <class real, bool PERIODIC, bool FIRST>
+template <class real, bool PERIODIC, bool FIRST, int ZB>
 __global__ __launch_bounds__(256) void k_adj_wave(real* __restrict__ pn, const real* __restrict__ p,
                                                   const real* __restrict__ po, real* __restrict__ sbar, int NX,
                                                   WaveK<real> k) {
@@ -243,11 +243,11 @@ __global__ __launch_bounds__(256) void k_adj_wave(real* __restrict__ pn, const r
     const int xm = nb(ix - 1), xp = nb(ix + 1), ym = nb(iy - 1), yp = nb(iy + 1);
     const bool edge_xy = ix == 0 || ix == NX - 1 || iy == 0 || iy == NX - 1;
     const size_t col = size_t(ix) + iy * sy;
-    const int z0 = bz * kZB;
+    const int z0 = bz * ZB;
     real below = p[col + nb(z0 - 1) * sz];
     real cen = p[col + size_t(z0) * sz];
 #pragma unroll 2
-    for (int t = 0; t < kZB; ++t) {
+    for (int t = 0; t < ZB; ++t) {
         const int iz = z0 + t;
         if (iz >= NX) break;
         const real above = p[col + nb(iz + 1) * sz];
@@ -350,14 +350,16 @@ template <class real> void adj_gradient_t(hipStream_t s, int nx, real dx, bool p
 template <class real>
 void adj_wave(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* pn, const real* p, const real* po,
               real* sbar, bool first) {
-    const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned((nx + kZB - 1) / kZB));
+    const int zb = adj_wave_zb(nx);
+    const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned((nx + zb - 1) / zb));
     const dim3 block(64, 4, 1);
-    if (periodic) {
-        if (first) k_adj_wave<real, true, true><<<grid, block, 0, s>>>(pn, p, po, sbar, nx, k);
-        else k_adj_wave<real, true, false><<<grid, block, 0, s>>>(pn, p, po, sbar, nx, k);
+    auto go = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, sbar, nx, k); };
+    if (zb == 2) {
+        if (periodic) first ? go(k_adj_wave<real, true, true, 2>) : go(k_adj_wave<real, true, false, 2>);
+        else first ? go(k_adj_wave<real, false, true, 2>) : go(k_adj_wave<real, false, false, 2>);
     } else {
-        if (first) k_adj_wave<real, false, true><<<grid, block, 0, s>>>(pn, p, po, sbar, nx, k);
-        else k_adj_wave<real, false, false><<<grid, block, 0, s>>>(pn, p, po, sbar, nx, k);
+        if (periodic) first ? go(k_adj_wave<real, true, true, 4>) : go(k_adj_wave<real, true, false, 4>);
+        else first ? go(k_adj_wave<real, false, true, 4>) : go(k_adj_wave<real, false, false, 4>);
     }
     RRG_LAUNCH_CHECK();
 }

@@ -264,10 +264,12 @@ __device__ __forceinline__ void xcd_block(int nbx, int nby, int nbz, int& bx, in
 }
 
 // Each thread owns one (x, y) column and marches ZB planes in z, keeping
-// phi at z-1, z, z+1 in registers; x/y neighbours come from the cache.
-constexpr int kWaveZB = 8;
+// phi at z-1, z, z+1 in registers; x/y neighbours come from the cache.  ZB
+// trades the z-halo re-reads (ZB+2 planes per ZB) against the workgroup
+// count: 2 planes up to 96^3 (enough workgroups to fill 256 CUs), 4 above.
+inline int wave_zb(int nx) { return nx <= 96 ? 2 : 4; }
 
-template <class real, bool PERIODIC, bool CHECK>
+template <class real, bool PERIODIC, bool CHECK, int ZB>
 __global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real* __restrict__ p,
                                               const real* __restrict__ po, const real* __restrict__ src, int NX,
                                               WaveK<real> k, typename MaxBits<real>::T* __restrict__ maxbits) {
@@ -284,11 +286,11 @@ __global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real*
         const int xm = nb(ix - 1), xp = nb(ix + 1), ym = nb(iy - 1), yp = nb(iy + 1);
         const bool edge_xy = ix == 0 || ix == NX - 1 || iy == 0 || iy == NX - 1;
         const size_t col = size_t(ix) + iy * sy;
-        const int z0 = bz * kWaveZB;
+        const int z0 = bz * ZB;
         real below = p[col + nb(z0 - 1) * sz];
         real cen = p[col + size_t(z0) * sz];
 #pragma unroll 2
-        for (int t = 0; t < kWaveZB; ++t) {
+        for (int t = 0; t < ZB; ++t) {
             const int iz = z0 + t;
             if (iz >= NX) break;
             const real above = p[col + nb(iz + 1) * sz];
@@ -522,14 +524,16 @@ void wave_iteration(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, 
                     const real* po, const real* src, void* maxbits) {
     using MB = typename MaxBits<real>::T;
     MB* mb = reinterpret_cast<MB*>(maxbits);
-    const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned((nx + kWaveZB - 1) / kWaveZB));
+    const int zb = wave_zb(nx);
+    const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned((nx + zb - 1) / zb));
     const dim3 block(64, 4, 1);
-    if (periodic) {
-        if (mb) k_wave<real, true, true><<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb);
-        else k_wave<real, true, false><<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb);
+    auto go = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb); };
+    if (zb == 2) {
+        if (periodic) mb ? go(k_wave<real, true, true, 2>) : go(k_wave<real, true, false, 2>);
+        else mb ? go(k_wave<real, false, true, 2>) : go(k_wave<real, false, false, 2>);
     } else {
-        if (mb) k_wave<real, false, true><<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb);
-        else k_wave<real, false, false><<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb);
+        if (periodic) mb ? go(k_wave<real, true, true, 4>) : go(k_wave<real, true, false, 4>);
+        else mb ? go(k_wave<real, false, true, 4>) : go(k_wave<real, false, false, 4>);
     }
     RRG_LAUNCH_CHECK();
 }
