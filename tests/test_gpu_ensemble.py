@@ -128,3 +128,27 @@ def test_pm_graph_replay_bitwise_equals_eager(rrg):
     for (a, pa), (b, pb) in zip(res[False][0], res[True][0]):
         assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
         assert np.array_equal(pa, pb)
+
+
+def test_ensemble_ragged_tiny_systems(rrg, orc):
+    """Systems of 1, 2 and 3 particles next to larger ones (items shorter
+    than one tile, single-particle systems with only the self pair)."""
+    sizes = [1, 2, 3, 257, 64, 1, 700]
+    rng = np.random.default_rng(21)
+    parts = [ics.plummer(max(k, 2), rng=rng) for k in sizes]
+    pos = np.concatenate([p[:k] for (p, _, _), k in zip(parts, sizes)])
+    vel = np.concatenate([v[:k] for (_, v, _), k in zip(parts, sizes)])
+    m = np.concatenate([q[:k] for (_, _, q), k in zip(parts, sizes)])
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    s2 = np.full(len(m), 1e-3)
+    for force, tol in (("fp64", 1e-12), ("mixed", 1e-7)):
+        p = rrg.make_params(dt=1e-3, force=force)
+        with rrg.Simulation(p, pos, vel, m, s2, offsets=off) as sim:
+            sim.precalc()
+            sim.calc_step(5)
+            st = sim.state()
+        for g in range(len(sizes)):
+            a, b = off[g], off[g + 1]
+            opos, ovel, _, _ = orc.nbody_run(pos[a:b], vel[a:b], m[a:b], s2[a:b], 1e-3, 5)
+            assert np.max(np.abs(st["pos"][a:b] - opos)) < tol, (force, g)
+            assert np.max(np.abs(st["vel"][a:b] - ovel)) < 10 * tol, (force, g)

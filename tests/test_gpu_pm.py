@@ -157,3 +157,46 @@ def test_pm_resort_and_state_roundtrip(rrg, orc):
         sim.set_state(pos=newp, vel=vel)
         st = sim.state()
         assert np.array_equal(st["pos"], newp) and np.array_equal(st["vel"], vel)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_pm_particles_outside_the_box_bitwise(rrg, orc, dtype):
+    """Particles beyond the outer cell centres and beyond the box: the CIC
+    clamps (rm_p2mesh.cu:196-199), the gather extrapolates from clamped
+    indices (rm_mesh2p.cu:76-86); both must match the oracle bitwise."""
+    pos, vel, m = ics.exp_disk(4000, seed=2)
+    rng = np.random.default_rng(4)
+    far = rng.uniform(-11.0, 11.0, (200, 3))  # box is [-8, 8]^3
+    pos[:200] = far
+    pos[200:210] = [[7.999, -7.999, 0.0]] * 10  # inside the box, past the last cell centre
+    wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=30, iterations=8, tolerance=0.0)
+    real = "fp32" if dtype == np.float32 else "fp64"
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=16, dx=1.0), wave=wave, real=real)
+    with rrg.Simulation(p, pos, vel, m) as sim:
+        sim.precalc()
+        sim.sync_particles_and_grid()
+        sim.calc_step(4)
+        sim.sync_particles_and_grid()
+        st = sim.state()
+        e = sim.stats()["fixed_exponent"]
+    o = orc.pm_run(pos, vel, m, 4, nx=16, dx=1.0, dt=5e-3, diss_base=0.785, wave_speed=C, setup_iterations=30,
+                   iterations=8, fixed_e=e, dtype=dtype)
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(st[k].astype(dtype), o[k]), k
+
+
+def test_pm_minimum_grid(rrg, orc):
+    """nx = 2: every cell is a face cell (one-sided gradient everywhere)."""
+    pos, vel, m = ics.exp_disk(500, seed=5)
+    pos = pos * 0.1
+    wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=10, iterations=4, tolerance=0.0)
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=2, dx=1.0), wave=wave)
+    with rrg.Simulation(p, pos, vel, m) as sim:
+        sim.precalc()
+        sim.sync_particles_and_grid()
+        sim.calc_step(3)
+        st = sim.state()
+        e = sim.stats()["fixed_exponent"]
+    o = orc.pm_run(pos, vel, m, 3, nx=2, dx=1.0, dt=5e-3, diss_base=0.785, wave_speed=C, setup_iterations=10,
+                   iterations=4, fixed_e=e)
+    assert np.array_equal(st["pos"], o["pos"]) and np.array_equal(st["vel"], o["vel"])
