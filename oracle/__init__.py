@@ -39,10 +39,20 @@ class PmConfig(ctypes.Structure):
         ("iterations", ctypes.c_int),
         ("check_step", ctypes.c_int),
         ("hold", ctypes.c_int),
-        ("periodic", ctypes.c_int),
+        ("boundary", ctypes.c_int),
         ("fixed_e", ctypes.c_int),
         ("dt", ctypes.c_double),
+        ("diss_extra", ctypes.c_double),
     ]
+
+
+BOUNDARY = {"zero_dirichlet": 0, "periodic": 1, "monopole_dirichlet": 2, "monopole_radiative": 3}
+
+
+def _bc(periodic, boundary):
+    if boundary is None:
+        return 1 if periodic else 0
+    return BOUNDARY[boundary] if isinstance(boundary, str) else int(boundary)
 
 
 def build() -> None:
@@ -188,6 +198,21 @@ def wave_iteration(nx, dx, diss, c, dt, periodic, rho_mean_, phi, phi_old, mass,
     return out
 
 
+def wave_iteration_bc(nx, dx, diss, c, dt, boundary, phi, phi_old, mass, *, bc_frac=0.0, diss_extra=0.0,
+                      rho_mean_=0.0, bc_mass=1.0, com=(0.0, 0.0, 0.0), r_min=None, dtype=np.float64):
+    """wave_diss_iteration (wave.cu:106-193) for any boundary type, with the
+    absorbing layer (bc_frac, diss_extra) and the monopole parameters."""
+    phi = _c(phi, dtype); phi_old = _c(phi_old, dtype); mass = _c(mass, dtype)
+    out = np.empty_like(phi)
+    r = lambda x: _r(x, dtype)  # noqa: E731
+    com = np.ascontiguousarray(com, dtype=dtype)
+    rmin = 0.5 * dx if r_min is None else r_min
+    getattr(lib(), f"orc_wave_iteration_bc_{_suf(dtype)}")(
+        _int(nx), r(dx), r(bc_frac), r(diss), r(diss_extra), r(c), r(dt), _int(_bc(True, boundary)), r(rho_mean_),
+        r(bc_mass), _p(com), r(rmin), _p(out), _p(phi), _p(phi_old), _p(mass))
+    return out
+
+
 def solve_wave(nx, dx, diss, c, dt, periodic, rho_mean_, prev, curr, mass, max_iter, tol,
                check_step=30, hold=3, dtype=np.float64):
     """solve_wave_equation (wave.cu:230-297). Returns (prev, curr, iterations)."""
@@ -245,13 +270,16 @@ def nbody_run(pos, vel, mass, soft2, dt, nsteps, dtype=np.float64):
 
 
 def pm_run(pos, vel, mass, nsteps, *, nx, dx, dt, diss_base, wave_speed, setup_iterations, iterations,
-           tolerance=0.0, check_step=30, hold=3, bc_frac=0.0, periodic=True, fixed_e=61, dtype=np.float64):
+           tolerance=0.0, check_step=30, hold=3, bc_frac=0.0, periodic=True, fixed_e=61, dtype=np.float64,
+           boundary=None, diss_extra=0.0):
     """PM strategy: precalc + sync + nsteps x (predict, deposit, wave, gradient, gather, correct).
+    ``boundary`` (name or BoundaryType value) overrides ``periodic``.
 
     Returns dict(pos, vel, acc, phi, phi_prev, grid_acc, grav, iterations)."""
     cf = PmConfig(nx=nx, dx=dx, bc_frac=bc_frac, diss_base=diss_base, wave_speed=wave_speed,
                   tolerance=tolerance, setup_iterations=setup_iterations, iterations=iterations,
-                  check_step=check_step, hold=hold, periodic=int(periodic), fixed_e=fixed_e, dt=dt)
+                  check_step=check_step, hold=hold, boundary=_bc(periodic, boundary), fixed_e=fixed_e, dt=dt,
+                  diss_extra=diss_extra)
     pos = _c(pos, dtype).copy(); vel = _c(vel, dtype).copy(); mass = _c(mass, dtype)
     n = len(mass)
     ncell = nx ** 3
@@ -267,20 +295,33 @@ def pm_run(pos, vel, mass, nsteps, *, nx, dx, dt, diss_base, wave_speed, setup_i
 
 
 def pm_adjoint(pos, vel, mass, nsteps, gx_T, gv_T=None, *, nx, dx, dt, diss_base, wave_speed, setup_iterations,
-               iterations, tolerance=0.0, check_step=30, hold=3, bc_frac=0.0, periodic=True, fixed_e=61):
+               iterations, tolerance=0.0, check_step=30, hold=3, bc_frac=0.0, periodic=True, fixed_e=61,
+               boundary=None, diss_extra=0.0):
     """Reverse-mode gradient of pm_run (FP64): returns (dL/dx0, dL/dv0, dL/dm)
-    for a loss with dL/dx_T = gx_T and dL/dv_T = gv_T."""
+    for a loss with dL/dx_T = gx_T and dL/dv_T = gv_T.  Periodic and
+    zero-Dirichlet boundaries only."""
     cf = PmConfig(nx=nx, dx=dx, bc_frac=bc_frac, diss_base=diss_base, wave_speed=wave_speed,
                   tolerance=tolerance, setup_iterations=setup_iterations, iterations=iterations,
-                  check_step=check_step, hold=hold, periodic=int(periodic), fixed_e=fixed_e, dt=dt)
+                  check_step=check_step, hold=hold, boundary=_bc(periodic, boundary), fixed_e=fixed_e, dt=dt,
+                  diss_extra=diss_extra)
     pos = _c(pos, np.float64); vel = _c(vel, np.float64); mass = _c(mass, np.float64)
     n = len(mass)
     gx = np.zeros((n, 3)) if gx_T is None else _c(gx_T, np.float64)
     gv = np.zeros((n, 3)) if gv_T is None else _c(gv_T, np.float64)
     gx0 = np.empty((n, 3)); gv0 = np.empty((n, 3)); gm = np.empty(n)
-    lib().orc_pm_adjoint_f64(ctypes.byref(cf), _i64(n), _p(pos), _p(vel), _p(mass), _int(nsteps), _p(gx), _p(gv),
-                             _p(gx0), _p(gv0), _p(gm))
+    rc = lib().orc_pm_adjoint_f64(ctypes.byref(cf), _i64(n), _p(pos), _p(vel), _p(mass), _int(nsteps), _p(gx),
+                                  _p(gv), _p(gx0), _p(gv0), _p(gm))
+    if rc != 0:
+        raise ValueError("pm_adjoint: periodic and zero-Dirichlet boundaries only")
     return gx0, gv0, gm
+
+
+def center_of_mass(pos, mass, fixed_e):
+    """Order-independent fixed-point centre of mass used by the monopole boundaries.  FP64."""
+    pos = _c(pos, np.float64); mass = _c(mass, np.float64)
+    out = np.empty(3)
+    lib().orc_center_of_mass_f64(_i64(len(mass)), _p(pos), _p(mass), _int(fixed_e), _p(out))
+    return out
 
 
 def gradient_t(gbar, nx, dx):

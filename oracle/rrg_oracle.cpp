@@ -313,12 +313,97 @@ template <class real> WaveCoef<real> wave_coef(real diss, real C, real DT, real 
 
 inline int wrapi(int i, int n) { return (i < 0) ? i + n : ((i >= n) ? i - n : i); }  // wave.cu:22-24
 
+// Boundary description (co_boundary.cuh:4-27): the reference's BoundaryType
+// values, the monopole mass and centre, r_min, and the periodic mean density.
+enum { BC_ZERO_DIRICHLET = 0, BC_PERIODIC = 1, BC_MONOPOLE_DIRICHLET = 2, BC_MONOPOLE_RADIATIVE = 3 };
+
+template <class real> struct WaveBC {
+    int type;
+    real mass;
+    V3<real> com;
+    real r_min;
+    real rho_mean;
+};
+
+// Everything wave_diss_iteration reads from its __constant__ copies
+// (grid_info_, wave_params_, bc_params_; co_device_structs.cu:7-13).
+template <class real> struct WaveModel {
+    Grid<real> g;
+    real diss_base, diss_extra, C, DT;
+    WaveBC<real> bc;
+};
+
+template <class real> inline real coord(const Grid<real>& g, int i) {  // wave.cu:18-20
+    return g.domain_min + i * g.dx;
+}
+
+template <class real> inline real layer_diss(const WaveModel<real>& w, real x) {  // wave.cu:27-33
+    const real x_in = real(0.5) * w.g.sim_l;
+    const real d = std::fabs(x) - x_in;
+    return (d > 0) ? w.diss_extra * d * d : real(0);
+}
+
+template <class real> inline real monopole_phi(const WaveModel<real>& w, real x, real y, real z) {  // wave.cu:36-41
+    const real G = 1.;
+    const real rx = x - w.bc.com.x, ry = y - w.bc.com.y, rz = z - w.bc.com.z;  // distance(com, p)
+    const real dist = std::sqrt(rx * rx + ry * ry + rz * rz);
+    const real r = std::fmax(dist, w.bc.r_min);
+    return -G * w.bc.mass / r;
+}
+
+template <class real> inline real psi_at(const WaveModel<real>& w, const real* phi, int i, int j, int k) {
+    const int NX = w.g.nx;  // wave.cu:44-49
+    return phi[AT(i, j, k, NX)] - monopole_phi(w, coord(w.g, i), coord(w.g, j), coord(w.g, k));
+}
+
 template <class real>
-void wave_iteration(int nx, real dx, real diss, real C, real DT, int periodic, real rho_mean,
-                    real* phi_new, const real* phi, const real* phi_old, const real* mass) {
-    const int NX = nx;
-    const real DX = dx;
-    const WaveCoef<real> k = wave_coef<real>(diss, C, DT, DX);
+inline real upwind_axis(const WaveModel<real>& w, const real* phi, real psi0, real n_comp, int ix, int iy, int iz,
+                        int axis) {  // wave.cu:53-76
+    const int NX = w.g.nx;
+    const real DX = w.g.dx;
+    const int i0 = (axis == 0) ? ix : (axis == 1) ? iy : iz;
+    int in = (n_comp > 0) ? i0 - 1 : i0 + 1;
+    if (in < 0 || in >= NX) in = 2 * i0 - in;
+    int jx = ix, jy = iy, jz = iz;
+    if (axis == 0) jx = in;
+    else if (axis == 1) jy = in;
+    else jz = in;
+    const real d = (psi0 - psi_at(w, phi, jx, jy, jz)) / ((i0 - in) * DX);
+    return n_comp * d;
+}
+
+template <class real> real radiative_boundary(const WaveModel<real>& w, const real* phi, int ix, int iy, int iz) {
+    const real C = w.C, DT = w.DT;  // wave.cu:80-104
+    const real x = coord(w.g, ix), y = coord(w.g, iy), z = coord(w.g, iz);
+    const real rx = x - w.bc.com.x, ry = y - w.bc.com.y, rz = z - w.bc.com.z;
+    const real r = std::fmax(std::sqrt(rx * rx + ry * ry + rz * rz), w.bc.r_min);
+    const real nx = rx / r, ny = ry / r, nz = rz / r;
+    const real psi0 = psi_at(w, phi, ix, iy, iz);
+    const real dpsi_dr = upwind_axis(w, phi, psi0, nx, ix, iy, iz, 0) + upwind_axis(w, phi, psi0, ny, ix, iy, iz, 1) +
+                         upwind_axis(w, phi, psi0, nz, ix, iy, iz, 2);
+    const real psi_new = psi0 - C * DT * (dpsi_dr + psi0 / r);
+    return psi_new + monopole_phi(w, x, y, z);
+}
+
+// Per-cell scheme coefficients (wave.cu:155-171): the absorbing layer adds
+// to the dissipation on every non-periodic grid (periodic_keep_layer is 0).
+template <class real> WaveCoef<real> cell_coef(const WaveModel<real>& w, int ix, int iy, int iz) {
+    real diss = w.diss_base;
+    if (w.bc.type != BC_PERIODIC) {
+        diss += layer_diss(w, coord(w.g, ix));
+        diss += layer_diss(w, coord(w.g, iy));
+        diss += layer_diss(w, coord(w.g, iz));
+    }
+    return wave_coef<real>(diss, w.C, w.DT, w.g.dx);
+}
+
+// wave_diss_iteration (wave.cu:106-193) for every boundary type.
+template <class real>
+void wave_iteration_model(const WaveModel<real>& w, real* phi_new, const real* phi, const real* phi_old,
+                          const real* mass) {
+    const int NX = w.g.nx;
+    const real DX = w.g.dx;
+    const bool periodic = w.bc.type == BC_PERIODIC;
     const real G = 1.;
 #pragma omp parallel for schedule(static)
     for (int iz = 0; iz < NX; ++iz) {
@@ -326,13 +411,21 @@ void wave_iteration(int nx, real dx, real diss, real C, real DT, int periodic, r
             for (int ix = 0; ix < NX; ++ix) {
                 const bool on_edge = ix == 0 || ix == NX - 1 || iy == 0 || iy == NX - 1 ||
                                      iz == 0 || iz == NX - 1;
-                if (!periodic && on_edge) {  // ZeroDirichlet (wave.cu:140-145)
-                    phi_new[AT(ix, iy, iz, NX)] = 0.;
+                if (!periodic && on_edge) {  // wave.cu:129-147
+                    real v;
+                    if (w.bc.type == BC_MONOPOLE_DIRICHLET)
+                        v = monopole_phi(w, coord(w.g, ix), coord(w.g, iy), coord(w.g, iz));
+                    else if (w.bc.type == BC_MONOPOLE_RADIATIVE)
+                        v = radiative_boundary(w, phi, ix, iy, iz);
+                    else
+                        v = 0.;
+                    phi_new[AT(ix, iy, iz, NX)] = v;
                     continue;
                 }
                 real rho = mass[AT(ix, iy, iz, NX)] / (DX * DX * DX);
-                if (periodic) rho -= rho_mean;
+                if (periodic) rho -= w.bc.rho_mean;
                 const real f = 4 * kPi * G * rho;
+                const WaveCoef<real> k = cell_coef(w, ix, iy, iz);
                 int xm = ix - 1, xp = ix + 1, ym = iy - 1, yp = iy + 1, zm = iz - 1, zp = iz + 1;
                 if (periodic) {
                     xm = wrapi(xm, NX); xp = wrapi(xp, NX);
@@ -347,6 +440,26 @@ void wave_iteration(int nx, real dx, real diss, real C, real DT, int periodic, r
             }
         }
     }
+}
+
+template <class real>
+WaveModel<real> plain_model(int nx, real dx, real diss, real C, real DT, int periodic, real rho_mean) {
+    WaveModel<real> w{};
+    w.g = make_grid<real>(nx, dx, real(0));
+    w.diss_base = diss;
+    w.diss_extra = 0;
+    w.C = C;
+    w.DT = DT;
+    w.bc.type = periodic ? BC_PERIODIC : BC_ZERO_DIRICHLET;
+    w.bc.rho_mean = rho_mean;
+    return w;
+}
+
+// The periodic / zero-Dirichlet iteration with no absorbing layer.
+template <class real>
+void wave_iteration(int nx, real dx, real diss, real C, real DT, int periodic, real rho_mean,
+                    real* phi_new, const real* phi, const real* phi_old, const real* mass) {
+    wave_iteration_model(plain_model<real>(nx, dx, diss, C, DT, periodic, rho_mean), phi_new, phi, phi_old, mass);
 }
 
 // SteadyStateChecker::calc_max_diff (wave_steady_checker.cu:95-127):
@@ -368,16 +481,16 @@ template <class real> double steady_err(size_t ncell, const real* u_new, const r
 // bufs = {prev, curr, next}; on return bufs[1] holds the newest field.  The
 // rotation is done by permuting the three pointers, and `perm` reports it.
 template <class real>
-int solve_wave(int nx, real dx, real diss, real C, real DT, int periodic, real rho_mean,
-               real* prev, real* curr, real* next, const real* mass, int max_iter, real tol,
-               int check_step, int hold, int* perm) {
+int solve_wave_model(const WaveModel<real>& w, real* prev, real* curr, real* next, const real* mass, int max_iter,
+                     real tol, int check_step, int hold, int* perm) {
+    const int nx = w.g.nx;
     real* b[3] = {prev, curr, next};
     int id[3] = {0, 1, 2};
     int hold_count = 0;
     const size_t ncell = size_t(nx) * nx * nx;
     int iter = 0;
     for (iter = 0; iter < max_iter; ++iter) {
-        wave_iteration<real>(nx, dx, diss, C, DT, periodic, rho_mean, b[2], b[1], b[0], mass);
+        wave_iteration_model<real>(w, b[2], b[1], b[0], mass);
         std::swap(b[0], b[1]); std::swap(id[0], id[1]);
         std::swap(b[1], b[2]); std::swap(id[1], id[2]);
         if (iter && iter % check_step == 0) {
@@ -388,6 +501,14 @@ int solve_wave(int nx, real dx, real diss, real C, real DT, int periodic, real r
     }
     for (int q = 0; q < 3; ++q) perm[q] = id[q];
     return iter;  // number of stencil iterations performed
+}
+
+template <class real>
+int solve_wave(int nx, real dx, real diss, real C, real DT, int periodic, real rho_mean,
+               real* prev, real* curr, real* next, const real* mass, int max_iter, real tol,
+               int check_step, int hold, int* perm) {
+    return solve_wave_model(plain_model<real>(nx, dx, diss, C, DT, periodic, rho_mean), prev, curr, next, mass,
+                            max_iter, tol, check_step, hold, perm);
 }
 
 // calc_acceleration_field_ (rm_p2mesh.cu:238-290): a = -grad(phi), central
@@ -591,10 +712,39 @@ struct PmConfig {
     double dx, bc_frac;
     double diss_base, wave_speed, tolerance;
     int setup_iterations, iterations, check_step, hold;
-    int periodic;
+    int boundary;   // BoundaryType value (co_boundary.cuh:4-9): 0 zero-Dirichlet, 1 periodic, 2, 3 monopole
     int fixed_e;    // fixed-point exponent (deposit resolution 2^-e)
     double dt;      // particle time step (sim_params.dt_dynamics)
+    double diss_extra;
 };
+
+// Centre of mass for the monopole boundaries (calc_mass_center,
+// nbody.cu:294-439, in an order-independent form): each product m*x (FP64)
+// and each mass are rounded once to fixed point and summed in int64.  The
+// moment scale 2^E is the largest keeping n * max|m x| below 2^61, the mass
+// scale 2^e is the deposit's.  No mass gives the origin, as in the reference.
+template <class real>
+V3<real> center_of_mass_fixed(int64_t n, const real* pos, const real* mass, int e_mass) {
+    double mx = 0;
+    for (int64_t q = 0; q < n; ++q)
+        for (int d = 0; d < 3; ++d) mx = std::max(mx, std::fabs((double)mass[q] * (double)pos[3 * q + d]));
+    int ex = 0;
+    std::frexp(std::max(mx * (double)std::max<int64_t>(n, 1), 1e-300), &ex);
+    const int E = std::min(61 - ex, 1000);
+    int64_t S[3] = {0, 0, 0}, Sm = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        for (int d = 0; d < 3; ++d) S[d] += (int64_t)std::rint((double)mass[q] * (double)pos[3 * q + d] * std::ldexp(1.0, E));
+        Sm += (int64_t)std::rint((double)mass[q] * std::ldexp(1.0, e_mass));
+    }
+    const double M = (double)Sm * std::ldexp(1.0, -e_mass);
+    V3<real> c{0, 0, 0};
+    if (M > 0) {
+        c.x = (real)((double)S[0] * std::ldexp(1.0, -E) / M);
+        c.y = (real)((double)S[1] * std::ldexp(1.0, -E) / M);
+        c.z = (real)((double)S[2] * std::ldexp(1.0, -E) / M);
+    }
+    return c;
+}
 
 // What the reverse sweep of a PM run needs per grid build (entry 0 = precalc,
 // entry s = step s): the positions the mass was deposited from (after the
@@ -616,6 +766,14 @@ void pm_run(const PmConfig& cf, int64_t n, real* pos, real* vel, const real* mas
     const real diss = (real)cf.diss_base;
     const real dtp = (real)cf.dt;
     const real rho_mean = rho_mean_periodic<real>(1., nx, dx);
+    WaveModel<real> wm{};
+    wm.g = make_grid<real>(nx, dx, bcf);
+    wm.diss_base = diss;
+    wm.diss_extra = (real)cf.diss_extra;
+    wm.C = C;
+    wm.DT = DT;
+    wm.bc.type = cf.boundary;
+    wm.bc.rho_mean = rho_mean;
     std::vector<int64_t> mfix(ncell);
     std::vector<real> m(ncell), pa(ncell, 0), pb(ncell, 0), pc(ncell, 0), gacc(3 * ncell);
     std::vector<real> acc(3 * n), vp(3 * n);
@@ -627,9 +785,8 @@ void pm_run(const PmConfig& cf, int64_t n, real* pos, real* vel, const real* mas
     auto solve = [&](int iters) {
         int perm[3];
         real* s[3] = {b[0], b[1], b[2]};
-        const int done = solve_wave<real>(nx, dx, diss, C, DT, cf.periodic, rho_mean, s[0], s[1], s[2],
-                                          m.data(), iters, (real)cf.tolerance, cf.check_step, cf.hold,
-                                          perm);
+        const int done = solve_wave_model<real>(wm, s[0], s[1], s[2], m.data(), iters, (real)cf.tolerance,
+                                                cf.check_step, cf.hold, perm);
         real* nb[3] = {s[perm[0]], s[perm[1]], s[perm[2]]};
         b[0] = nb[0]; b[1] = nb[1]; b[2] = nb[2];
         return done;
@@ -643,8 +800,14 @@ void pm_run(const PmConfig& cf, int64_t n, real* pos, real* vel, const real* mas
             tape->iters.push_back(done);
         }
     };
-    // precalc
+    // precalc (rrgsim_integration_rect.cuh:53-83); the monopole boundaries
+    // take total mass 1 and the centre of mass at precalc
     deposit();
+    if (cf.boundary == BC_MONOPOLE_DIRICHLET || cf.boundary == BC_MONOPOLE_RADIATIVE) {
+        wm.bc.mass = 1.;
+        wm.bc.com = center_of_mass_fixed<real>(n, pos, mass, cf.fixed_e);
+        wm.bc.r_min = real(0.5) * dx;
+    }
     int done = solve(cf.setup_iterations);
     record(done);
     gradient<real>(nx, dx, b[1], gacc.data());
@@ -775,8 +938,9 @@ InterpW cic_weights(const double* p, const Grid<double>& g) {
     return r;
 }
 
-void pm_adjoint(const PmConfig& cf, int64_t n, const double* x0, const double* v0, const double* mass, int nsteps,
-                const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) {
+int pm_adjoint(const PmConfig& cf, int64_t n, const double* x0, const double* v0, const double* mass, int nsteps,
+               const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) {
+    if (cf.boundary != BC_PERIODIC && cf.boundary != BC_ZERO_DIRICHLET) return -1;  // monopole: not differentiated
     std::vector<double> pos(x0, x0 + 3 * n), vel(v0, v0 + 3 * n);
     PmTape<double> tape;
     pm_run<double>(cf, n, pos.data(), vel.data(), mass, nsteps, nullptr, nullptr, nullptr, nullptr, nullptr, &tape);
@@ -785,13 +949,31 @@ void pm_adjoint(const PmConfig& cf, int64_t n, const double* x0, const double* v
     const double dx = cf.dx, dt = cf.dt;
     const double C = cf.wave_speed;
     const double DT = wave_dt<double>(dx, C);
-    const WaveCoef<double> k = wave_coef<double>(cf.diss_base, C, DT, dx);
-    const double alpha = 4 * kPi * k.k_f / (dx * dx * dx);
     const Grid<double> g = make_grid<double>(nx, dx, cf.bc_frac);
+    WaveModel<double> wm{};
+    wm.g = g;
+    wm.diss_base = cf.diss_base;
+    wm.diss_extra = cf.diss_extra;
+    wm.C = C;
+    wm.DT = DT;
+    wm.bc.type = cf.boundary;
+    // per-cell coefficients (the absorbing layer makes them vary off-periodic)
+    std::vector<double> Km(ncell), Ka(ncell), Ko(ncell), alpha(ncell);
+    for (int iz = 0; iz < nx; ++iz)
+        for (int iy = 0; iy < nx; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const WaveCoef<double> k = cell_coef(wm, ix, iy, iz);
+                const size_t c = AT(ix, iy, iz, nx);
+                Km[c] = k.k_main;
+                Ka[c] = k.k_actual;
+                Ko[c] = k.k_old;
+                alpha[c] = 4 * kPi * k.k_f / (dx * dx * dx);  // d source / d cell mass
+            }
+    const bool periodic = cf.boundary == BC_PERIODIC;
     std::vector<double> X(3 * n, 0.0), V(3 * n, 0.0), A(3 * n, 0.0), GM(n, 0.0);
     if (gxT) std::memcpy(X.data(), gxT, sizeof(double) * 3 * n);
     if (gvT) std::memcpy(V.data(), gvT, sizeof(double) * 3 * n);
-    std::vector<double> G(3 * ncell), Gb(3 * ncell), phib(ncell), sbar(ncell), zero(ncell, 0.0);
+    std::vector<double> G(3 * ncell), Gb(3 * ncell), phib(ncell), sbar(ncell);
     std::vector<double> mu[3] = {std::vector<double>(ncell, 0.0), std::vector<double>(ncell, 0.0),
                                  std::vector<double>(ncell, 0.0)};
     int ip = 0, ic = 1;  // mu_{k+1}, mu_k
@@ -819,14 +1001,29 @@ void pm_adjoint(const PmConfig& cf, int64_t n, const double* x0, const double* v
         }
         gradient_t(nx, dx, Gb.data(), phib.data());
         for (size_t c = 0; c < ncell; ++c)
-            if (cf.periodic || !edge(c)) mu[ic][c] += phib[c];
+            if (periodic || !edge(c)) mu[ic][c] += phib[c];
         std::fill(sbar.begin(), sbar.end(), 0.0);
         for (int it = 0; it < tape.iters[s]; ++it) {
             for (size_t c = 0; c < ncell; ++c) sbar[c] += mu[ic][c];
             const int in = 3 - ip - ic;
-            // the stencil with a zero source is the transposed iteration
-            wave_iteration<double>(nx, dx, cf.diss_base, C, DT, cf.periodic, 0.0, mu[in].data(), mu[ic].data(),
-                                   mu[ip].data(), zero.data());
+            // transposed iteration: mu_{k-1}[c] = B(sum_n Km[n] mu_k[n] + Ka[c] mu_k[c] + Ko[c] mu_{k+1}[c])
+            const double* mk = mu[ic].data();
+            for (int iz = 0; iz < nx; ++iz)
+                for (int iy = 0; iy < nx; ++iy)
+                    for (int ix = 0; ix < nx; ++ix) {
+                        const size_t c = AT(ix, iy, iz, nx);
+                        if (!periodic && edge(c)) {
+                            mu[in][c] = 0.0;
+                            continue;
+                        }
+                        const int xm = wrapi(ix - 1, nx), xp = wrapi(ix + 1, nx), ym = wrapi(iy - 1, nx),
+                                  yp = wrapi(iy + 1, nx), zm = wrapi(iz - 1, nx), zp = wrapi(iz + 1, nx);
+                        const size_t nb[6] = {AT(xm, iy, iz, nx), AT(xp, iy, iz, nx), AT(ix, ym, iz, nx),
+                                              AT(ix, yp, iz, nx), AT(ix, iy, zm, nx), AT(ix, iy, zp, nx)};
+                        double acc = 0;
+                        for (int q = 0; q < 6; ++q) acc += Km[nb[q]] * mk[nb[q]];
+                        mu[in][c] = acc + Ka[c] * mk[c] + Ko[c] * mu[ip][c];
+                    }
             ip = ic;
             ic = in;
         }
@@ -834,11 +1031,12 @@ void pm_adjoint(const PmConfig& cf, int64_t n, const double* x0, const double* v
             const InterpW W = cic_weights(x + 3 * q, g);
             double mb = 0, dm[3] = {0, 0, 0};
             for (int c = 0; c < 8; ++c) {
-                mb += W.w[c] * sbar[W.cell[c]];
-                for (int d = 0; d < 3; ++d) dm[d] += W.dw[c][d] * sbar[W.cell[c]];
+                const double sa = alpha[W.cell[c]] * sbar[W.cell[c]];
+                mb += W.w[c] * sa;
+                for (int d = 0; d < 3; ++d) dm[d] += W.dw[c][d] * sa;
             }
-            GM[q] += alpha * mb;
-            for (int d = 0; d < 3; ++d) X[3 * q + d] += alpha * mass[q] * dm[d];
+            GM[q] += mb;
+            for (int d = 0; d < 3; ++d) X[3 * q + d] += mass[q] * dm[d];
         }
         if (s > 0) {
             for (int64_t i = 0; i < 3 * n; ++i) {
@@ -851,6 +1049,7 @@ void pm_adjoint(const PmConfig& cf, int64_t n, const double* x0, const double* v
     std::memcpy(gx0, X.data(), sizeof(double) * 3 * n);
     std::memcpy(gv0, V.data(), sizeof(double) * 3 * n);
     std::memcpy(gm, GM.data(), sizeof(double) * n);
+    return 0;
 }
 
 }  // namespace
@@ -887,6 +1086,16 @@ void pm_adjoint(const PmConfig& cf, int64_t n, const double* x0, const double* v
                                              int periodic, real rho_mean, real* pn, const real* p, \
                                              const real* po, const real* m) {                      \
         wave_iteration<real>(nx, dx, diss, c, dt, periodic, rho_mean, pn, p, po, m); }             \
+    ORC_EXPORT void orc_wave_iteration_bc_##SUF(int nx, real dx, real bc_frac, real diss_base,          \
+                                                real diss_extra, real c, real dt, int boundary,            \
+                                                real rho_mean, real bc_mass, const real* com, real r_min,  \
+                                                real* pn, const real* p, const real* po, const real* m) {  \
+        WaveModel<real> w{};                                                                        \
+        w.g = make_grid<real>(nx, dx, bc_frac);                                                     \
+        w.diss_base = diss_base; w.diss_extra = diss_extra; w.C = c; w.DT = dt;                     \
+        w.bc.type = boundary; w.bc.rho_mean = rho_mean; w.bc.mass = bc_mass;                        \
+        w.bc.com = V3<real>{com[0], com[1], com[2]}; w.bc.r_min = r_min;                            \
+        wave_iteration_model<real>(w, pn, p, po, m); }                                              \
     ORC_EXPORT int orc_solve_wave_##SUF(int nx, real dx, real diss, real c, real dt, int periodic, \
                                         real rho_mean, real* prev, real* curr, real* next,         \
                                         const real* m, int max_iter, real tol, int check_step,     \
@@ -923,10 +1132,14 @@ ORC_EXPORT void orc_nbody_adjoint_f64(int64_t n, const double* x0, const double*
                                       const double* gvT, double* gx0, double* gv0, double* gm) {
     nbody_adjoint(n, x0, v0, m, s2, dt, nsteps, gxT, gvT, gx0, gv0, gm);
 }
-ORC_EXPORT void orc_pm_adjoint_f64(const PmConfig* cf, int64_t n, const double* x0, const double* v0,
-                                   const double* m, int nsteps, const double* gxT, const double* gvT,
-                                   double* gx0, double* gv0, double* gm) {
-    pm_adjoint(*cf, n, x0, v0, m, nsteps, gxT, gvT, gx0, gv0, gm);
+ORC_EXPORT int orc_pm_adjoint_f64(const PmConfig* cf, int64_t n, const double* x0, const double* v0,
+                                  const double* m, int nsteps, const double* gxT, const double* gvT,
+                                  double* gx0, double* gv0, double* gm) {
+    return pm_adjoint(*cf, n, x0, v0, m, nsteps, gxT, gvT, gx0, gv0, gm);
+}
+ORC_EXPORT void orc_center_of_mass_f64(int64_t n, const double* pos, const double* m, int e, double* out) {
+    const V3<double> c = center_of_mass_fixed<double>(n, pos, m, e);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 ORC_EXPORT void orc_gradient_t_f64(int nx, double dx, const double* gbar, double* phibar) {
     gradient_t(nx, dx, gbar, phibar);

@@ -103,3 +103,34 @@ def test_gradient_transpose_dot_product(orc, nx):
     lhs = np.sum(orc.gradient(nx, 0.7, phi) * g)
     rhs = np.sum(phi * orc.gradient_t(g, nx, 0.7))
     assert abs(lhs - rhs) < 1e-12 * (abs(lhs) + 1)
+
+
+def test_pm_adjoint_zero_dirichlet_with_absorbing_layer(orc):
+    """Off-periodic grids carry per-cell coefficients (the absorbing layer),
+    so the transposed stencil is L(K_main mu) rather than K_main L(mu) and the
+    source adjoint is scaled per cell."""
+    n, steps = 300, 4
+    pos, vel, m = ics.exp_disk(n, seed=12)
+    cfg = _cfg(False, bc_frac=0.5, diss_extra=4.0)
+    rng = np.random.default_rng(13)
+    gx, gv = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    gx0, gv0, gm = orc.pm_adjoint(pos, vel, m, steps, gx, gv, **cfg)
+    for what in ("x", "v", "m"):
+        d = rng.standard_normal((n, 3)) if what != "m" else m * rng.standard_normal(n)
+        h = 1e-6
+        args = {"x": lambda s: (pos + s * d, vel, m), "v": lambda s: (pos, vel + s * d, m),
+                "m": lambda s: (pos, vel, m + s * d)}[what]
+        fd = (_loss(orc, *args(h), steps, gx, gv, cfg)[0] - _loss(orc, *args(-h), steps, gx, gv, cfg)[0]) / (2 * h)
+        an = float(np.sum({"x": gx0, "v": gv0, "m": gm}[what] * d))
+        assert abs(fd - an) < 1e-6 * (abs(fd) + 1e-3), (what, fd, an)
+    # the layer matters for the gradient
+    gx0_plain, _, _ = orc.pm_adjoint(pos, vel, m, steps, gx, gv, **_cfg(False))
+    assert np.max(np.abs(gx0 - gx0_plain)) > 1e-6 * np.max(np.abs(gx0))
+
+
+def test_pm_adjoint_rejects_monopole_boundaries(orc):
+    pos, vel, m = ics.exp_disk(50, seed=1)
+    cfg = _cfg(True)
+    cfg["boundary"] = "monopole_dirichlet"
+    with pytest.raises(ValueError):
+        orc.pm_adjoint(pos, vel, m, 2, np.ones((50, 3)), None, **cfg)
