@@ -58,9 +58,19 @@ enum { RRG_REAL_F64 = 0, RRG_REAL_F32 = 1 };
  * The FP32 kinds have no counterpart in the reference. */
 enum { RRG_FORCE_F64 = 0, RRG_FORCE_MIXED = 1, RRG_FORCE_FP32 = 2, RRG_FORCE_F64_EXACT = 3 };
 
-/* Wave-equation boundary types (co_boundary.cuh:4-9).  The reference's live
- * configuration is periodic (rrgsim_integration_rect.cuh:65-69). */
-enum { RRG_BC_ZERO_DIRICHLET = 0, RRG_BC_PERIODIC = 1 };
+/* Wave-equation boundary types (co_boundary.cuh:4-9, same values).  The
+ * reference's live configuration is periodic (rrgsim_integration_rect.cuh:65-69).
+ * Off-periodic grids carry the absorbing layer (grid bc_frac, wave diss_extra;
+ * wave.cu:27-33, 155-161); the monopole types take total mass 1 and the centre
+ * of mass at precalc (the reference's disabled call, rrgsim_integration_rect.cuh:57-63):
+ * phi = -M/r on the faces (Dirichlet) or the radiation condition for
+ * phi + M/r (wave.cu:36-104). */
+enum {
+    RRG_BC_ZERO_DIRICHLET = 0,
+    RRG_BC_PERIODIC = 1,
+    RRG_BC_MONOPOLE_DIRICHLET = 2,
+    RRG_BC_MONOPOLE_RADIATIVE = 3
+};
 
 /* sim_params section (co_sim_params.h:6-13; io_json.cpp:13-21). */
 typedef struct {

@@ -30,6 +30,10 @@ FORCE_F64_EXACT = 3
 FORCE_KINDS = {"fp64": 0, "mixed": 1, "fp32": 2, "fp64_exact": 3}
 BC_ZERO_DIRICHLET = 0
 BC_PERIODIC = 1
+BC_MONOPOLE_DIRICHLET = 2
+BC_MONOPOLE_RADIATIVE = 3
+BOUNDARIES = {"zero_dirichlet": BC_ZERO_DIRICHLET, "periodic": BC_PERIODIC, "monopole_dirichlet": BC_MONOPOLE_DIRICHLET,
+              "monopole_radiative": BC_MONOPOLE_RADIATIVE}
 
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64

@@ -268,15 +268,52 @@ __global__ __launch_bounds__(256) void k_adj_wave(real* __restrict__ pn, const r
     }
 }
 
-// CIC adjoint at x' and the transposed Verlet step.  mbar = alpha * CIC^T
-// sbar feeds the mass gradient; alpha * m * d(CIC^T sbar)/dx' joins the
-// position adjoint; then (step builds only)
+// The transposed stencil with per-cell coefficients (non-periodic grids,
+// where the absorbing layer varies them): the forward reads its neighbours
+// with its own K_MAIN, so the transpose gathers K_MAIN of each neighbour:
+//   pn[c] = B(sum_nbrs km[n] p[n] + ka[c] p[c] + ko[c] po[c]),  sbar (+)= p.
+// p is zero on the faces, so the clamped neighbour of a face cell is unused.
+template <class real, bool FIRST, int ZB>
+__global__ __launch_bounds__(256) void k_adj_wave_var(real* __restrict__ pn, const real* __restrict__ p,
+                                                      const real* __restrict__ po, real* __restrict__ sbar, int NX,
+                                                      const real* __restrict__ km, const real* __restrict__ ka,
+                                                      const real* __restrict__ ko) {
+    int bx, by, bz;
+    xcd_block(gridDim.x, gridDim.y, gridDim.z, bx, by, bz);
+    const int ix = bx * 64 + threadIdx.x;
+    const int iy = by * 4 + threadIdx.y;
+    if (ix >= NX || iy >= NX) return;
+    const size_t sy = size_t(NX), sz = size_t(NX) * NX;
+    const bool edge_xy = ix == 0 || ix == NX - 1 || iy == 0 || iy == NX - 1;
+    const size_t col = size_t(ix) + iy * sy;
+    for (int t = 0; t < ZB; ++t) {
+        const int iz = bz * ZB + t;
+        if (iz >= NX) break;
+        const size_t c = col + size_t(iz) * sz;
+        real out = 0;
+        if (!(edge_xy || iz == 0 || iz == NX - 1)) {
+            const size_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
+            real acc = 0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) acc += km[nb[q]] * p[nb[q]];
+            out = acc + ka[c] * p[c] + ko[c] * po[c];
+        }
+        pn[c] = out;
+        sbar[c] = FIRST ? p[c] : sbar[c] + p[c];
+    }
+}
+
+// CIC adjoint at x' and the transposed Verlet step.  mbar = CIC^T (alpha
+// sbar) feeds the mass gradient (alpha = d source / d cell mass, per cell on
+// non-periodic grids); m * d(CIC^T (alpha sbar))/dx' joins the position
+// adjoint; then (step builds only)
 //   vbar <- vbar + dt xbar,  abar <- dt/2 vbar_old + dt^2/2 xbar.
 template <class real, bool CLOSING>
 __global__ __launch_bounds__(256) void k_adj_cic(int64_t n, const real* __restrict__ x, const real* __restrict__ m,
                                                  Geo<real> g, const real* __restrict__ sbar, double alpha, double dt,
                                                  double* __restrict__ Xb, double* __restrict__ Vb,
-                                                 double* __restrict__ Ab, double* __restrict__ gm) {
+                                                 double* __restrict__ Ab, double* __restrict__ gm,
+                                                 const real* __restrict__ alpha_cell) {
     const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int NX = g.nx;
@@ -303,7 +340,7 @@ __global__ __launch_bounds__(256) void k_adj_cic(int64_t n, const real* __restri
             for (int i = 0; i <= 1; ++i) {
                 const size_t c = size_t(i ? i1[0] : i0[0]) + size_t(j ? i1[1] : i0[1]) * NX +
                                  size_t(k ? i1[2] : i0[2]) * NX * NX;
-                const double s = (double)sbar[c];
+                const double s = alpha_cell ? (double)alpha_cell[c] * (double)sbar[c] : alpha * (double)sbar[c];
                 const double wi = w[0][i], wj = w[1][j], wk = w[2][k];
                 mb += wi * wj * wk * s;
                 dm0 += (i ? dw[0] : -dw[0]) * wj * wk * s;
@@ -312,8 +349,8 @@ __global__ __launch_bounds__(256) void k_adj_cic(int64_t n, const real* __restri
             }
         }
     }
-    gm[q] += alpha * mb;
-    const double am = alpha * (double)m[q];
+    gm[q] += mb;
+    const double am = (double)m[q];
     const double dm[3] = {dm0, dm1, dm2};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -349,10 +386,17 @@ template <class real> void adj_gradient_t(hipStream_t s, int nx, real dx, bool p
 
 template <class real>
 void adj_wave(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* pn, const real* p, const real* po,
-              real* sbar, bool first) {
+              real* sbar, bool first, const real* km, const real* ka, const real* ko) {
     const int zb = adj_wave_zb(nx);
     const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned((nx + zb - 1) / zb));
     const dim3 block(64, 4, 1);
+    if (km) {
+        auto gv = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, sbar, nx, km, ka, ko); };
+        if (zb == 2) first ? gv(k_adj_wave_var<real, true, 2>) : gv(k_adj_wave_var<real, false, 2>);
+        else first ? gv(k_adj_wave_var<real, true, 4>) : gv(k_adj_wave_var<real, false, 4>);
+        RRG_LAUNCH_CHECK();
+        return;
+    }
     auto go = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, sbar, nx, k); };
     if (zb == 2) {
         if (periodic) first ? go(k_adj_wave<real, true, true, 2>) : go(k_adj_wave<real, true, false, 2>);
@@ -366,10 +410,13 @@ void adj_wave(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* 
 
 template <class real>
 void adj_cic(hipStream_t s, int64_t n, const real* x, const real* m, const Geo<real>& g, const real* sbar,
-             double alpha, double dt, bool closing, double* Xb, double* Vb, double* Ab, double* gm) {
+             double alpha, double dt, bool closing, double* Xb, double* Vb, double* Ab, double* gm,
+             const real* alpha_cell) {
     if (n == 0) return;
-    if (closing) k_adj_cic<real, true><<<grid1d(n, 256), 256, 0, s>>>(n, x, m, g, sbar, alpha, dt, Xb, Vb, Ab, gm);
-    else k_adj_cic<real, false><<<grid1d(n, 256), 256, 0, s>>>(n, x, m, g, sbar, alpha, dt, Xb, Vb, Ab, gm);
+    if (closing) k_adj_cic<real, true><<<grid1d(n, 256), 256, 0, s>>>(n, x, m, g, sbar, alpha, dt, Xb, Vb, Ab, gm,
+                                                                      alpha_cell);
+    else k_adj_cic<real, false><<<grid1d(n, 256), 256, 0, s>>>(n, x, m, g, sbar, alpha, dt, Xb, Vb, Ab, gm,
+                                                                alpha_cell);
     RRG_LAUNCH_CHECK();
 }
 
@@ -378,9 +425,9 @@ void adj_cic(hipStream_t s, int64_t n, const real* x, const real* m, const Geo<r
                                    double*, const double*, const double*, double*);                              \
     template void adj_gradient_t<real>(hipStream_t, int, real, bool, const double*, real*);                      \
     template void adj_wave<real>(hipStream_t, int, bool, const WaveK<real>&, real*, const real*, const real*,    \
-                                 real*, bool);                                                                   \
+                                 real*, bool, const real*, const real*, const real*);                            \
     template void adj_cic<real>(hipStream_t, int64_t, const real*, const real*, const Geo<real>&, const real*,  \
-                                double, double, bool, double*, double*, double*, double*);
+                                double, double, bool, double*, double*, double*, double*, const real*);
 
 RRG_PMADJ_INST(float)
 RRG_PMADJ_INST(double)

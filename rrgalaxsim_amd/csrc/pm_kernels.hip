@@ -18,6 +18,7 @@
 #include "rrg_internal.hpp"
 
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 
 namespace rrg {
@@ -199,6 +200,84 @@ __global__ __launch_bounds__(256) void k_permute(PState<real> src, PState<real> 
     id_dst[i] = id_src[j];
 }
 
+// ------------------------------------------------- non-periodic boundaries
+// Division and square root rounded once to `real`: FP32 goes through FP64,
+// which rounds to the same float as a correctly rounded FP32 operation
+// (53 >= 2*24 + 2 bits), so results match an IEEE host evaluation.
+template <class real> __device__ __forceinline__ real div_r(real a, real b) {
+    if constexpr (std::is_same_v<real, float>) return (float)((double)a / (double)b);
+    else return a / b;
+}
+template <class real> __device__ __forceinline__ real sqrt_r(real a) {
+    if constexpr (std::is_same_v<real, float>) return (float)sqrt((double)a);
+    else return sqrt(a);
+}
+
+template <class real> __device__ __forceinline__ real coord_of(const WaveModel<real>& w, int i) {  // wave.cu:18-20
+    return w.domain_min + (real)i * w.dx;
+}
+
+template <class real> __device__ __forceinline__ real layer_diss(const WaveModel<real>& w, real x) {  // wave.cu:27-33
+    const real d = fabs(x) - w.x_in;
+    return (d > 0) ? w.diss_extra * d * d : real(0);
+}
+
+// The scheme coefficients of one cell (wave.cu:155-171); non-periodic grids
+// add the layer dissipation of each axis.
+template <class real> __device__ __forceinline__ WaveK<real> cell_coef(const WaveModel<real>& w, int ix, int iy, int iz) {
+    real diss = w.diss_base;
+    diss += layer_diss(w, coord_of(w, ix));
+    diss += layer_diss(w, coord_of(w, iy));
+    diss += layer_diss(w, coord_of(w, iz));
+    const int DIM = 3;
+    const real C = w.C, DT = w.DT, DX = w.dx;
+    const real Q = diss * C * DT * real(0.5);
+    const real W = 1 + Q;
+    const real K = div_r<real>(C * C * DT * DT, DX * DX);
+    WaveK<real> k;
+    k.k_main = div_r<real>(K, W);
+    k.k_f = div_r<real>(-C * C * DT * DT, W);
+    k.k_actual = div_r<real>(2 * (1 - DIM * K), W);
+    k.k_old = div_r<real>(Q - 1, W);
+    return k;
+}
+
+template <class real> __device__ __forceinline__ real monopole_phi(const WaveModel<real>& w, real x, real y, real z) {
+    const real G = 1.;  // wave.cu:36-41
+    const real rx = x - w.com[0], ry = y - w.com[1], rz = z - w.com[2];
+    const real r = fmax(sqrt_r<real>(rx * rx + ry * ry + rz * rz), w.r_min);
+    return div_r<real>(-G * w.mass, r);
+}
+
+template <class real>
+__device__ __forceinline__ real psi_at(const WaveModel<real>& w, const real* __restrict__ phi, int i, int j, int k) {
+    const size_t NX = size_t(w.nx);  // wave.cu:44-49
+    return phi[size_t(i) + size_t(j) * NX + size_t(k) * NX * NX] -
+           monopole_phi(w, coord_of(w, i), coord_of(w, j), coord_of(w, k));
+}
+
+template <class real>
+__device__ real radiative_value(const WaveModel<real>& w, const real* __restrict__ phi, int ix, int iy, int iz) {
+    const real x = coord_of(w, ix), y = coord_of(w, iy), z = coord_of(w, iz);  // wave.cu:80-104
+    const real rx = x - w.com[0], ry = y - w.com[1], rz = z - w.com[2];
+    const real r = fmax(sqrt_r<real>(rx * rx + ry * ry + rz * rz), w.r_min);
+    const real n3[3] = {div_r<real>(rx, r), div_r<real>(ry, r), div_r<real>(rz, r)};
+    const real psi0 = psi_at(w, phi, ix, iy, iz);
+    real dpsi_dr = 0;
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {  // upwind_axis (wave.cu:53-76), summed in axis order
+        const int i0 = axis == 0 ? ix : (axis == 1 ? iy : iz);
+        int in = (n3[axis] > 0) ? i0 - 1 : i0 + 1;
+        if (in < 0 || in >= w.nx) in = 2 * i0 - in;
+        const int jx = axis == 0 ? in : ix, jy = axis == 1 ? in : iy, jz = axis == 2 ? in : iz;
+        const real d = div_r<real>(psi0 - psi_at(w, phi, jx, jy, jz), (real)(i0 - in) * w.dx);
+        const real term = n3[axis] * d;
+        dpsi_dr = axis == 0 ? term : dpsi_dr + term;
+    }
+    const real psi_new = psi0 - w.C * w.DT * (dpsi_dr + div_r<real>(psi0, r));
+    return psi_new + monopole_phi(w, x, y, z);
+}
+
 // Fixed point -> cell mass and source term.  rho = m / (DX*DX*DX) - rho_mean
 // and f = 4*PI*G*rho in double then rounded to real (wave.cu:150-153); the
 // stencil's last term f*K_F is formed here once per deposit.
@@ -210,25 +289,37 @@ __device__ __forceinline__ real source_term(real m, real dx3, real rho_mean, int
     return f * k_f;
 }
 
-template <class real>
+template <class real, bool LAYER>
 __global__ __launch_bounds__(256) void k_fixed_to_source(int64_t ncell, unsigned long long* __restrict__ mfix,
                                                          double inv_scale, real dx3, real rho_mean, int periodic,
-                                                         real k_f, real* __restrict__ mass, real* __restrict__ src) {
+                                                         real k_f, real* __restrict__ mass, real* __restrict__ src,
+                                                         WaveModel<real> w) {
     const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const long long q = (long long)mfix[c];
     mfix[c] = 0ull;
     const real m = (real)((double)q * inv_scale);
     mass[c] = m;
-    src[c] = source_term<real>(m, dx3, rho_mean, periodic, k_f);
+    real kf = k_f;
+    if (LAYER) {
+        const int NX = w.nx;
+        kf = cell_coef(w, int(c % NX), int((c / NX) % NX), int(c / (int64_t(NX) * NX))).k_f;
+    }
+    src[c] = source_term<real>(m, dx3, rho_mean, periodic, kf);
 }
 
-template <class real>
+template <class real, bool LAYER>
 __global__ __launch_bounds__(256) void k_mass_to_source(int64_t ncell, const real* __restrict__ mass, real dx3,
-                                                        real rho_mean, int periodic, real k_f, real* __restrict__ src) {
+                                                        real rho_mean, int periodic, real k_f, real* __restrict__ src,
+                                                        WaveModel<real> w) {
     const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    src[c] = source_term<real>(mass[c], dx3, rho_mean, periodic, k_f);
+    real kf = k_f;
+    if (LAYER) {
+        const int NX = w.nx;
+        kf = cell_coef(w, int(c % NX), int((c / NX) % NX), int(c / (int64_t(NX) * NX))).k_f;
+    }
+    src[c] = source_term<real>(mass[c], dx3, rho_mean, periodic, kf);
 }
 
 // -------------------------------------------------------------- wave stencil
@@ -337,6 +428,105 @@ __global__ __launch_bounds__(256) void k_max_diff(int64_t ncell, const real* __r
     if ((threadIdx.x & 63) == 0) {
         atomicMax(maxbits + 0, to_bits<real>(dmax));
         atomicMax(maxbits + 1, to_bits<real>(amax));
+    }
+}
+
+// wave_diss_iteration on a non-periodic grid: per-cell coefficients, face
+// values by boundary type BT (RRG_BC_ZERO_DIRICHLET / _MONOPOLE_*).  The
+// source term src already carries the cell's own K_F.
+template <class real, int BT, bool CHECK, int ZB>
+__global__ __launch_bounds__(256) void k_wave_bc(real* __restrict__ pn, const real* __restrict__ p,
+                                                 const real* __restrict__ po, const real* __restrict__ src,
+                                                 WaveModel<real> w, typename MaxBits<real>::T* __restrict__ maxbits) {
+    int bx, by, bz;
+    xcd_block(gridDim.x, gridDim.y, gridDim.z, bx, by, bz);
+    const int NX = w.nx;
+    const int ix = bx * 64 + threadIdx.x;
+    const int iy = by * 4 + threadIdx.y;
+    real dmax = 0, amax = 0;
+    if (ix < NX && iy < NX) {
+        const size_t sy = size_t(NX), sz = size_t(NX) * NX;
+        const bool edge_xy = ix == 0 || ix == NX - 1 || iy == 0 || iy == NX - 1;
+        const size_t col = size_t(ix) + iy * sy;
+        for (int t = 0; t < ZB; ++t) {
+            const int iz = bz * ZB + t;
+            if (iz >= NX) break;
+            const size_t c = col + size_t(iz) * sz;
+            const real cen = p[c];
+            real out;
+            if (edge_xy || iz == 0 || iz == NX - 1) {  // wave.cu:129-147
+                if constexpr (BT == RRG_BC_MONOPOLE_DIRICHLET) out = monopole_phi(w, coord_of(w, ix), coord_of(w, iy), coord_of(w, iz));
+                else if constexpr (BT == RRG_BC_MONOPOLE_RADIATIVE) out = radiative_value(w, p, ix, iy, iz);
+                else out = 0;
+            } else {
+                const WaveK<real> k = cell_coef(w, ix, iy, iz);
+                const real lap_sum = p[c - 1] + p[c + 1] + p[c - sy] + p[c + sy] + p[c - sz] + p[c + sz];
+                out = lap_sum * k.k_main + cen * k.k_actual + po[c] * k.k_old + src[c];
+            }
+            pn[c] = out;
+            if (CHECK) {
+                dmax = fmax(dmax, fabs(out - cen));
+                amax = fmax(amax, fabs(out));
+            }
+        }
+    }
+    if (CHECK) {
+        dmax = wave_max(dmax);
+        amax = wave_max(amax);
+        if (((threadIdx.y * 64 + threadIdx.x) & 63) == 0) {
+            atomicMax(maxbits + 0, to_bits<real>(dmax));
+            atomicMax(maxbits + 1, to_bits<real>(amax));
+        }
+    }
+}
+
+// Per-cell coefficient grids of a non-periodic grid (transposed stencil).
+template <class real>
+__global__ __launch_bounds__(256) void k_cell_coefs(WaveModel<real> w, real* __restrict__ km, real* __restrict__ ka,
+                                                    real* __restrict__ ko, real* __restrict__ alpha) {
+    const int NX = w.nx;
+    const int64_t ncell = int64_t(NX) * NX * NX;
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int ix = int(c % NX), iy = int((c / NX) % NX), iz = int(c / (int64_t(NX) * NX));
+    const WaveK<real> k = cell_coef(w, ix, iy, iz);
+    km[c] = k.k_main;
+    ka[c] = k.k_actual;
+    ko[c] = k.k_old;
+    alpha[c] = (real)(4 * kPi * (double)k.k_f / ((double)w.dx * (double)w.dx * (double)w.dx));
+}
+
+// Centre of mass in fixed point: pass 1 max|m x| (non-negative doubles
+// order like their bit patterns), pass 2 the int64 moment and mass sums.
+template <class real>
+__global__ __launch_bounds__(256) void k_com_max(PState<real> p, unsigned long long* __restrict__ maxbits) {
+    const int64_t n = p.n;
+    double mx = 0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        for (int d = 0; d < 3; ++d) mx = fmax(mx, fabs((double)p.m[i] * (double)p.x[d * n + i]));
+    mx = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(mx));
+}
+
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_com_sum(PState<real> p, double scale_x, double scale_m,
+                                                 unsigned long long* __restrict__ sums) {
+    const int64_t n = p.n;
+    long long q[4] = {0, 0, 0, 0};
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const double m = (double)p.m[i];
+        for (int d = 0; d < 3; ++d) q[d] += (long long)rint(m * (double)p.x[d * n + i] * scale_x);
+        q[3] += (long long)rint(m * scale_m);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long v = wave_sum_i64(q[k]);
+        if ((threadIdx.x & 63) == 0) atomicAdd(sums + k, (unsigned long long)v);
     }
 }
 
@@ -501,40 +691,113 @@ void permute(hipStream_t s, const PState<real>& src, const PState<real>& dst, co
 }
 
 template <class real>
+WaveModel<real> make_model(int nx, double dx_in, double bc_frac_in, double diss_base, double diss_extra,
+                           double wave_speed, double wave_dt, int bc) {
+    // GridInfo from params (io_json.cpp:28-40), in `real`
+    WaveModel<real> w{};
+    w.nx = nx;
+    w.bc = bc;
+    w.dx = (real)dx_in;
+    const real domain_l = w.dx * nx;
+    const real bc_frac = (real)bc_frac_in;
+    const real sim_l = (real)((1. - bc_frac) * domain_l);
+    w.domain_min = (real)(-0.5 * domain_l);
+    w.x_in = real(0.5) * sim_l;
+    w.diss_base = (real)diss_base;
+    w.diss_extra = (real)diss_extra;
+    w.C = (real)wave_speed;
+    w.DT = (real)wave_dt;
+    w.mass = 0;
+    w.r_min = real(0.5) * w.dx;
+    w.com[0] = w.com[1] = w.com[2] = 0;
+    return w;
+}
+
+template <class real>
+void center_of_mass(hipStream_t s, const PState<real>& p, int e_mass, unsigned long long* scratch, real out[3]) {
+    out[0] = out[1] = out[2] = 0;
+    if (p.n == 0) return;
+    const unsigned nb = unsigned(std::min<int64_t>(1024, (p.n + 255) / 256));
+    k_fill_u64<<<1, 64, 0, s>>>(scratch, 5, 0ull);
+    k_com_max<real><<<nb, 256, 0, s>>>(p, scratch);
+    RRG_LAUNCH_CHECK();
+    unsigned long long h[5];
+    RRG_HIP(hipMemcpyAsync(h, scratch, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    RRG_HIP(hipStreamSynchronize(s));
+    double mx;
+    std::memcpy(&mx, &h[0], sizeof(mx));
+    int ex = 0;
+    std::frexp(std::max(mx * (double)p.n, 1e-300), &ex);
+    const int E = std::min(61 - ex, 1000);
+    k_com_sum<real><<<nb, 256, 0, s>>>(p, std::ldexp(1.0, E), std::ldexp(1.0, e_mass), scratch + 1);
+    RRG_LAUNCH_CHECK();
+    RRG_HIP(hipMemcpyAsync(h + 1, scratch + 1, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    RRG_HIP(hipStreamSynchronize(s));
+    const double M = (double)(long long)h[4] * std::ldexp(1.0, -e_mass);
+    if (M > 0)
+        for (int d = 0; d < 3; ++d) out[d] = (real)((double)(long long)h[1 + d] * std::ldexp(1.0, -E) / M);
+}
+
+template <class real>
 void fixed_to_source(hipStream_t s, int nx, unsigned long long* mfix, double inv_scale, real dx, real rho_mean,
-                     bool periodic, real k_f, real* mass, real* src) {
+                     bool periodic, real k_f, real* mass, real* src, const WaveModel<real>* model) {
     const int64_t ncell = int64_t(nx) * nx * nx;
     const real dx3 = dx * dx * dx;
-    k_fixed_to_source<real><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mfix, inv_scale, dx3, rho_mean,
-                                                              periodic ? 1 : 0, k_f, mass, src);
+    const WaveModel<real> w = model ? *model : WaveModel<real>{};
+    if (model) k_fixed_to_source<real, true><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mfix, inv_scale, dx3, rho_mean,
+                                                                               periodic ? 1 : 0, k_f, mass, src, w);
+    else k_fixed_to_source<real, false><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mfix, inv_scale, dx3, rho_mean,
+                                                                         periodic ? 1 : 0, k_f, mass, src, w);
     RRG_LAUNCH_CHECK();
 }
 
 template <class real>
 void mass_to_source(hipStream_t s, int nx, const real* mass, real dx, real rho_mean, bool periodic, real k_f,
-                    real* src) {
+                    real* src, const WaveModel<real>* model) {
     const int64_t ncell = int64_t(nx) * nx * nx;
     const real dx3 = dx * dx * dx;
-    k_mass_to_source<real><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mass, dx3, rho_mean, periodic ? 1 : 0, k_f, src);
+    const WaveModel<real> w = model ? *model : WaveModel<real>{};
+    if (model) k_mass_to_source<real, true><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mass, dx3, rho_mean,
+                                                                             periodic ? 1 : 0, k_f, src, w);
+    else k_mass_to_source<real, false><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mass, dx3, rho_mean,
+                                                                        periodic ? 1 : 0, k_f, src, w);
     RRG_LAUNCH_CHECK();
 }
 
 template <class real>
-void wave_iteration(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* pn, const real* p,
+void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, real* pn, const real* p,
                     const real* po, const real* src, void* maxbits) {
     using MB = typename MaxBits<real>::T;
     MB* mb = reinterpret_cast<MB*>(maxbits);
+    const int nx = m.nx;
     const int zb = wave_zb(nx);
     const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned((nx + zb - 1) / zb));
     const dim3 block(64, 4, 1);
-    auto go = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb); };
-    if (zb == 2) {
-        if (periodic) mb ? go(k_wave<real, true, true, 2>) : go(k_wave<real, true, false, 2>);
-        else mb ? go(k_wave<real, false, true, 2>) : go(k_wave<real, false, false, 2>);
+    if (m.bc == RRG_BC_PERIODIC) {
+        auto go = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb); };
+        if (zb == 2) mb ? go(k_wave<real, true, true, 2>) : go(k_wave<real, true, false, 2>);
+        else mb ? go(k_wave<real, true, true, 4>) : go(k_wave<real, true, false, 4>);
     } else {
-        if (periodic) mb ? go(k_wave<real, true, true, 4>) : go(k_wave<real, true, false, 4>);
-        else mb ? go(k_wave<real, false, true, 4>) : go(k_wave<real, false, false, 4>);
+        auto go = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, src, m, mb); };
+        // face-heavy small grids use the same z depth rule
+        if (m.bc == RRG_BC_MONOPOLE_DIRICHLET) {
+            if (zb == 2) mb ? go(k_wave_bc<real, RRG_BC_MONOPOLE_DIRICHLET, true, 2>) : go(k_wave_bc<real, RRG_BC_MONOPOLE_DIRICHLET, false, 2>);
+            else mb ? go(k_wave_bc<real, RRG_BC_MONOPOLE_DIRICHLET, true, 4>) : go(k_wave_bc<real, RRG_BC_MONOPOLE_DIRICHLET, false, 4>);
+        } else if (m.bc == RRG_BC_MONOPOLE_RADIATIVE) {
+            if (zb == 2) mb ? go(k_wave_bc<real, RRG_BC_MONOPOLE_RADIATIVE, true, 2>) : go(k_wave_bc<real, RRG_BC_MONOPOLE_RADIATIVE, false, 2>);
+            else mb ? go(k_wave_bc<real, RRG_BC_MONOPOLE_RADIATIVE, true, 4>) : go(k_wave_bc<real, RRG_BC_MONOPOLE_RADIATIVE, false, 4>);
+        } else {
+            if (zb == 2) mb ? go(k_wave_bc<real, RRG_BC_ZERO_DIRICHLET, true, 2>) : go(k_wave_bc<real, RRG_BC_ZERO_DIRICHLET, false, 2>);
+            else mb ? go(k_wave_bc<real, RRG_BC_ZERO_DIRICHLET, true, 4>) : go(k_wave_bc<real, RRG_BC_ZERO_DIRICHLET, false, 4>);
+        }
     }
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real>
+void cell_coefficients(hipStream_t s, const WaveModel<real>& m, real* km, real* ka, real* ko, real* alpha) {
+    const int64_t ncell = int64_t(m.nx) * m.nx * m.nx;
+    k_cell_coefs<real><<<grid1d(ncell, 256), 256, 0, s>>>(m, km, ka, ko, alpha);
     RRG_LAUNCH_CHECK();
 }
 
@@ -580,13 +843,17 @@ void fill_u64(hipStream_t s, unsigned long long* p, size_t count, unsigned long 
     template void predict_deposit<real>(hipStream_t, const PState<real>&, real, bool, const Geo<real>&, double, \
                                         unsigned long long*);                                                   \
     template void fixed_to_source<real>(hipStream_t, int, unsigned long long*, double, real, real, bool, real,  \
-                                        real*, real*);                                                          \
+                                        real*, real*, const WaveModel<real>*);                                  \
+    template WaveModel<real> make_model<real>(int, double, double, double, double, double, double, int);        \
+    template void center_of_mass<real>(hipStream_t, const PState<real>&, int, unsigned long long*, real*);       \
+    template void cell_coefficients<real>(hipStream_t, const WaveModel<real>&, real*, real*, real*, real*);      \
     template void cell_keys<real>(hipStream_t, const PState<real>&, const Geo<real>&, unsigned*, unsigned*);     \
     template void permute<real>(hipStream_t, const PState<real>&, const PState<real>&, const unsigned*,          \
                                 const int*, int*);                                                              \
-    template void mass_to_source<real>(hipStream_t, int, const real*, real, real, bool, real, real*);           \
-    template void wave_iteration<real>(hipStream_t, int, bool, const WaveK<real>&, real*, const real*,          \
-                                       const real*, const real*, void*);                                        \
+    template void mass_to_source<real>(hipStream_t, int, const real*, real, real, bool, real, real*,            \
+                                       const WaveModel<real>*);                                                 \
+    template void wave_iteration<real>(hipStream_t, const WaveModel<real>&, const WaveK<real>&, real*,          \
+                                       const real*, const real*, const real*, void*);                           \
     template void gradient<real>(hipStream_t, int, real, const real*, real*);                                   \
     template void gather<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*, real, bool);    \
     template void ngp_phi<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*);               \

@@ -759,7 +759,9 @@ public:
         require(g.dx > 0, "rrg_create: grid dx must be positive");
         require(p.wave.iterations >= 0 && p.wave.setup_iterations >= 0, "rrg_create: negative wave iterations");
         require(p.wave.steady_check_step > 0, "rrg_create: steady_check_step must be positive");
-        require(p.boundary == RRG_BC_PERIODIC || p.boundary == RRG_BC_ZERO_DIRICHLET, "rrg_create: unknown boundary");
+        require(p.boundary >= RRG_BC_ZERO_DIRICHLET && p.boundary <= RRG_BC_MONOPOLE_RADIATIVE,
+                "rrg_create: unknown boundary");
+        require(g.bc_frac >= 0 && g.bc_frac < 1, "rrg_create: grid bc_frac must be in [0, 1)");
         nx_ = g.nx;
         ncell_ = int64_t(nx_) * nx_ * nx_;
         periodic_ = p.boundary == RRG_BC_PERIODIC;
@@ -768,6 +770,8 @@ public:
         const real C = (real)p.wave.wave_speed;
         const real DT = (real)p.wave.dt;
         k_ = pm::wave_coef<real>((real)p.wave.diss_base, C, DT, dxr);
+        model_ = pm::make_model<real>(nx_, g.dx, g.bc_frac, p.wave.diss_base, p.wave.diss_extra, p.wave.wave_speed,
+                                      p.wave.dt, p.boundary);
         // set_boundary_periodic(1., ...) (rrgsim_integration_rect.cuh:65-69; wave.cu:214-228)
         {
             const real L = nx_ * dxr;
@@ -830,6 +834,12 @@ public:
     void precalc() override {  // rrgsim_integration_rect.cuh:53-83
         sort_particles();
         deposit(false);
+        if (model_.bc == RRG_BC_MONOPOLE_DIRICHLET || model_.bc == RRG_BC_MONOPOLE_RADIATIVE) {
+            // set_boundary_monopole(1., calc_mass_center(...), ...) (rrgsim_integration_rect.cuh:57-63)
+            if (com_scratch_.n == 0) com_scratch_.alloc(5);
+            pm::center_of_mass<real>(s_, P_.view(), e_, com_scratch_.p, model_.com);
+            model_.mass = (real)1.;
+        }
         last_iters_ = solve(prm_.wave.setup_iterations);
         pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
         grav_actual_ = false;
@@ -873,6 +883,8 @@ public:
         require(every >= 1, "rrg_adjoint_begin: checkpoint_every must be >= 1");
         if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_adjoint_begin before rrg_precalc");
         if (!fresh_) fail(RRG_ERR_STATE, "rrg_adjoint_begin: a PM context records from the state right after rrg_precalc");
+        require(periodic_ || model_.bc == RRG_BC_ZERO_DIRICHLET,
+                "rrg_adjoint_begin: the PM adjoint covers periodic and zero-Dirichlet boundaries");
         sort_particles();
         sync();
         ck_every_ = every;
@@ -1028,7 +1040,7 @@ private:
         });
         segment(RRG_STAGE_SOURCE, 1, graph, b0 + 1, [&] {
             pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, mass_.p,
-                                      src_.p);
+                                      src_.p, layered());
         });
         const int iters = prm_.wave.iterations;
         if (graph) {
@@ -1057,7 +1069,7 @@ private:
     void wave_launches(int iters, int& ip, int& ic) {
         for (int it = 0; it < iters; ++it) {
             const int inx = 3 - ic - ip;
-            pm::wave_iteration<real>(s_, nx_, periodic_, k_, phi_[inx].p, phi_[ic].p, phi_[ip].p, src_.p, nullptr);
+            pm::wave_iteration<real>(s_, model_, k_, phi_[inx].p, phi_[ic].p, phi_[ip].p, src_.p, nullptr);
             ip = ic;
             ic = inx;
         }
@@ -1073,6 +1085,9 @@ private:
     }
 
     real* cur() { return phi_[ic_].p; }
+
+    // the per-cell source coefficient applies off-periodic (absorbing layer)
+    const pm::WaveModel<real>* layered() const { return periodic_ ? nullptr : &model_; }
 
     std::vector<int> ids() {
         std::vector<int> id(P_.n);
@@ -1112,6 +1127,10 @@ private:
             sbar_.alloc(ncell_);
             gtmp_.alloc(4 * ncell_);
             zero_async(s_, Gb_.p, Gb_.bytes());
+            if (!periodic_) {
+                for (auto& b : kc_) b.alloc(ncell_);
+                pm::cell_coefficients<real>(s_, model_, kc_[0].p, kc_[1].p, kc_[2].p, kc_[3].p);
+            }
         }
         const size_t K1 = size_t(ck_every_) + 1;
         if (xs_.n < K1 * 3 * n) xs_.alloc(K1 * 3 * n);
@@ -1136,7 +1155,8 @@ private:
             auto run = [&](int& ip, int& ic) {
                 for (int it = 0; it < iters; ++it) {
                     const int in = 3 - ip - ic;
-                    pm::adj_wave<real>(s_, nx_, periodic_, k_, mu_[in].p, mu_[ic].p, mu_[ip].p, sbar_.p, it == 0);
+                    pm::adj_wave<real>(s_, nx_, periodic_, k_, mu_[in].p, mu_[ic].p, mu_[ip].p, sbar_.p, it == 0,
+                                       periodic_ ? nullptr : kc_[0].p, kc_[1].p, kc_[2].p);
                     ip = ic;
                     ic = in;
                 }
@@ -1161,7 +1181,8 @@ private:
         const real dx3 = geo_.dx * geo_.dx * geo_.dx;
         const double alpha = 4 * 3.14159265358979 * (double)k_.k_f / (double)dx3;  // d source / d cell mass
         t = timer.start(s_, RRG_STAGE_ADJ_PARTICLE);
-        pm::adj_cic<real>(s_, n, x, P_.M.p, geo_, sbar_.p, alpha, dt, closing, Xb_.p, Vb_.p, Ab_.p, gm_.p);
+        pm::adj_cic<real>(s_, n, x, P_.M.p, geo_, sbar_.p, alpha, dt, closing, Xb_.p, Vb_.p, Ab_.p, gm_.p,
+                          periodic_ ? nullptr : kc_[3].p);
         timer.stop(s_, t, 1);
     }
 
@@ -1184,7 +1205,8 @@ private:
         pm::predict_deposit<real>(s_, P_.view(), (real)prm_.sim.dt_dynamics, predict, geo_, scale_, mfix_.p);
         timer.stop(s_, t, 1);
         t = timer.start(s_, RRG_STAGE_SOURCE);
-        pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, mass_.p, src_.p);
+        pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, mass_.p, src_.p,
+                                  layered());
         timer.stop(s_, t, 1);
     }
 
@@ -1202,7 +1224,7 @@ private:
             const bool check = checking && iter && (iter % check_step == 0);
             if (check) pm::fill_u64(s_, reinterpret_cast<unsigned long long*>(maxbits_.p), 2, 0ull);
             const int inx = 3 - ic_ - ip_;
-            pm::wave_iteration<real>(s_, nx_, periodic_, k_, phi_[inx].p, phi_[ic_].p, phi_[ip_].p, src_.p,
+            pm::wave_iteration<real>(s_, model_, k_, phi_[inx].p, phi_[ic_].p, phi_[ip_].p, src_.p,
                                      check ? maxbits_.p : nullptr);
             ip_ = ic_;
             ic_ = inx;
@@ -1236,6 +1258,8 @@ private:
     pm::Geo<real> geo_{};
     pm::WaveK<real> k_{};
     real rho_mean_ = 0, tol_ = 0;
+    pm::WaveModel<real> model_{};
+    DBuf<unsigned long long> com_scratch_;
     int e_ = 61;
     double scale_ = 0, inv_scale_ = 0;
     Particles<real> P_, P2_;
@@ -1267,6 +1291,7 @@ private:
     DBuf<real> backup_, xs_, phis_;     // live state; x' and phi_I of one segment
     DBuf<double> Xb_, Vb_, Ab_, gm_, Gb_;
     DBuf<real> mu_[3], sbar_, gtmp_;
+    DBuf<real> kc_[4];                  // off-periodic K_MAIN, K_ACTUAL, K_OLD, d source / d mass per cell
     int aip_ = 0, aic_ = 1;             // potential adjoints mu_{k+1}, mu_k
 };
 
@@ -1602,11 +1627,12 @@ void wave_op(int32_t nx, double dx, double diss, double c, double wdt, int32_t b
     h2d(os.s, m.p, mass, sizeof(real) * ncell);
     const bool periodic = boundary == RRG_BC_PERIODIC;
     const pm::WaveK<real> k = pm::wave_coef<real>((real)diss, (real)c, (real)wdt, (real)dx);
-    pm::mass_to_source<real>(os.s, nx, m.p, (real)dx, (real)rho_mean, periodic, k.k_f, src.p);
+    const pm::WaveModel<real> wm = pm::make_model<real>(nx, dx, 0.0, diss, 0.0, c, wdt, boundary);
+    pm::mass_to_source<real>(os.s, nx, m.p, (real)dx, (real)rho_mean, periodic, k.k_f, src.p, periodic ? nullptr : &wm);
     int ip = 0, ic = 1;
     for (int it = 0; it < iters; ++it) {
         const int in = 3 - ic - ip;
-        pm::wave_iteration<real>(os.s, nx, periodic, k, b[in].p, b[ic].p, b[ip].p, src.p, nullptr);
+        pm::wave_iteration<real>(os.s, wm, k, b[in].p, b[ic].p, b[ip].p, src.p, nullptr);
         ip = ic;
         ic = in;
     }

@@ -184,6 +184,28 @@ template <class real> struct WaveK {
 };
 template <class real> WaveK<real> wave_coef(real diss, real c, real dt, real dx);
 
+// What the stencil reads per cell on a non-periodic grid (wave.cu:18-193):
+// the absorbing layer (x_in = sim_l/2, diss_extra) makes the coefficients
+// vary per cell, and the face cells take the boundary type's value.
+template <class real> struct WaveModel {
+    int nx;
+    int bc;                 // RRG_BC_* (the reference's BoundaryType values)
+    real dx, domain_min;
+    real x_in;              // 0.5 * sim_l, sim_l = (1 - bc_frac) * nx * dx
+    real diss_base, diss_extra, C, DT;
+    real mass, r_min;       // monopole: total mass, guard radius 0.5 dx
+    real com[3];            // monopole centre
+};
+template <class real>
+WaveModel<real> make_model(int nx, double dx, double bc_frac, double diss_base, double diss_extra, double wave_speed,
+                           double wave_dt, int bc);
+
+// Centre of mass for the monopole boundaries, order-independent: every m*x
+// (FP64 product) and m rounded once to fixed point (2^E, 2^e) and summed in
+// int64; E keeps n * max|m x| below 2^61.  `scratch` holds 5 int64.
+template <class real>
+void center_of_mass(hipStream_t s, const PState<real>& p, int e_mass, unsigned long long* scratch, real out[3]);
+
 // predict_step_ fused with deposit_cell_mass_CIC_ at the new position into an
 // int64 fixed-point grid (scale = 2^e, exact).  When `predict` is false only
 // the deposit runs (precalc).
@@ -204,21 +226,30 @@ void radix_sort_pairs(hipStream_t s, void* temp, size_t& temp_bytes, const unsig
 
 // Fixed-point grid -> cell mass (real) and stencil source term
 // K_F * 4 pi G (m/dx^3 - rho_mean); clears the fixed-point grid for the next deposit.
+// On a non-periodic grid (`model` non-null) K_F is the cell's own.
 template <class real>
 void fixed_to_source(hipStream_t s, int nx, unsigned long long* mfix, double inv_scale, real dx,
-                     real rho_mean, bool periodic, real k_f, real* mass, real* src);
+                     real rho_mean, bool periodic, real k_f, real* mass, real* src,
+                     const WaveModel<real>* model = nullptr);
 
 // Cell mass (real) -> stencil source term, for callers that hold a mass grid.
 template <class real>
 void mass_to_source(hipStream_t s, int nx, const real* mass, real dx, real rho_mean, bool periodic, real k_f,
-                    real* src);
+                    real* src, const WaveModel<real>* model = nullptr);
 
-// One wave_diss_iteration over the grid.  When `maxbits` is non-null the
-// kernel also accumulates max|new - cur| and max|new| as bit patterns into
-// maxbits[0], maxbits[1] (which must hold zero beforehand).
+// One wave_diss_iteration over the grid: uniform coefficients `k` on a
+// periodic grid, per-cell coefficients and boundary values from `m`
+// otherwise.  When `maxbits` is non-null the kernel also accumulates
+// max|new - cur| and max|new| as bit patterns into maxbits[0], maxbits[1]
+// (which must hold zero beforehand).
 template <class real>
-void wave_iteration(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* pn,
-                    const real* p, const real* po, const real* src, void* maxbits);
+void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, real* pn, const real* p,
+                    const real* po, const real* src, void* maxbits);
+
+// Per-cell K_MAIN, K_ACTUAL, K_OLD and d(source)/d(cell mass) = 4 pi K_F / dx^3
+// of a non-periodic grid (for the transposed stencil).
+template <class real>
+void cell_coefficients(hipStream_t s, const WaveModel<real>& m, real* km, real* ka, real* ko, real* alpha);
 
 // calc_acceleration_field_ into a padded (x, y, z, 0) real4 grid.
 template <class real> void gradient(hipStream_t s, int nx, real dx, const real* phi, real* acc4);
@@ -246,15 +277,18 @@ void adj_interp(hipStream_t s, int64_t n, const real* x, const Geo<real>& g, con
 // mu += B(-1/dx D^T Gb) (B masks the faces of zero-Dirichlet grids).
 template <class real> void adj_gradient_t(hipStream_t s, int nx, real dx, bool periodic, const double* Gb, real* mu);
 // One transposed stencil iteration pn = B(K_m L p + K_a p + K_o po) with
-// sbar = p (first) or sbar += p.
+// sbar = p (first) or sbar += p.  With per-cell coefficient grids (km
+// non-null, non-periodic grids) pn = B(L(km p) + ka p + ko po).
 template <class real>
 void adj_wave(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* pn, const real* p, const real* po,
-              real* sbar, bool first);
-// CIC adjoint at x' (gm += alpha CIC^T sbar, Xb += alpha m dCIC^T sbar) and,
-// unless `closing`, the transposed Verlet update of (Xb, Vb, Ab).
+              real* sbar, bool first, const real* km = nullptr, const real* ka = nullptr, const real* ko = nullptr);
+// CIC adjoint at x' (gm += alpha CIC^T sbar, Xb += alpha m dCIC^T sbar; alpha
+// per cell when `alpha_cell` is non-null) and, unless `closing`, the
+// transposed Verlet update of (Xb, Vb, Ab).
 template <class real>
 void adj_cic(hipStream_t s, int64_t n, const real* x, const real* m, const Geo<real>& g, const real* sbar,
-             double alpha, double dt, bool closing, double* Xb, double* Vb, double* Ab, double* gm);
+             double alpha, double dt, bool closing, double* Xb, double* Vb, double* Ab, double* gm,
+             const real* alpha_cell = nullptr);
 
 }  // namespace pm
 

@@ -293,9 +293,13 @@ void params_parse_json(const std::string& path, rrg_params* p, rrg_ini_params* i
             p->force_kind = kinds.at(v->str);
         }
         if (const JValue* v = m->find("boundary")) {
-            if (v->kind != JValue::String || (v->str != "periodic" && v->str != "zero_dirichlet"))
-                fail(RRG_ERR_IO, "mi355x.boundary must be \"periodic\" or \"zero_dirichlet\"");
-            p->boundary = v->str == "periodic" ? RRG_BC_PERIODIC : RRG_BC_ZERO_DIRICHLET;
+            static const std::map<std::string, int32_t> types = {
+                {"periodic", RRG_BC_PERIODIC}, {"zero_dirichlet", RRG_BC_ZERO_DIRICHLET},
+                {"monopole_dirichlet", RRG_BC_MONOPOLE_DIRICHLET}, {"monopole_radiative", RRG_BC_MONOPOLE_RADIATIVE}};
+            if (v->kind != JValue::String || !types.count(v->str))
+                fail(RRG_ERR_IO, "mi355x.boundary must be \"periodic\", \"zero_dirichlet\", \"monopole_dirichlet\" "
+                                 "or \"monopole_radiative\"");
+            p->boundary = types.at(v->str);
         }
         opt_num(*m, "fixed_exponent", p->fixed_exponent);
         opt_num(*m, "device", p->device);

@@ -43,7 +43,7 @@ def make_params(*, dt, time_max=None, dt_save=None, grid=None, wave=None, real="
             setattr(p.wave, k, v)
     p.real_type = {"fp64": L.REAL_F64, "fp32": L.REAL_F32}[real]
     p.force_kind = L.FORCE_KINDS[force]
-    p.boundary = {"periodic": L.BC_PERIODIC, "zero_dirichlet": L.BC_ZERO_DIRICHLET}[boundary]
+    p.boundary = L.BOUNDARIES[boundary]
     p.fixed_exponent = fixed_exponent
     p.device = device
     p.use_graphs = 1 if graphs else 0
