@@ -239,8 +239,12 @@ int rrg_set_state(rrg_ctx* ctx, const double* pos, const double* vel);
 /* Energy, momentum and angular momentum by FP64 device reduction, with the
  * definitions of calc_conservation (nbody.cu:246-291). */
 int rrg_get_conservation(rrg_ctx* ctx, rrg_conservation* out);
-/* PM grids (any pointer may be NULL): cell mass (nx^3), phi (nx^3), -grad phi
- * (nx^3,3), x-fastest (co_device_utils.cuh:61). */
+/* Grids (any pointer may be NULL), x-fastest (co_device_utils.cuh:61): PM
+ * contexts give cell mass (nx^3), phi (nx^3) and -grad phi (nx^3,3);
+ * direct-sum contexts created with a grid section give the CIC cell mass and
+ * the NGP cell averages of the particle potential and acceleration, as the
+ * reference's direct-sum strategy projects them at sync
+ * (rrgsim_integration_nbody.cuh:32-48, rm_p2mesh.cu:139-168, 453-485). */
 int rrg_get_grid(rrg_ctx* ctx, double* mass, double* phi, double* acc);
 /* Per-system conservation sums of an ensemble (k entries; k = 1 for a
  * single system gives the same as rrg_get_conservation). */
@@ -293,6 +297,14 @@ int rrg_op_gradient(int32_t real_type, int32_t nx, double dx, const void* phi, v
 /* interpolate_acceleration_to_particles_kernel (rm_mesh2p.cu:48-114). */
 int rrg_op_gather(int32_t real_type, int64_t n, const void* pos, int32_t nx, double dx,
                   double bc_frac, const void* field, void* acc);
+/* NGP binning of particles (assign_particles_to_cells_ and the prefix sums,
+ * rm_p2mesh.cu:25-126) as a counting sort: cell_of[n] (AT index of the
+ * truncated (x - min)/dx, clamped), count[nx^3], start[nx^3] (the exclusive
+ * prefix of count, which the reference's prefix kernel intends but does not
+ * write), order[n] (particle indices sorted by cell, ascending within a
+ * cell).  Any output may be NULL. */
+int rrg_op_ngp_bin(int32_t real_type, int64_t n, const void* pos, int32_t nx, double dx, double bc_frac,
+                   int32_t* cell_of, int32_t* count, int32_t* start, int32_t* order);
 /* SteadyStateChecker::calc_max_diff (wave_steady_checker.cu:95-127). */
 int rrg_op_steady_err(int32_t real_type, int64_t ncell, const void* u_new, const void* u_old,
                       double* err);

@@ -239,6 +239,29 @@ def gradient(nx, dx, phi, dtype=np.float64):
     return out
 
 
+def ngp_bin(pos, nx, dx, bc_frac=0.0, dtype=np.float64):
+    """NGP counting sort (rm_p2mesh.cu:25-126, with the intended exclusive
+    prefix).  Returns (cell_of, count, start, order)."""
+    pos = _c(pos, dtype)
+    n = len(pos)
+    cell_of = np.empty(n, dtype=np.int32); order = np.empty(n, dtype=np.int32)
+    count = np.empty(nx ** 3, dtype=np.int32); start = np.empty(nx ** 3, dtype=np.int32)
+    getattr(lib(), f"orc_ngp_bin_{_suf(dtype)}")(_i64(n), _p(pos), _int(nx), _r(dx, dtype), _r(bc_frac, dtype),
+                                                _p(cell_of), _p(count), _p(start), _p(order))
+    return cell_of, count, start, order
+
+
+def ngp_project(nx, count, start, order, grav, acc, dtype=np.float64):
+    """Cell averages of particle potential and acceleration (rm_p2mesh.cu:139-168)."""
+    count = np.ascontiguousarray(count, np.int32); start = np.ascontiguousarray(start, np.int32)
+    order = np.ascontiguousarray(order, np.int32)
+    grav = _c(grav, dtype); acc = _c(acc, dtype)
+    phi = np.empty(nx ** 3, dtype=dtype); cacc = np.empty((nx ** 3, 3), dtype=dtype)
+    getattr(lib(), f"orc_ngp_project_{_suf(dtype)}")(_int(nx), _p(count), _p(start), _p(order), _p(grav), _p(acc),
+                                                    _p(phi), _p(cacc))
+    return phi, cacc
+
+
 def gather(pos, nx, dx, field, bc_frac=0.0, dtype=np.float64):
     """interpolate_acceleration_to_particles_kernel (rm_mesh2p.cu:48-114)."""
     pos = _c(pos, dtype); field = _c(field, dtype)

@@ -605,6 +605,63 @@ void ngp_phi(int64_t n, const real* pos, int nx, real dx, real bc_frac, const re
 }
 
 // CFL step of validate_dependent_fields (io_json.cpp:174-178).
+// NGP binning of the particles (assign_particles_to_cells_ and the prefix
+// sums, rm_p2mesh.cu:25-126) as a counting sort: each particle's cell (the
+// truncated (x - min)/dx, clamped), the per-cell counts, their exclusive
+// prefix sums and the particle order sorted by cell (ascending particle index
+// within a cell).  The reference's prefix kernel never writes its in-block
+// scan back, so its start_id holds only block offsets; this is the intended
+// exclusive prefix.
+template <class real>
+void ngp_bin(int64_t n, const real* pos, int nx, real dx, real bc_frac, int* cell_of, int* count, int* start,
+             int* order) {
+    const Grid<real> g = make_grid<real>(nx, dx, bc_frac);
+    const V3<real>* p = reinterpret_cast<const V3<real>*>(pos);
+    const size_t ncell = size_t(nx) * nx * nx;
+    std::fill(count, count + ncell, 0);
+    for (int64_t q = 0; q < n; ++q) {
+        const int ix = clampv<int>((int)((p[q].x - g.domain_min) / g.dx), 0, nx - 1);
+        const int iy = clampv<int>((int)((p[q].y - g.domain_min) / g.dx), 0, nx - 1);
+        const int iz = clampv<int>((int)((p[q].z - g.domain_min) / g.dx), 0, nx - 1);
+        cell_of[q] = int(AT(ix, iy, iz, nx));
+        ++count[cell_of[q]];
+    }
+    int run = 0;
+    for (size_t c = 0; c < ncell; ++c) {
+        start[c] = run;
+        run += count[c];
+    }
+    std::vector<int> fill(start, start + ncell);
+    for (int64_t q = 0; q < n; ++q) order[fill[cell_of[q]]++] = int(q);
+}
+
+// Cell averages of the particle potential and acceleration
+// (deposit_cell_grav_NGP_ and deposit_cell_acc_NGP_, rm_p2mesh.cu:139-168):
+// phi_c = sum_p phi_p / count_c and acc_c = sum_p a_p * (1. / count_c), the
+// sums taken in particle order (the reference's atomics leave it open).
+template <class real>
+void ngp_project(int nx, const int* count, const int* start, const int* order, const real* grav, const real* acc,
+                 real* cell_phi, real* cell_acc) {
+    const size_t ncell = size_t(nx) * nx * nx;
+    for (size_t c = 0; c < ncell; ++c) {
+        real ph = 0, ax = 0, ay = 0, az = 0;
+        if (count[c] > 0) {
+            const real inv_count = 1. / count[c];
+            for (int k = start[c]; k < start[c] + count[c]; ++k) {
+                const int q = order[k];
+                ph += grav[q] / count[c];
+                ax += acc[3 * q + 0] * inv_count;
+                ay += acc[3 * q + 1] * inv_count;
+                az += acc[3 * q + 2] * inv_count;
+            }
+        }
+        cell_phi[c] = ph;
+        cell_acc[3 * c + 0] = ax;
+        cell_acc[3 * c + 1] = ay;
+        cell_acc[3 * c + 2] = az;
+    }
+}
+
 template <class real> real wave_dt(real dx, real c) {
     constexpr real DIM = 3;
     const real dt = 0.5 * dx / (c * std::sqrt(DIM));
@@ -1112,6 +1169,13 @@ int pm_adjoint(const PmConfig& cf, int64_t n, const double* x0, const double* v0
     ORC_EXPORT void orc_ngp_phi_##SUF(int64_t n, const real* pos, int nx, real dx, real bcf,       \
                                       const real* phi, real* out) {                                \
         ngp_phi<real>(n, pos, nx, dx, bcf, phi, out); }                                            \
+    ORC_EXPORT void orc_ngp_bin_##SUF(int64_t n, const real* pos, int nx, real dx, real bcf,         \
+                                      int* cell_of, int* count, int* start, int* order) {          \
+        ngp_bin<real>(n, pos, nx, dx, bcf, cell_of, count, start, order); }                        \
+    ORC_EXPORT void orc_ngp_project_##SUF(int nx, const int* count, const int* start,               \
+                                          const int* order, const real* grav, const real* acc,     \
+                                          real* cell_phi, real* cell_acc) {                        \
+        ngp_project<real>(nx, count, start, order, grav, acc, cell_phi, cell_acc); }               \
     ORC_EXPORT real orc_wave_dt_##SUF(real dx, real c) { return wave_dt<real>(dx, c); }            \
     ORC_EXPORT real orc_rho_mean_##SUF(real total_mass, int nx, real dx) {                         \
         return rho_mean_periodic<real>(total_mass, nx, dx); }                                      \

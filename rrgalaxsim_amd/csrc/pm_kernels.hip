@@ -631,6 +631,64 @@ __global__ __launch_bounds__(256) void k_ngp_phi(PState<real> p, Geo<real> g, co
     p.g[q] = phi[size_t(ix) + size_t(iy) * NX + size_t(iz) * NX * NX];
 }
 
+// NGP binning (assign_particles_to_cells_, rm_p2mesh.cu:25-66): the cell of
+// each particle, and the per-cell counts (exact integer atomics).
+template <class real>
+__global__ __launch_bounds__(256) void k_ngp_keys(PState<real> p, Geo<real> g, unsigned* __restrict__ keys,
+                                                  unsigned* __restrict__ idx, int* __restrict__ count) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= p.n) return;
+    const int64_t n = p.n;
+    const int NX = g.nx;
+    const int ix = clampr<int>((int)((p.x[q] - g.domain_min) / g.dx), 0, NX - 1);
+    const int iy = clampr<int>((int)((p.x[n + q] - g.domain_min) / g.dx), 0, NX - 1);
+    const int iz = clampr<int>((int)((p.x[2 * n + q] - g.domain_min) / g.dx), 0, NX - 1);
+    const unsigned cell = unsigned(ix) + unsigned(NX) * (unsigned(iy) + unsigned(NX) * unsigned(iz));
+    keys[q] = cell;
+    idx[q] = unsigned(q);
+    atomicAdd(count + cell, 1);
+}
+
+// Cell averages of the particle potential and acceleration
+// (deposit_cell_grav_NGP_ / deposit_cell_acc_NGP_, rm_p2mesh.cu:139-168):
+// one thread per cell sums its particles in index order (the stable sort
+// keeps that order), so the result is deterministic.
+template <class real>
+__global__ __launch_bounds__(256) void k_ngp_project(int64_t ncell, const int* __restrict__ count,
+                                                     const int* __restrict__ start, const unsigned* __restrict__ order,
+                                                     int64_t n, const real* __restrict__ grav,
+                                                     const real* __restrict__ acc, real* __restrict__ cell_phi,
+                                                     real* __restrict__ cell_acc) {
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int cnt = count[c];
+    real ph = 0, ax = 0, ay = 0, az = 0;
+    if (cnt > 0) {
+        const real inv_count = (real)(1. / (double)cnt);
+        for (int k = start[c]; k < start[c] + cnt; ++k) {
+            const int64_t q = order[k];
+            ph += grav[q] / (real)cnt;
+            ax += acc[q] * inv_count;
+            ay += acc[n + q] * inv_count;
+            az += acc[2 * n + q] * inv_count;
+        }
+    }
+    cell_phi[c] = ph;
+    cell_acc[3 * c + 0] = ax;
+    cell_acc[3 * c + 1] = ay;
+    cell_acc[3 * c + 2] = az;
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_fixed_to_mass(int64_t ncell, unsigned long long* __restrict__ mfix,
+                                                       double inv_scale, real* __restrict__ mass) {
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const long long q = (long long)mfix[c];
+    mfix[c] = 0ull;
+    mass[c] = (real)((double)q * inv_scale);
+}
+
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t count, unsigned long long value) {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
         p[i] = value;
@@ -837,6 +895,48 @@ void fill_u64(hipStream_t s, unsigned long long* p, size_t count, unsigned long 
     RRG_LAUNCH_CHECK();
 }
 
+template <class real>
+void ngp_bin(hipStream_t s, const PState<real>& p, const Geo<real>& g, NgpBins& b) {
+    const int64_t ncell = int64_t(g.nx) * g.nx * g.nx;
+    RRG_HIP(hipMemsetAsync(b.count, 0, sizeof(int) * ncell, s));
+    if (p.n > 0) {
+        k_ngp_keys<real><<<grid1d(p.n, 256), 256, 0, s>>>(p, g, b.keys, b.idx, b.count);
+        RRG_LAUNCH_CHECK();
+        int end_bit = 1;
+        while ((int64_t(1) << end_bit) < ncell) ++end_bit;
+        size_t bytes = b.temp_bytes;
+        radix_sort_pairs(s, b.temp, bytes, b.keys, b.keys2, b.idx, b.order, p.n, end_bit);
+    }
+    size_t bytes = b.temp_bytes;
+    exclusive_scan_i32(s, b.temp, bytes, b.count, b.start, ncell);
+}
+
+template <class real>
+void ngp_project(hipStream_t s, int nx, const NgpBins& b, int64_t n, const real* grav, const real* acc, real* cell_phi,
+                 real* cell_acc) {
+    const int64_t ncell = int64_t(nx) * nx * nx;
+    k_ngp_project<real><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, b.count, b.start, b.order, n, grav, acc, cell_phi,
+                                                          cell_acc);
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real>
+void fixed_to_mass(hipStream_t s, int nx, unsigned long long* mfix, double inv_scale, real* mass) {
+    const int64_t ncell = int64_t(nx) * nx * nx;
+    k_fixed_to_mass<real><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mfix, inv_scale, mass);
+    RRG_LAUNCH_CHECK();
+}
+
+size_t ngp_temp_bytes(int64_t n, int nx) {
+    const int64_t ncell = int64_t(nx) * nx * nx;
+    int end_bit = 1;
+    while ((int64_t(1) << end_bit) < ncell) ++end_bit;
+    size_t a = 0, b = 0;
+    radix_sort_pairs(nullptr, nullptr, a, nullptr, nullptr, nullptr, nullptr, std::max<int64_t>(n, 1), end_bit);
+    exclusive_scan_i32(nullptr, nullptr, b, nullptr, nullptr, ncell);
+    return std::max<size_t>(std::max(a, b), 1);
+}
+
 #define RRG_PM_INST(real)                                                                                       \
     template Geo<real> make_geo<real>(int, double, double);                                                     \
     template WaveK<real> wave_coef<real>(real, real, real, real);                                               \
@@ -857,7 +957,11 @@ void fill_u64(hipStream_t s, unsigned long long* p, size_t count, unsigned long 
     template void gradient<real>(hipStream_t, int, real, const real*, real*);                                   \
     template void gather<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*, real, bool);    \
     template void ngp_phi<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*);               \
-    template void max_diff<real>(hipStream_t, int64_t, const real*, const real*, void*);
+    template void max_diff<real>(hipStream_t, int64_t, const real*, const real*, void*);                       \
+    template void ngp_bin<real>(hipStream_t, const PState<real>&, const Geo<real>&, NgpBins&);                   \
+    template void ngp_project<real>(hipStream_t, int, const NgpBins&, int64_t, const real*, const real*, real*,  \
+                                    real*);                                                                     \
+    template void fixed_to_mass<real>(hipStream_t, int, unsigned long long*, double, real*);
 
 RRG_PM_INST(float)
 RRG_PM_INST(double)

@@ -370,6 +370,7 @@ public:
             A_.alloc(n);
             B_.alloc(n);
         }
+        if (p.has_grid) grid_init(n, mass);
         configure(0, 0);
         RRG_HIP(hipStreamSynchronize(s_));
     }
@@ -453,9 +454,10 @@ public:
 
     // The reference returns early without a grid and leaves the potential
     // stale (rrgsim_integration_nbody.cuh:32-35); here it is recomputed.
-    void sync() override {
+    void sync() override {  // rrgsim_integration_nbody.cuh:32-48
         if (!grav_actual_) grav();
         grav_actual_ = true;
+        if (has_grid_) project_grid();
     }
 
     void get_state(double* pos, double* vel, double* acc, double* gr) override {
@@ -491,8 +493,17 @@ public:
         for (int g = 0; g < k_; ++g) fill_conservation(h + 8 * g, time_, out + g);
     }
 
-    void get_grid(double*, double*, double*) override {
-        fail(RRG_ERR_STATE, "rrg_get_grid: the direct-sum model has no grid");
+    // With a grid the direct-sum model projects onto it at every sync
+    // (rrgsim_integration_nbody.cuh:32-48): CIC cell mass, and the NGP cell
+    // averages of the particle potential and acceleration.
+    void get_grid(double* mass, double* phi, double* acc) override {
+        if (!has_grid_) fail(RRG_ERR_STATE, "rrg_get_grid: this direct-sum context has no grid");
+        sync();
+        const size_t nc = size_t(ncell_);
+        if (mass) RRG_HIP(hipMemcpyAsync(mass, gmass_.p, sizeof(double) * nc, hipMemcpyDeviceToHost, s_));
+        if (phi) RRG_HIP(hipMemcpyAsync(phi, gphi_.p, sizeof(double) * nc, hipMemcpyDeviceToHost, s_));
+        if (acc) RRG_HIP(hipMemcpyAsync(acc, gacc_.p, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost, s_));
+        RRG_HIP(hipStreamSynchronize(s_));
     }
 
     void stats(rrg_stats* out) override {
@@ -618,6 +629,45 @@ private:
         tuned_ = true;
         std::lock_guard<std::mutex> lk(g_tune_mutex);
         g_tune_cache[key] = best;
+    }
+
+    void grid_init(int64_t n, const double* mass) {
+        require(!ensemble_, "rrg_create_ensemble: an ensemble has no grid");
+        const rrg_grid_params& g = prm_.grid;
+        require(g.nx >= 2 && g.nx <= 1290, "rrg_create: grid nx must be in [2, 1290]");
+        require(g.dx > 0, "rrg_create: grid dx must be positive");
+        has_grid_ = true;
+        nx_ = g.nx;
+        ncell_ = int64_t(nx_) * nx_ * nx_;
+        geo_ = pm::make_geo<double>(nx_, g.dx, g.bc_frac);
+        double msum = 0;
+        for (int64_t i = 0; i < n; ++i) msum += std::fabs(mass[i]);
+        int ex = 0;
+        std::frexp(std::max(msum, 1e-300), &ex);
+        const int e = prm_.fixed_exponent > 0 ? prm_.fixed_exponent : std::min(61 - ex, 1000);
+        require(std::ldexp(msum, e) < std::ldexp(1.0, 62), "rrg_create: fixed_exponent too large for total mass");
+        scale_ = std::ldexp(1.0, e);
+        inv_scale_ = std::ldexp(1.0, -e);
+        mfix_.alloc(ncell_);
+        gmass_.alloc(ncell_);
+        gphi_.alloc(ncell_);
+        gacc_.alloc(3 * ncell_);
+        bkeys_.alloc(n); bkeys2_.alloc(n); bidx_.alloc(n); border_.alloc(n);
+        bcount_.alloc(ncell_); bstart_.alloc(ncell_);
+        btemp_.alloc(pm::ngp_temp_bytes(n, nx_));
+        zero_async(s_, mfix_.p, mfix_.bytes());
+    }
+
+    // convert_particles_to_grid (CIC mass, rm_p2mesh.cu:398-451) and the NGP
+    // projections of potential and acceleration (rm_p2mesh.cu:453-485).
+    void project_grid() {
+        pm::predict_deposit<double>(s_, P_.view(), 0.0, false, geo_, scale_, mfix_.p);
+        pm::fixed_to_mass<double>(s_, nx_, mfix_.p, inv_scale_, gmass_.p);
+        pm::NgpBins b;
+        b.keys = bkeys_.p; b.keys2 = bkeys2_.p; b.idx = bidx_.p; b.order = border_.p;
+        b.count = bcount_.p; b.start = bstart_.p; b.temp = btemp_.p; b.temp_bytes = btemp_.n;
+        pm::ngp_bin<double>(s_, P_.view(), geo_, b);
+        pm::ngp_project<double>(s_, nx_, b, P_.n, P_.G.p, P_.A.p, gphi_.p, gacc_.p);
     }
 
     void store_checkpoint() {
@@ -747,6 +797,17 @@ private:
     DBuf<double> Xb_, Vb_, Ab_, At_, gm_, vpart_;
     DBuf<float4> Uf_, Ap_, Bp_;
     nbody::Plan vplan_;
+    // optional grid (direct-sum model with a grid section)
+    bool has_grid_ = false;
+    int nx_ = 0;
+    int64_t ncell_ = 0;
+    pm::Geo<double> geo_{};
+    double scale_ = 0, inv_scale_ = 0;
+    DBuf<unsigned long long> mfix_;
+    DBuf<double> gmass_, gphi_, gacc_;
+    DBuf<unsigned> bkeys_, bkeys2_, bidx_, border_;
+    DBuf<int> bcount_, bstart_;
+    DBuf<char> btemp_;
 };
 
 // Particle-mesh strategy (RectWaveIntegrationSolver), state in `real`.
@@ -1680,6 +1741,33 @@ void gather_op(int64_t n, const void* pos, int32_t nx, double dx, double bc_frac
         for (int c = 0; c < 3; ++c) out[3 * i + c] = soa[c * n + i];
 }
 
+template <class real>
+void ngp_bin_op(int64_t n, const void* pos, int32_t nx, double dx, double bc_frac, int32_t* cell_of, int32_t* count,
+                int32_t* start, int32_t* order) {
+    OpStream os;
+    const int64_t ncell = int64_t(nx) * nx * nx;
+    Particles<real> P;
+    P.alloc(n);
+    std::vector<real> soa(3 * n);
+    const real* pr = static_cast<const real*>(pos);
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) soa[c * n + i] = pr[3 * i + c];
+    if (n) h2d(os.s, P.X.p, soa.data(), sizeof(real) * 3 * n);
+    DBuf<unsigned> keys(std::max<int64_t>(n, 1)), keys2(std::max<int64_t>(n, 1)), idx(std::max<int64_t>(n, 1)),
+        ord(std::max<int64_t>(n, 1));
+    DBuf<int> cnt(ncell), st(ncell);
+    DBuf<char> temp(pm::ngp_temp_bytes(n, nx));
+    pm::NgpBins b;
+    b.keys = keys.p; b.keys2 = keys2.p; b.idx = idx.p; b.order = ord.p;
+    b.count = cnt.p; b.start = st.p; b.temp = temp.p; b.temp_bytes = temp.n;
+    const pm::Geo<real> geo = pm::make_geo<real>(nx, dx, bc_frac);
+    pm::ngp_bin<real>(os.s, P.view(), geo, b);
+    if (cell_of && n) d2h(os.s, cell_of, keys.p, sizeof(int32_t) * n);
+    if (count) d2h(os.s, count, cnt.p, sizeof(int32_t) * ncell);
+    if (start) d2h(os.s, start, st.p, sizeof(int32_t) * ncell);
+    if (order && n) d2h(os.s, order, ord.p, sizeof(int32_t) * n);
+}
+
 template <class real> void steady_op(int64_t ncell, const void* un, const void* uo, double* err) {
     using MB = std::conditional_t<std::is_same_v<real, float>, unsigned int, unsigned long long>;
     OpStream os;
@@ -1751,6 +1839,16 @@ RRG_API int rrg_op_gather(int32_t real_type, int64_t n, const void* pos, int32_t
         real_size(real_type);
         if (real_type == RRG_REAL_F64) gather_op<double>(n, pos, nx, dx, bc_frac, field, acc);
         else gather_op<float>(n, pos, nx, dx, bc_frac, field, acc);
+    });
+}
+
+RRG_API int rrg_op_ngp_bin(int32_t real_type, int64_t n, const void* pos, int32_t nx, double dx, double bc_frac,
+                           int32_t* cell_of, int32_t* count, int32_t* start, int32_t* order) {
+    return guarded([&] {
+        require(n >= 0 && (n == 0 || pos) && nx >= 1 && nx <= 1290 && dx > 0, "rrg_op_ngp_bin: bad arguments");
+        real_size(real_type);
+        if (real_type == RRG_REAL_F64) ngp_bin_op<double>(n, pos, nx, dx, bc_frac, cell_of, count, start, order);
+        else ngp_bin_op<float>(n, pos, nx, dx, bc_frac, cell_of, count, start, order);
     });
 }
 

@@ -265,6 +265,26 @@ template <class real> void ngp_phi(hipStream_t s, const PState<real>& p, const G
 // Standalone max|a-b|, max|a| reduction (bit patterns, see wave_iteration).
 template <class real> void max_diff(hipStream_t s, int64_t ncell, const real* a, const real* b, void* maxbits);
 
+// NGP counting sort of the particles by cell (the binning of
+// rm_p2mesh.cu:25-126 with the intended exclusive prefix): count[c], start[c]
+// and order[] (particle indices sorted by cell, ascending within a cell).
+struct NgpBins {
+    unsigned *keys = nullptr, *keys2 = nullptr, *idx = nullptr, *order = nullptr;  // n each
+    int *count = nullptr, *start = nullptr;                                       // ncell each
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+};
+size_t ngp_temp_bytes(int64_t n, int nx);
+void exclusive_scan_i32(hipStream_t s, void* temp, size_t& temp_bytes, const int* in, int* out, int64_t n);
+template <class real> void ngp_bin(hipStream_t s, const PState<real>& p, const Geo<real>& g, NgpBins& b);
+// Cell averages of the particle potential (grav, n) and acceleration (acc, SoA
+// 3n) into cell_phi (ncell) and cell_acc (ncell x 3).
+template <class real>
+void ngp_project(hipStream_t s, int nx, const NgpBins& b, int64_t n, const real* grav, const real* acc, real* cell_phi,
+                 real* cell_acc);
+// Fixed-point grid -> cell mass; clears the fixed-point grid.
+template <class real> void fixed_to_mass(hipStream_t s, int nx, unsigned long long* mfix, double inv_scale, real* mass);
+
 // Fill kernel (used instead of hipMemsetAsync so the work stays capturable).
 void fill_u64(hipStream_t s, unsigned long long* p, size_t count, unsigned long long value);
 

@@ -13,5 +13,9 @@ void radix_sort_pairs(hipStream_t s, void* temp, size_t& temp_bytes, const unsig
                                                end_bit, s));
 }
 
+void exclusive_scan_i32(hipStream_t s, void* temp, size_t& temp_bytes, const int* in, int* out, int64_t n) {
+    RRG_HIP(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, int(n), s));
+}
+
 }  // namespace pm
 }  // namespace rrg

@@ -248,6 +248,17 @@ def gather(pos, nx, dx, field, bc_frac=0.0, dtype=np.float64):
     return out
 
 
+def ngp_bin(pos, nx, dx, bc_frac=0.0, dtype=np.float64):
+    """NGP counting sort of particles by cell -> (cell_of, count, start, order) (int32)."""
+    n = len(pos)
+    pos = np.ascontiguousarray(pos, dtype=dtype).reshape(n, 3)
+    cell_of = np.empty(n, np.int32); order = np.empty(n, np.int32)
+    count = np.empty(nx ** 3, np.int32); start = np.empty(nx ** 3, np.int32)
+    L.check(L.lib().rrg_op_ngp_bin(_rt(dtype), n, L.ptr(pos), nx, dx, bc_frac, L.ptr(cell_of), L.ptr(count),
+                                   L.ptr(start), L.ptr(order)))
+    return cell_of, count, start, order
+
+
 def steady_err(u_new, u_old, dtype=np.float64) -> float:
     a = np.ascontiguousarray(u_new, dtype=dtype); b = np.ascontiguousarray(u_old, dtype=dtype)
     e = L._f64(0)
