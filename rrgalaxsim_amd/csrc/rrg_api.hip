@@ -519,6 +519,7 @@ public:
         out->tuned = tuned_ ? 1 : 0;
         out->tune_candidates = tune_candidates_;
         out->graph_launches = graphs_.launches();
+        out->fixed_exponent = has_grid_ ? grid_e_ : 0;
     }
 
 private:
@@ -646,6 +647,7 @@ private:
         std::frexp(std::max(msum, 1e-300), &ex);
         const int e = prm_.fixed_exponent > 0 ? prm_.fixed_exponent : std::min(61 - ex, 1000);
         require(std::ldexp(msum, e) < std::ldexp(1.0, 62), "rrg_create: fixed_exponent too large for total mass");
+        grid_e_ = e;
         scale_ = std::ldexp(1.0, e);
         inv_scale_ = std::ldexp(1.0, -e);
         mfix_.alloc(ncell_);
@@ -803,6 +805,7 @@ private:
     int64_t ncell_ = 0;
     pm::Geo<double> geo_{};
     double scale_ = 0, inv_scale_ = 0;
+    int grid_e_ = 0;
     DBuf<unsigned long long> mfix_;
     DBuf<double> gmass_, gphi_, gacc_;
     DBuf<unsigned> bkeys_, bkeys2_, bidx_, border_;
