@@ -12,7 +12,7 @@ workload (N = 262,144, eps = 0.005) and the plain FP64 direct-sum kernel at
 the same N, as interactions/s; the dominant kernel's roofline; and the CPU
 oracle timed on a bounded sample of the headline workload.
 
-python bench.py [--gpus 1] [--steps 50] [--warmup 3] [--no-direct] [--no-cpu]
+python bench.py [--gpus 1] [--steps 50] [--warmup 3] [--no-direct] [--no-cpu] [--skip legs]
 """
 from __future__ import annotations
 
@@ -454,7 +454,14 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-pass", action="store_true", help="short PM run + calibration copy for rocprofv3 --pmc")
+    ap.add_argument("--skip", default="", help="comma-separated legs to leave out: direct, c1, scaling, ensemble, "
+                                              "adjoint, adjoint_pm, cpu")
     args = ap.parse_args()
+    skip = set(x for x in args.skip.split(",") if x)
+    t_start = time.perf_counter()
+
+    def done(leg):  # progress on stderr; stdout carries only the JSON line
+        print(f"bench: {leg} done at {time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
     if args.profile_pass:
         import rrgalaxsim_amd as rrg
         from rrgalaxsim_amd import ics
@@ -471,6 +478,7 @@ def main():
 
     rrg.lib()
     pm = pm_bench(rrg, ics, args.steps, args.warmup)
+    done("pm")
     achieved = pm["wave_bytes_per_launch"] / (pm["wave_avg_launch_us"] * 1e-6) / 1e9
     line = {
         "metric": METRIC,
@@ -500,19 +508,25 @@ def main():
         "pm_force_err_vs_fp64_direct": pm["pm_force_err_vs_fp64_direct"],
     }
     if not args.no_direct:
-        d = direct_bench(rrg, ics)
-        for k in ("mixed", "fp32"):
-            d[k]["fp32_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP32_PEAK_TFLOPS
-        for k in ("fp64", "fp64_exact"):
-            d[k]["fp64_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP64_PEAK_TFLOPS
-        line["direct_sum"] = d
-        line["direct_sum_c1"] = c1_bench(rrg, ics)
-        line["wave_scaling"] = wave_scaling(rrg, ics)
-        line["ensemble"] = ensemble_bench(rrg, ics)
-        line["adjoint"] = adjoint_bench(rrg, ics)
-        line["adjoint_pm"] = pm_adjoint_bench(rrg, ics)
-    if not args.no_cpu:
+        legs = [("direct", "direct_sum", lambda: direct_bench(rrg, ics)), ("c1", "direct_sum_c1", lambda: c1_bench(rrg, ics)),
+                ("scaling", "wave_scaling", lambda: wave_scaling(rrg, ics)),
+                ("ensemble", "ensemble", lambda: ensemble_bench(rrg, ics)),
+                ("adjoint", "adjoint", lambda: adjoint_bench(rrg, ics)),
+                ("adjoint_pm", "adjoint_pm", lambda: pm_adjoint_bench(rrg, ics))]
+        for leg, key, fn in legs:
+            if leg in skip:
+                continue
+            line[key] = fn()
+            done(leg)
+        if "direct_sum" in line:
+            d = line["direct_sum"]
+            for k in ("mixed", "fp32"):
+                d[k]["fp32_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP32_PEAK_TFLOPS
+            for k in ("fp64", "fp64_exact"):
+                d[k]["fp64_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP64_PEAK_TFLOPS
+    if not args.no_cpu and "cpu" not in skip:
         line["cpu_baseline"] = cpu_baseline(ics)
+        done("cpu")
     print(json.dumps(line), flush=True)
 
 

@@ -1848,7 +1848,8 @@ RRG_API int rrg_op_gather(int32_t real_type, int64_t n, const void* pos, int32_t
 RRG_API int rrg_op_ngp_bin(int32_t real_type, int64_t n, const void* pos, int32_t nx, double dx, double bc_frac,
                            int32_t* cell_of, int32_t* count, int32_t* start, int32_t* order) {
     return guarded([&] {
-        require(n >= 0 && (n == 0 || pos) && nx >= 1 && nx <= 1290 && dx > 0, "rrg_op_ngp_bin: bad arguments");
+        require(n >= 0 && n <= (int64_t(1) << 31) - 1 && (n == 0 || pos) && nx >= 1 && nx <= 1290 && dx > 0,
+                "rrg_op_ngp_bin: bad arguments");
         real_size(real_type);
         if (real_type == RRG_REAL_F64) ngp_bin_op<double>(n, pos, nx, dx, bc_frac, cell_of, count, start, order);
         else ngp_bin_op<float>(n, pos, nx, dx, bc_frac, cell_of, count, start, order);
