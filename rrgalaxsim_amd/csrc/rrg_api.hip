@@ -896,6 +896,7 @@ public:
     }
 
     void precalc() override {  // rrgsim_integration_rect.cuh:53-83
+        graphs_.clear();  // captured segments hold the boundary model by value (the centre of mass may change)
         sort_particles();
         deposit(false);
         if (model_.bc == RRG_BC_MONOPOLE_DIRICHLET || model_.bc == RRG_BC_MONOPOLE_RADIATIVE) {

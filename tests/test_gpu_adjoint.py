@@ -1,7 +1,7 @@
 """Adjoint of the direct-sum run on the GPU vs the FP64 CPU reverse-mode
 gradient of the oracle (itself pinned by finite differences in
 tests/test_oracle_nbody.py).  Bars: FP64 kinds agree to 1e-9 relative; the
-FP32 kinds' gradients within 1e-4 of the FP64 reference (the north-star
+FP32 kinds' gradients within 1e-4 of the FP64 reference (the target
 bound), measured as max |g - g_ref| / max |g_ref| per output.
 """
 import numpy as np

@@ -106,3 +106,26 @@ def test_monopole_centre_of_mass_is_order_free(rrg, orc):
     e = 61 - math.frexp(float(np.sum(m)))[1]
     com = orc.center_of_mass(pos, m, e)
     assert np.max(np.abs(com - np.average(pos, axis=0, weights=m))) < 1e-12
+
+
+def test_monopole_precalc_again_after_set_state(rrg):
+    """A second precalc recomputes the centre of mass; replayed step graphs
+    must use it (same result as a fresh context started from that state)."""
+    pos, vel, m = ics.exp_disk(4000, seed=12)
+    wave = dict(diss_base=0.785, diss_extra=1.0, wave_speed=C, setup_iterations=30, iterations=10, tolerance=0.0)
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=16, dx=1.0, bc_frac=0.2), wave=wave, boundary="monopole_dirichlet")
+    shifted = pos + np.array([0.9, -0.6, 0.3])
+    with rrg.Simulation(p, pos, vel, m) as sim:
+        sim.precalc()
+        sim.calc_step(3)
+        sim.set_state(pos=shifted, vel=vel)
+        sim.precalc()
+        sim.sync_particles_and_grid()
+        sim.calc_step(3)
+        a = sim.state()
+    with rrg.Simulation(p, shifted, vel, m) as sim:
+        sim.precalc()
+        sim.sync_particles_and_grid()
+        sim.calc_step(3)
+        b = sim.state()
+    assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])

@@ -5,7 +5,7 @@ potential, predictor/corrector) and nbody.cu:246-291 (conservation).
 Tolerances: FP64 kernel vs FP64 oracle -- summation order and fused
 multiply-adds differ, so agreement is to a few ulps of the per-particle
 force scale; mixed kernel -- median relative force error <= 1e-6 (the
-north-star bound), p99 reported.
+target bound), p99 reported.
 """
 import numpy as np
 import pytest
