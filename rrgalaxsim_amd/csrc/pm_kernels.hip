@@ -415,6 +415,126 @@ __global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real*
     }
 }
 
+// Two iterations per launch on a periodic NX^3 grid, NX = 64 XJ RS:
+// (phi^n, phi^{n-1}, src) -> (phi^{n+1}, phi^{n+2}), each level with the
+// same arithmetic as k_wave, so the pair is bitwise two k_wave launches.
+//
+// A workgroup owns TY rows x ZB planes of full x rows.  It has TY + 2 row
+// groups of RS waves (XJ cells per lane): rows Y0-1 .. Y0+TY compute level 1
+// (phi^{n+1}) marching z from Z0-1 to Z0+ZB and publish each plane in an LDS
+// ring of two planes; rows Y0 .. Y0+TY-1 then compute level 2 one plane
+// behind, reading the level-1 x/y neighbours from LDS and the z neighbours
+// from registers.  phi^n's x/y neighbours come from the cache, its own column
+// from registers.  Loads for the next plane are issued PD planes ahead; the
+// barrier only waits for LDS traffic, so they stay in flight across it.
+// HBM traffic per cell: 12 B (fp32) read + 8 B written for two iterations,
+// against 32 B for two single launches, plus one launch instead of two.
+template <class real, int XJ> struct PairIn { real a[XJ], b[XJ], c[XJ], d[XJ], po[XJ], src[XJ], pn[XJ]; };
+
+template <class real, int XJ, int RS, int TY, int ZB, int PD>
+__global__ __launch_bounds__(64 * RS * (TY + 2)) void k_wave_pair(real* __restrict__ q1, real* __restrict__ q2,
+                                                                  const real* __restrict__ p,
+                                                                  const real* __restrict__ po,
+                                                                  const real* __restrict__ src, WaveK<real> k) {
+    constexpr int NX = 64 * XJ * RS;
+    constexpr int NY = NX / TY, NB = NY * (NX / ZB);
+    constexpr int S = ZB + 2;  // level-1 planes Z0-1 .. Z0+ZB
+    static_assert((NX & (NX - 1)) == 0 && NX % TY == 0 && NX % ZB == 0, "tile");
+    __shared__ real lq[2][TY + 2][NX];
+    int b = blockIdx.x;
+    if constexpr ((NB & 7) == 0) b = (b & 7) * (NB >> 3) + (b >> 3);  // one z slab per XCD
+    const int by = b % NY, bz = b / NY;
+    const int wid = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int w = wid / RS, seg = wid - w * RS;
+    const int Y0 = by * TY, Z0 = bz * ZB;
+    const int y = (Y0 - 1 + w + NX) & (NX - 1);
+    const int ym = (y - 1) & (NX - 1), yp = (y + 1) & (NX - 1);
+    const bool inner = w >= 1 && w <= TY;
+    int x[XJ], xm[XJ], xp[XJ];
+#pragma unroll
+    for (int j = 0; j < XJ; ++j) {
+        x[j] = seg * 64 * XJ + 64 * j + l;
+        xm[j] = (x[j] - 1) & (NX - 1);
+        xp[j] = (x[j] + 1) & (NX - 1);
+    }
+    auto plane = [](int z) { return ((z + NX) & (NX - 1)) * NX * NX; };
+    auto load = [&](PairIn<real, XJ>& in, int zq) {
+        const int pl = plane(zq) + y * NX, pl2 = plane(zq + 2) + y * NX;
+        const int plm = plane(zq) + ym * NX, plp = plane(zq) + yp * NX;
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) {
+            in.a[j] = p[pl + xm[j]];
+            in.b[j] = p[pl + xp[j]];
+            in.c[j] = p[plm + x[j]];
+            in.d[j] = p[plp + x[j]];
+            in.po[j] = po[pl + x[j]];
+            in.src[j] = src[pl + x[j]];
+            in.pn[j] = p[pl2 + x[j]];
+        }
+    };
+    // phi^n at z-1, z, z+1 of the level-1 plane z; level 1 at z2-1, z2 of the
+    // level-2 plane z2; the source at z2
+    real Pm[XJ], P0[XJ], Pp[XJ], Qm[XJ], Q0[XJ], Sp[XJ];
+    {
+        const int a = plane(Z0 - 2) + y * NX, c = plane(Z0 - 1) + y * NX, d = plane(Z0) + y * NX;
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) {
+            Pm[j] = p[a + x[j]];
+            P0[j] = p[c + x[j]];
+            Pp[j] = p[d + x[j]];
+            Qm[j] = Q0[j] = Sp[j] = 0;
+        }
+    }
+    PairIn<real, XJ> ring[PD];
+#pragma unroll
+    for (int i = 0; i < PD; ++i) load(ring[i], Z0 - 1 + i);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const PairIn<real, XJ> in = ring[s % PD];
+        if (s + PD < S) load(ring[s % PD], Z0 - 1 + s + PD);
+        real qn[XJ];
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) {
+            // level 1 at plane Z0-1+s (wave.cu:183-190 order)
+            const real lap_sum = in.a[j] + in.b[j] + in.c[j] + in.d[j] + Pm[j] + Pp[j];
+            qn[j] = lap_sum * k.k_main + P0[j] * k.k_actual + in.po[j] * k.k_old + in.src[j];
+            lq[s & 1][w][x[j]] = qn[j];
+        }
+        if (s >= 2 && inner) {
+            // level 2 at plane z2 = Z0+s-2 from level 1 at z2-1 (Qm), z2 (Q0,
+            // x/y neighbours in LDS slot (s-1)&1) and z2+1 (qn)
+            const int sl = (s - 1) & 1;
+            const int base = (Z0 + s - 2) * NX * NX + y * NX;
+#pragma unroll
+            for (int j = 0; j < XJ; ++j) {
+                const real lap_sum = lq[sl][w][xm[j]] + lq[sl][w][xp[j]] + lq[sl][w - 1][x[j]] + lq[sl][w + 1][x[j]] +
+                                     Qm[j] + qn[j];
+                const real r = lap_sum * k.k_main + Q0[j] * k.k_actual + Pm[j] * k.k_old + Sp[j];
+                q1[base + x[j]] = Q0[j];
+                q2[base + x[j]] = r;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) {
+            Qm[j] = Q0[j];
+            Q0[j] = qn[j];
+            Sp[j] = in.src[j];
+            Pm[j] = P0[j];
+            P0[j] = Pp[j];
+            Pp[j] = in.pn[j];
+        }
+        // slot s&1 is rewritten at s+2, after every wave passed this barrier
+        if (s + 1 < S) __syncthreads();
+    }
+}
+
+template <class real, int XJ, int RS, int TY, int ZB, int PD>
+void launch_wave_pair(hipStream_t s, real* q1, real* q2, const real* p, const real* po, const real* src,
+                      const WaveK<real>& k) {
+    constexpr int NX = 64 * XJ * RS;
+    k_wave_pair<real, XJ, RS, TY, ZB, PD><<<(NX / TY) * (NX / ZB), 64 * RS * (TY + 2), 0, s>>>(q1, q2, p, po, src, k);
+}
+
 template <class real>
 __global__ __launch_bounds__(256) void k_max_diff(int64_t ncell, const real* __restrict__ a, const real* __restrict__ b,
                                                   typename MaxBits<real>::T* __restrict__ maxbits) {
@@ -852,6 +972,27 @@ void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& 
     RRG_LAUNCH_CHECK();
 }
 
+// Launch shapes of the two-iteration kernel, (XJ, RS, TY, ZB, PD), from
+// sweeps of graph-replayed 200-iteration chains on MI355X; grids without an
+// entry use single launches (256^3, where one iteration per launch is as
+// fast, and FP64 512^3, whose shape would spill registers).
+template <class real> bool wave_pair_ok(const WaveModel<real>& m) {
+    if (m.bc != RRG_BC_PERIODIC) return false;
+    return m.nx == 64 || m.nx == 128 || (m.nx == 512 && std::is_same_v<real, float>);
+}
+
+template <class real>
+void wave_pair(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, real* q1, real* q2, const real* p,
+               const real* po, const real* src) {
+    if (!wave_pair_ok(m)) fail(RRG_ERR_INVALID, "wave_pair: no two-iteration launch shape for this grid");
+    if (m.nx == 64) launch_wave_pair<real, 1, 1, 4, 4, 2>(s, q1, q2, p, po, src, k);
+    else if (m.nx == 128) {
+        if constexpr (std::is_same_v<real, float>) launch_wave_pair<real, 1, 2, 4, 8, 1>(s, q1, q2, p, po, src, k);
+        else launch_wave_pair<real, 1, 2, 4, 16, 1>(s, q1, q2, p, po, src, k);
+    } else if constexpr (std::is_same_v<real, float>) launch_wave_pair<real, 4, 2, 4, 32, 1>(s, q1, q2, p, po, src, k);
+    RRG_LAUNCH_CHECK();
+}
+
 template <class real>
 void cell_coefficients(hipStream_t s, const WaveModel<real>& m, real* km, real* ka, real* ko, real* alpha) {
     const int64_t ncell = int64_t(m.nx) * m.nx * m.nx;
@@ -954,6 +1095,9 @@ size_t ngp_temp_bytes(int64_t n, int nx) {
                                        const WaveModel<real>*);                                                 \
     template void wave_iteration<real>(hipStream_t, const WaveModel<real>&, const WaveK<real>&, real*,          \
                                        const real*, const real*, const real*, void*);                           \
+    template bool wave_pair_ok<real>(const WaveModel<real>&);                                                  \
+    template void wave_pair<real>(hipStream_t, const WaveModel<real>&, const WaveK<real>&, real*, real*,        \
+                                  const real*, const real*, const real*);                                       \
     template void gradient<real>(hipStream_t, int, real, const real*, real*);                                   \
     template void gather<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*, real, bool);    \
     template void ngp_phi<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*);               \

@@ -842,6 +842,10 @@ public:
             rho_mean_ = (real)1. / (L * L * L);
         }
         tol_ = (real)p.wave.tolerance;
+        // two stencil iterations per launch where a launch shape exists; the
+        // pair writes two grids other than its inputs, hence a fourth grid
+        pair_ = p.wave_pair != 0 && pm::wave_pair_ok<real>(model_);
+        nbuf_ = pair_ ? 4 : 3;
         // Fixed-point exponent: the largest e with sum|m| * 2^e below 2^62.
         double msum = 0;
         for (int64_t i = 0; i < n; ++i) msum += std::fabs(mass[i]);
@@ -884,13 +888,13 @@ public:
         mfix_.alloc(ncell_);
         mass_.alloc(ncell_);
         src_.alloc(ncell_);
-        for (auto& b : phi_) b.alloc(ncell_);
+        for (int i = 0; i < nbuf_; ++i) phi_[i].alloc(ncell_);
         gacc_.alloc(4 * ncell_);
         maxbits_.alloc(2);
         zero_async(s_, mfix_.p, mfix_.bytes());
         zero_async(s_, mass_.p, mass_.bytes());
         zero_async(s_, src_.p, src_.bytes());
-        for (auto& b : phi_) zero_async(s_, b.p, b.bytes());  // initial potential: zero
+        for (int i = 0; i < nbuf_; ++i) zero_async(s_, phi_[i].p, phi_[i].bytes());  // initial potential: zero
         zero_async(s_, gacc_.p, gacc_.bytes());
         RRG_HIP(hipStreamSynchronize(s_));
     }
@@ -905,7 +909,8 @@ public:
             pm::center_of_mass<real>(s_, P_.view(), e_, com_scratch_.p, model_.com);
             model_.mass = (real)1.;
         }
-        last_iters_ = solve(prm_.wave.setup_iterations);
+        int launches = 0;
+        last_iters_ = solve(prm_.wave.setup_iterations, &launches);
         pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
         grav_actual_ = false;
         acc_actual_ = false;
@@ -1091,7 +1096,7 @@ private:
     }
 
     uint64_t graph_base() const {
-        return (uint64_t(ip_) * 3 + uint64_t(ic_)) * 16 + (P_.X.p == x_first_ ? 0 : 8);
+        return (uint64_t(ip_) * 4 + uint64_t(ic_)) * 16 + (P_.X.p == x_first_ ? 0 : 8);
     }
 
     // One PM step as segments: predictor + deposit, source, the wave
@@ -1109,44 +1114,52 @@ private:
         });
         const int iters = prm_.wave.iterations;
         if (graph) {
-            segment(RRG_STAGE_WAVE, iters, true, b0 + 2, [&] {
-                int ip = ip_, ic = ic_;
+            const int ip0 = ip_, ic0 = ic_;
+            const int launches = rotate(iters);
+            segment(RRG_STAGE_WAVE, launches, true, b0 + 2, [&] {
+                int ip = ip0, ic = ic0;  // captured from the arrangement before the rotation
                 wave_launches(iters, ip, ic);
             });
-            rotate(iters);
             last_iters_ = iters;
         } else if (fixed_iters >= 0) {  // a recorded count, no checks (recomputation)
             const int t = timer.start(s_, RRG_STAGE_WAVE);
-            wave_launches(fixed_iters, ip_, ic_);
+            const int launches = wave_launches(fixed_iters, ip_, ic_);
             last_iters_ = fixed_iters;
-            timer.stop(s_, t, fixed_iters);
+            timer.stop(s_, t, launches);
         } else {
             const int t = timer.start(s_, RRG_STAGE_WAVE);
-            last_iters_ = solve(iters);
-            timer.stop(s_, t, last_iters_);
+            int launches = 0;
+            last_iters_ = solve(iters, &launches);
+            timer.stop(s_, t, launches);
         }
         const uint64_t b1 = graph_base();
         segment(RRG_STAGE_GRADIENT, 1, graph, b1 + 3, [&] { pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p); });
         segment(RRG_STAGE_GATHER, 1, graph, b1 + 4, [&] { pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true); });
     }
 
-    // `iters` stencil launches with the three-buffer rotation, no check.
-    void wave_launches(int iters, int& ip, int& ic) {
-        for (int it = 0; it < iters; ++it) {
-            const int inx = 3 - ic - ip;
-            pm::wave_iteration<real>(s_, model_, k_, phi_[inx].p, phi_[ic].p, phi_[ip].p, src_.p, nullptr);
-            ip = ic;
-            ic = inx;
-        }
+    // `iters` stencil iterations without a check; returns the launches.
+    int wave_launches(int iters, int& ip, int& ic) {
+        int launches = 0;
+        pm::wave_schedule(
+            iters, pair_, nbuf_, ip, ic,
+            [&](int out, int cur, int prev) {
+                pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p, nullptr);
+                ++launches;
+            },
+            [&](int o1, int o2, int cur, int prev) {
+                pm::wave_pair<real>(s_, model_, k_, phi_[o1].p, phi_[o2].p, phi_[cur].p, phi_[prev].p, src_.p);
+                ++launches;
+            });
+        return launches;
     }
 
-    // The three-buffer rotation of `iters` iterations (wave.cu:271-272).
-    void rotate(int iters) {
-        for (int i = 0; i < iters % 3; ++i) {
-            const int in = 3 - ic_ - ip_;
-            ip_ = ic_;
-            ic_ = in;
-        }
+    // The buffer rotation of `iters` iterations (wave.cu:271-272) as
+    // wave_launches leaves it, without launching.
+    int rotate(int iters) {
+        int launches = 0;
+        pm::wave_schedule(
+            iters, pair_, nbuf_, ip_, ic_, [&](int, int, int) { ++launches; }, [&](int, int, int, int) { ++launches; });
+        return launches;
     }
 
     real* cur() { return phi_[ic_].p; }
@@ -1279,38 +1292,49 @@ private:
     // (wave_steady_checker.cu:95-143).  A tolerance <= 0 can never be met
     // (the error is >= 0), so the checks are skipped and exactly max_iter
     // iterations run.
-    int solve(int max_iter) {
+    int solve(int max_iter, int* launches) {
         using MB = std::conditional_t<std::is_same_v<real, float>, unsigned int, unsigned long long>;
         const bool checking = tol_ > (real)0;
         const int check_step = prm_.wave.steady_check_step;
         int hold_count = 0;
         int iter = 0;
-        for (iter = 0; iter < max_iter; ++iter) {
-            const bool check = checking && iter && (iter % check_step == 0);
-            if (check) pm::fill_u64(s_, reinterpret_cast<unsigned long long*>(maxbits_.p), 2, 0ull);
-            const int inx = 3 - ic_ - ip_;
-            pm::wave_iteration<real>(s_, model_, k_, phi_[inx].p, phi_[ic_].p, phi_[ip_].p, src_.p,
-                                     check ? maxbits_.p : nullptr);
-            ip_ = ic_;
-            ic_ = inx;
-            if (check) {
-                host_.ensure(sizeof(MB) * 2);
-                RRG_HIP(hipMemcpyAsync(host_.p, maxbits_.p, sizeof(MB) * 2, hipMemcpyDeviceToHost, s_));
-                RRG_HIP(hipStreamSynchronize(s_));
-                MB bits[2];
-                std::memcpy(bits, host_.p, sizeof(bits));
-                real diff, absu;
-                std::memcpy(&diff, &bits[0], sizeof(real));
-                std::memcpy(&absu, &bits[1], sizeof(real));
-                const real EPS = (real)1.E-30;
-                const real err = diff / (absu + EPS);
-                if ((double)err < (double)tol_) ++hold_count;
-                else hold_count = 0;
-                if (hold_count >= prm_.wave.steady_hold_steps) {
-                    ++iter;
-                    break;
-                }
+        *launches = 0;
+        while (iter < max_iter) {
+            // the iterations before the next check (iter > 0, iter % check_step == 0) run unchecked
+            int c = max_iter;
+            if (checking) {
+                c = ((iter + check_step - 1) / check_step) * check_step;
+                if (c == 0) c = check_step;
+                c = std::min(c, max_iter);
             }
+            if (c > iter) {
+                *launches += wave_launches(c - iter, ip_, ic_);
+                iter = c;
+                continue;
+            }
+            pm::fill_u64(s_, reinterpret_cast<unsigned long long*>(maxbits_.p), 2, 0ull);
+            pm::wave_schedule(
+                1, false, nbuf_, ip_, ic_,
+                [&](int out, int cur, int prev) {
+                    pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p,
+                                             maxbits_.p);
+                },
+                [](int, int, int, int) {});
+            ++*launches;
+            ++iter;
+            host_.ensure(sizeof(MB) * 2);
+            RRG_HIP(hipMemcpyAsync(host_.p, maxbits_.p, sizeof(MB) * 2, hipMemcpyDeviceToHost, s_));
+            RRG_HIP(hipStreamSynchronize(s_));
+            MB bits[2];
+            std::memcpy(bits, host_.p, sizeof(bits));
+            real diff, absu;
+            std::memcpy(&diff, &bits[0], sizeof(real));
+            std::memcpy(&absu, &bits[1], sizeof(real));
+            const real EPS = (real)1.E-30;
+            const real err = diff / (absu + EPS);
+            if ((double)err < (double)tol_) ++hold_count;
+            else hold_count = 0;
+            if (hold_count >= prm_.wave.steady_hold_steps) break;
         }
         return iter;
     }
@@ -1337,7 +1361,9 @@ private:
     real* x_first_ = nullptr;  // P_.X at creation (the sort swaps buffers)
     StepGraphs graphs_;
     DBuf<unsigned long long> mfix_;
-    DBuf<real> mass_, src_, phi_[3], gacc_;
+    DBuf<real> mass_, src_, phi_[4], gacc_;
+    bool pair_ = false;  // two iterations per launch
+    int nbuf_ = 3;       // potential grids in the rotation
     DBuf<unsigned long long> maxbits_;
     DBuf<double> cpart_;
     PinnedBuf host_;
@@ -1685,7 +1711,7 @@ void wave_op(int32_t nx, double dx, double diss, double c, double wdt, int32_t b
              int32_t iters, void* prev, void* curr, const void* mass) {
     OpStream os;
     const int64_t ncell = int64_t(nx) * nx * nx;
-    DBuf<real> b[3], m(ncell), src(ncell);
+    DBuf<real> b[4], m(ncell), src(ncell);
     for (auto& x : b) x.alloc(ncell);
     h2d(os.s, b[0].p, prev, sizeof(real) * ncell);
     h2d(os.s, b[1].p, curr, sizeof(real) * ncell);
@@ -1695,12 +1721,12 @@ void wave_op(int32_t nx, double dx, double diss, double c, double wdt, int32_t b
     const pm::WaveModel<real> wm = pm::make_model<real>(nx, dx, 0.0, diss, 0.0, c, wdt, boundary);
     pm::mass_to_source<real>(os.s, nx, m.p, (real)dx, (real)rho_mean, periodic, k.k_f, src.p, periodic ? nullptr : &wm);
     int ip = 0, ic = 1;
-    for (int it = 0; it < iters; ++it) {
-        const int in = 3 - ic - ip;
-        pm::wave_iteration<real>(os.s, wm, k, b[in].p, b[ic].p, b[ip].p, src.p, nullptr);
-        ip = ic;
-        ic = in;
-    }
+    pm::wave_schedule(
+        iters, pm::wave_pair_ok<real>(wm), 4, ip, ic,
+        [&](int out, int cur, int prv) {
+            pm::wave_iteration<real>(os.s, wm, k, b[out].p, b[cur].p, b[prv].p, src.p, nullptr);
+        },
+        [&](int o1, int o2, int cur, int prv) { pm::wave_pair<real>(os.s, wm, k, b[o1].p, b[o2].p, b[cur].p, b[prv].p, src.p); });
     d2h(os.s, prev, b[ip].p, sizeof(real) * ncell);
     d2h(os.s, curr, b[ic].p, sizeof(real) * ncell);
 }

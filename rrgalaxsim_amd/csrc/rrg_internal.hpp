@@ -246,6 +246,45 @@ template <class real>
 void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, real* pn, const real* p,
                     const real* po, const real* src, void* maxbits);
 
+// Two wave_diss_iterations in one launch, (p = phi^n, po = phi^{n-1}) ->
+// (q1 = phi^{n+1}, q2 = phi^{n+2}), bitwise equal to two wave_iteration
+// calls; q1 and q2 must not alias p, po or src.  Available where
+// wave_pair_ok(m): periodic grids with a tuned launch shape.
+template <class real> bool wave_pair_ok(const WaveModel<real>& m);
+template <class real>
+void wave_pair(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, real* q1, real* q2, const real* p,
+               const real* po, const real* src);
+
+// The buffer schedule of `iters` unchecked iterations of solve_wave_equation
+// (wave.cu:253-272) over `nbuf` grids, (ip, ic) = (phi^{n-1}, phi^n):
+// two-iteration launches while two remain when `pair`, then single ones.
+// single(out, cur, prev) and pair(out1, out2, cur, prev) issue the launches;
+// (ip, ic) end on the last two levels.  Every output is a grid other than
+// the two inputs, so nbuf = 4 for pairs and 3 otherwise.
+template <class Single, class Pair>
+void wave_schedule(int iters, bool pair, int nbuf, int& ip, int& ic, Single&& single, Pair&& pairf) {
+    auto free_after = [&](int from) {
+        int f = from;
+        while (f == ip || f == ic) ++f;
+        return f;
+    };
+    int it = 0;
+    if (pair && nbuf >= 4) {
+        for (; it + 2 <= iters; it += 2) {
+            const int f1 = free_after(0), f2 = free_after(f1 + 1);
+            pairf(f1, f2, ic, ip);
+            ip = f1;
+            ic = f2;
+        }
+    }
+    for (; it < iters; ++it) {
+        const int f = free_after(0);
+        single(f, ic, ip);
+        ip = ic;
+        ic = f;
+    }
+}
+
 // Per-cell K_MAIN, K_ACTUAL, K_OLD and d(source)/d(cell mass) = 4 pi K_F / dx^3
 // of a non-periodic grid (for the transposed stencil).
 template <class real>

@@ -1,0 +1,122 @@
+"""Two stencil iterations per launch (periodic 64^3, 128^3, FP32 512^3):
+bitwise the same as the oracle's sequence of wave_diss_iteration calls
+(reference wave.cu:106-193, solve_wave_equation wave.cu:230-297), for odd and
+even iteration counts, inside full PM runs (graph-replayed steps, the
+steady-state check splitting the unchecked stretches, the recomputing
+adjoint), and bitwise equal to the same runs with one iteration per launch.
+"""
+import numpy as np
+import pytest
+
+from rrgalaxsim_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+C = 4574.337022617616
+
+
+def _fields(nx, dtype, seed):
+    rng = np.random.default_rng(seed)
+    ncell = nx ** 3
+    prev = (rng.standard_normal(ncell) * 0.1).astype(dtype)
+    curr = prev + (rng.standard_normal(ncell) * 0.01).astype(dtype)
+    mass = (rng.uniform(0, 1, ncell) / ncell).astype(dtype)
+    return prev, curr, mass
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nx,iters", [(64, 1), (64, 2), (64, 7), (128, 8), (128, 5)])
+def test_wave_pair_iterations_bitwise(rrg, orc, dtype, nx, iters):
+    dx = 16.0 / nx
+    prev, curr, mass = _fields(nx, dtype, nx + iters)
+    wdt = orc.wave_dt(dx, C, dtype=dtype)
+    rm = orc.rho_mean(1.0, nx, dx, dtype=dtype)
+    rp, rc, it = orc.solve_wave(nx, dx, 0.785, C, wdt, True, rm, prev, curr, mass, iters, 0.0, dtype=dtype)
+    assert it == iters
+    gp, gc = rrg.wave_iterate(nx, dx, 0.785, C, wdt, rm, iters, prev, curr, mass, dtype=dtype)
+    assert np.array_equal(gp, rp) and np.array_equal(gc, rc)
+
+
+def test_wave_pair_512_fp32_bitwise(rrg, orc):
+    nx, iters = 512, 3
+    dx = 16.0 / nx
+    prev, curr, mass = _fields(nx, np.float32, 5)
+    wdt = orc.wave_dt(dx, C, dtype=np.float32)
+    rm = orc.rho_mean(1.0, nx, dx, dtype=np.float32)
+    rp, rc, _ = orc.solve_wave(nx, dx, 0.785, C, wdt, True, rm, prev, curr, mass, iters, 0.0, dtype=np.float32)
+    gp, gc = rrg.wave_iterate(nx, dx, 0.785, C, wdt, rm, iters, prev, curr, mass, dtype=np.float32)
+    assert np.array_equal(gp, rp) and np.array_equal(gc, rc)
+
+
+def _run(rrg, dtype, *, nx, iters, setup, tol=0.0, check=30, hold=3, steps=3, pair=True, graphs=True, n=20_000):
+    pos, vel, m = ics.exp_disk(n, seed=4)
+    wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=setup, iterations=iters,
+                tolerance=tol, steady_check_step=check, steady_hold_steps=hold)
+    real = "fp32" if dtype == np.float32 else "fp64"
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real=real, wave_pair=pair,
+                        graphs=graphs)
+    with rrg.Simulation(p, pos, vel, m) as sim:
+        sim.precalc()
+        sim.sync_particles_and_grid()
+        sim.calc_step(steps)
+        sim.sync_particles_and_grid()
+        return (pos, vel, m), sim.state(), sim.grid(), sim.stats()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("iters,setup", [(7, 9), (8, 12)])
+def test_pm_run_with_pairs_matches_oracle(rrg, orc, dtype, iters, setup):
+    (pos, vel, m), st, gr, stats = _run(rrg, dtype, nx=64, iters=iters, setup=setup)
+    o = orc.pm_run(pos, vel, m, 3, nx=64, dx=0.25, dt=5e-3, diss_base=0.785, wave_speed=C, setup_iterations=setup,
+                   iterations=iters, fixed_e=stats["fixed_exponent"], dtype=dtype)
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(st[k].astype(dtype), o[k]), k
+    assert np.array_equal(gr["phi"].astype(dtype), o["phi"])
+
+
+def test_pm_steady_check_with_pairs_matches_oracle(rrg, orc):
+    """Checks every 5 iterations: pairs fill the unchecked stretches, the
+    checking iteration is a single launch, the early exit is the oracle's."""
+    (pos, vel, m), st, gr, stats = _run(rrg, np.float64, nx=64, iters=60, setup=400, tol=2e-3, check=5, hold=2,
+                                        steps=2)
+    o = orc.pm_run(pos, vel, m, 2, nx=64, dx=0.25, dt=5e-3, diss_base=0.785, wave_speed=C, setup_iterations=400,
+                   iterations=60, tolerance=2e-3, check_step=5, hold=2, fixed_e=stats["fixed_exponent"])
+    assert o["iterations"][0] < 400
+    assert stats["last_wave_iterations"] == int(o["iterations"][-1])
+    assert np.array_equal(st["pos"], o["pos"]) and np.array_equal(gr["phi"], o["phi"])
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_headline_grid_pair_equals_single(rrg, graphs):
+    """The 128^3 FP32 headline configuration (odd iteration count so the
+    last launch is a single one): pairs on and off give the same bits."""
+    out = []
+    for pair in (True, False):
+        _, st, gr, _ = _run(rrg, np.float32, nx=128, iters=21, setup=30, steps=2, pair=pair, graphs=graphs)
+        out.append((st, gr))
+    (a, ga), (b, gb) = out
+    assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
+    assert np.array_equal(ga["phi"], gb["phi"])
+
+
+def test_pm_adjoint_with_pairs_equals_single(rrg):
+    """The PM adjoint recomputes forward segments through the same schedule;
+    the forward states are bitwise equal and the gradients agree to the
+    reordering of the adjoint's FP64 atomic sums (run-to-run, 1e-12)."""
+    pos, vel, m = ics.exp_disk(2000, seed=2)
+    rng = np.random.default_rng(5)
+    gx, gv = rng.standard_normal((2000, 3)), rng.standard_normal((2000, 3))
+    wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=11, iterations=5, tolerance=0.0)
+    out = []
+    for pair in (True, False):
+        p = rrg.make_params(dt=0.02, grid=dict(nx=64, dx=0.25), wave=wave, wave_pair=pair)
+        with rrg.Simulation(p, pos, vel, m) as sim:
+            sim.precalc()
+            sim.adjoint_begin(3)
+            sim.calc_step(7)
+            st = sim.state()
+            out.append((st["pos"], st["vel"]) + tuple(sim.adjoint(gx, gv)))
+    a, b = out
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for ga, gb in zip(a[2:], b[2:]):
+        assert np.max(np.abs(ga - gb)) <= 1e-12 * np.max(np.abs(gb))
