@@ -39,8 +39,10 @@ C = 4574.337022617616
 # 12824.5 KiB fetched and 8192.0 KiB written per launch.  The summary with the
 # commands is profiles/wave_pm1e6_128_hbm_counters.txt.
 WAVE_TRAFFIC_128 = 34_647_301
-# The same for one k_wave_pair<float> launch (two iterations): not yet measured.
-WAVE_PAIR_TRAFFIC_128 = None
+# The same for one k_wave_pair<float> launch (two iterations), same method:
+# 14392.8 KiB fetched (x 1.9995) and 16384.0 KiB written per launch
+# (profiles/wave_pair_pm1e6_128_hbm_counters.txt).
+WAVE_PAIR_TRAFFIC_128 = 46_246_745
 
 PM = dict(n=1_000_000, nx=128, dx=0.125, dt=5e-3, iterations=200, setup_iterations=2000, diss_base=0.785,
           diss_extra=0.01)
