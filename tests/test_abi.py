@@ -67,6 +67,21 @@ def test_parse_fp32_cfl_step(rrg, tmp_path):
     assert p.real_type == 1 and p.force_kind == 1 and p.boundary == 0 and p.fixed_exponent == 50
 
 
+def test_parse_launch_options(rrg, tmp_path):
+    d = json.load(open(os.path.join(GOLDEN, "sample_params.json")))
+    p0, _ = rrg.parse_params_json(os.path.join(GOLDEN, "sample_params.json"))
+    assert p0.use_graphs == 1 and p0.autotune == 0 and p0.wave_pair == 1
+    d["mi355x"] = {"graphs": False, "autotune": True, "wave_pair": False}
+    f = tmp_path / "params.json"
+    f.write_text(json.dumps(d))
+    p, _ = rrg.parse_params_json(str(f))
+    assert p.use_graphs == 0 and p.autotune == 1 and p.wave_pair == 0
+    d["mi355x"] = {"wave_pair": 1}
+    f.write_text(json.dumps(d))
+    with pytest.raises(rrg.RRGError):
+        rrg.parse_params_json(str(f))
+
+
 def test_nbody_model_when_wave_section_absent(rrg, tmp_path):
     d = json.load(open(os.path.join(GOLDEN, "sample_params.json")))
     del d["wave_params"]
