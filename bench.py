@@ -39,10 +39,27 @@ C = 4574.337022617616
 # 12824.5 KiB fetched and 8192.0 KiB written per launch.  The summary with the
 # commands is profiles/wave_pm1e6_128_hbm_counters.txt.
 WAVE_TRAFFIC_128 = 34_647_301
-# The same for one k_wave_pair<float> launch (two iterations), same method:
-# 14392.8 KiB fetched (x 1.9995) and 16384.0 KiB written per launch
-# (profiles/wave_pair_pm1e6_128_hbm_counters.txt).
-WAVE_PAIR_TRAFFIC_128 = 46_246_745
+# The same for one three-iteration k_wave_fused<float, 3> launch: not yet measured.
+WAVE_FUSED_TRAFFIC_128 = None
+# Which fused launch shapes the library has (pm_kernels.hip, wave_fused_levels):
+# (three iterations, two iterations) per (real, nx) on periodic grids.
+FUSED_SHAPES = {("fp32", 64): (True, True), ("fp32", 128): (True, True), ("fp32", 256): (True, False),
+                ("fp32", 512): (True, True), ("fp64", 64): (True, True), ("fp64", 128): (True, True),
+                ("fp64", 256): (True, False)}
+
+
+def wave_schedule(iters, nx, real="fp32", fused=True):
+    """Launches of one unchecked solve of `iters` iterations, by iterations per
+    launch: the library's rule (as many three-iteration launches as fit, then
+    a two-iteration one where that shape exists, else single launches)."""
+    k3, k2 = FUSED_SHAPES.get((real, nx), (False, False)) if fused else (False, False)
+    n = {3: 0, 2: 0, 1: 0}
+    r = iters
+    while r > 0:
+        lv = 3 if (r >= 3 and k3) else (2 if (r >= 2 and k2) else 1)
+        n[lv] += 1
+        r -= lv
+    return n
 
 PM = dict(n=1_000_000, nx=128, dx=0.125, dt=5e-3, iterations=200, setup_iterations=2000, diss_base=0.785,
           diss_extra=0.01)
@@ -84,18 +101,18 @@ def pm_bench(rrg, ics, steps, warmup):
     wave_ms, wave_launches = tm["wave"]
     avg_wave_s = wave_ms / 1e3 / max(wave_launches, 1)
     iters = PM["iterations"]
-    launches = wave_launches / steps
-    pairs = int(round(iters - launches))  # two-iteration launches per step
-    singles = iters - 2 * pairs
-    # pair: read phi^n, phi^(n-1), source; write phi^(n+1), phi^(n+2) (fp32);
-    # single: read the same three, write phi^(n+1)
-    pair_bytes, single_bytes = 20 * ncell, 16 * ncell
-    wave_bytes = pair_bytes if pairs else single_bytes
+    sched = wave_schedule(iters, PM["nx"])
+    assert sched[3] + sched[2] + sched[1] == round(wave_launches / steps), (sched, wave_launches, steps)
+    fused = sched[3] + sched[2]
+    # fused launch: read phi^n, phi^(n-1), source; write the last two levels
+    # (fp32); single: read the same three, write phi^(n+1)
+    fused_bytes, single_bytes = 20 * ncell, 16 * ncell
+    wave_bytes = fused_bytes if fused else single_bytes
     n = PM["n"]
     # algorithmic bytes per step (fp32): wave, gradient (4 in, 16 out padded),
     # source (8 in, 4+4 out, 8 clear), deposit (x,v,a,m in + x,v* out + 8 atomics
     # of 8 B), gather+kick (x, v, v* in; a, v out; 8 corners of 16 B)
-    step_bytes = (pairs * pair_bytes + singles * single_bytes + 20 * ncell + 24 * ncell + n * (40 + 24 + 64) +
+    step_bytes = (fused * fused_bytes + sched[1] * single_bytes + 20 * ncell + 24 * ncell + n * (40 + 24 + 64) +
                   n * (36 + 24 + 128))
     secs = t1 - t0
     # PM force vs an FP64 direct sum over all particles (softening = dx): the
@@ -110,7 +127,7 @@ def pm_bench(rrg, ics, steps, warmup):
         "stage_ms_per_step": {k: v[0] / steps for k, v in tm.items() if v[1]},
         "wave_avg_launch_us": avg_wave_s * 1e6,
         "wave_us_per_iteration": wave_ms * 1e3 / (iters * steps),
-        "wave_launches_per_step": {"pair": pairs, "single": singles},
+        "wave_launches_per_step": {"three_iterations": sched[3], "two_iterations": sched[2], "single": sched[1]},
         "wave_bytes_per_launch": wave_bytes,
         "step_algorithmic_bytes": step_bytes,
         "wave_iterations_per_step": stats["last_wave_iterations"],
@@ -280,17 +297,18 @@ def adjoint_bench(rrg, ics, n=65_536, steps=200, every=20):
 def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2):
     """The stencil across grid sizes: the headline disk on 64^3 - 512^3 grids
     over the same [-8,8]^3 box (20 setup iterations, 200 per step, fp32),
-    with two iterations per launch where the library has a launch shape and
-    with one per launch; time per iteration and per launch from the stage
-    events, and the per-launch rate against the launch's algorithmic bytes."""
+    with several iterations per launch where the library has a launch shape
+    and with one per launch; time per iteration and per launch from the stage
+    events, and the launches' algorithmic bytes (fused: 20 B/cell, single:
+    16 B/cell) over the stage time."""
     pos, vel, m = ics.exp_disk(PM["n"], seed=1)
     out = {}
     for nx in sizes:
         row = {"cells": nx ** 3}
-        for pair in (True, False):
+        for fused in (True, False):
             wave = dict(diss_base=PM["diss_base"], diss_extra=PM["diss_extra"], wave_speed=C, setup_iterations=20,
                         iterations=PM["iterations"], tolerance=0.0)
-            p = rrg.make_params(dt=PM["dt"], grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real="fp32", wave_pair=pair)
+            p = rrg.make_params(dt=PM["dt"], grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real="fp32", wave_fused=fused)
             with rrg.Simulation(p, pos, vel, m) as sim:
                 sim.precalc()
                 sim.calc_step(3)  # captures the step graphs of every buffer arrangement
@@ -299,14 +317,14 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2):
                 sim.calc_step(steps)
                 tm = sim.timing()
             ms, launches = tm["wave"]
-            per_launch = PM["iterations"] * steps / launches
-            us = ms * 1e3 / launches
-            nbytes = (20.0 if per_launch > 1.5 else 16.0) * nx ** 3
-            gbs = nbytes / (us * 1e-6) / 1e9
-            row["pair" if pair else "single"] = {
-                "iterations_per_launch": per_launch, "us_per_launch": us,
-                "us_per_iteration": ms * 1e3 / (PM["iterations"] * steps), "bytes_per_launch": nbytes,
-                "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+            sched = wave_schedule(PM["iterations"], nx, fused=fused)
+            assert sum(sched.values()) * steps == launches, (nx, fused, sched, launches)
+            nbytes = ((sched[3] + sched[2]) * 20.0 + sched[1] * 16.0) * nx ** 3 * steps
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            row["fused" if fused else "single"] = {
+                "launches_per_step": {str(k): v for k, v in sched.items() if v}, "us_per_launch": ms * 1e3 / launches,
+                "us_per_iteration": ms * 1e3 / (PM["iterations"] * steps), "achieved_gbs": gbs,
+                "frac": gbs / HBM_PEAK_GBS}
         out[str(nx)] = row
     return out
 
@@ -518,11 +536,11 @@ def main():
         "config": {"workload": "PM: exponential disk N=1e6 (seed 1), 128^3 grid on [-8,8]^3, CIC + 200 damped-wave "
                                "iterations + gradient + trilinear gather, dt=5e-3, fp32",
                    "n": PM["n"], "nx": PM["nx"], "wave_iterations": PM["iterations"]},
-        "roofline": {"bound": "hbm", "kernel": "k_wave_pair<float>" if pm["wave_launches_per_step"]["pair"]
+        "roofline": {"bound": "hbm", "kernel": "k_wave_fused<float, 3>" if pm["wave_launches_per_step"]["three_iterations"]
                      else "k_wave<float>", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (WAVE_PAIR_TRAFFIC_128 if pm["wave_launches_per_step"]["pair"] else WAVE_TRAFFIC_128)
-                     if PM["nx"] == 128 else None,
+                     "traffic": (WAVE_FUSED_TRAFFIC_128 if pm["wave_launches_per_step"]["three_iterations"]
+                                 else WAVE_TRAFFIC_128) if PM["nx"] == 128 else None,
                      "algorithmic_bytes": pm["wave_bytes_per_launch"]},
         "step_roofline": {"algorithmic_bytes": pm["step_algorithmic_bytes"],
                           "achieved_gbs": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9,

@@ -130,9 +130,9 @@ typedef struct {
     int32_t use_graphs;     /* 1 (default): replay captured hipGraphs for steps */
     int32_t autotune;       /* 1: time candidate launch shapes at precalc and keep
                                the fastest (cached per process); default 0 */
-    int32_t wave_pair;      /* 1 (default): two stencil iterations per launch on
-                               periodic 64^3 and 128^3 grids (and FP32 512^3),
-                               bitwise the same results; 0: one per launch */
+    int32_t wave_fused;     /* 1 (default): up to three stencil iterations per
+                               launch on periodic 64^3, 128^3, 256^3 grids (and
+                               FP32 512^3), bitwise the same results; 0: one */
 } rrg_params;
 
 typedef struct {

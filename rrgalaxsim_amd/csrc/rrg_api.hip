@@ -842,10 +842,10 @@ public:
             rho_mean_ = (real)1. / (L * L * L);
         }
         tol_ = (real)p.wave.tolerance;
-        // two stencil iterations per launch where a launch shape exists; the
-        // pair writes two grids other than its inputs, hence a fourth grid
-        pair_ = p.wave_pair != 0 && pm::wave_pair_ok<real>(model_);
-        nbuf_ = pair_ ? 4 : 3;
+        // several stencil iterations per launch where a launch shape exists;
+        // such a launch writes two grids other than its inputs, hence a fourth
+        fused_ = p.wave_fused != 0 && pm::wave_fused_levels<real>(model_, 3) > 1;
+        nbuf_ = fused_ ? 4 : 3;
         // Fixed-point exponent: the largest e with sum|m| * 2^e below 2^62.
         double msum = 0;
         for (int64_t i = 0; i < n; ++i) msum += std::fabs(mass[i]);
@@ -1137,17 +1137,20 @@ private:
         segment(RRG_STAGE_GATHER, 1, graph, b1 + 4, [&] { pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true); });
     }
 
+    // iterations the next launch runs when `remaining` are left
+    int levels(int remaining) const { return fused_ ? pm::wave_fused_levels<real>(model_, remaining) : 1; }
+
     // `iters` stencil iterations without a check; returns the launches.
     int wave_launches(int iters, int& ip, int& ic) {
         int launches = 0;
         pm::wave_schedule(
-            iters, pair_, nbuf_, ip, ic,
+            iters, [&](int r) { return levels(r); }, ip, ic,
             [&](int out, int cur, int prev) {
                 pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p, nullptr);
                 ++launches;
             },
-            [&](int o1, int o2, int cur, int prev) {
-                pm::wave_pair<real>(s_, model_, k_, phi_[o1].p, phi_[o2].p, phi_[cur].p, phi_[prev].p, src_.p);
+            [&](int lv, int o1, int o2, int cur, int prev) {
+                pm::wave_fused<real>(s_, model_, k_, lv, phi_[o1].p, phi_[o2].p, phi_[cur].p, phi_[prev].p, src_.p);
                 ++launches;
             });
         return launches;
@@ -1158,7 +1161,8 @@ private:
     int rotate(int iters) {
         int launches = 0;
         pm::wave_schedule(
-            iters, pair_, nbuf_, ip_, ic_, [&](int, int, int) { ++launches; }, [&](int, int, int, int) { ++launches; });
+            iters, [&](int r) { return levels(r); }, ip_, ic_, [&](int, int, int) { ++launches; },
+            [&](int, int, int, int, int) { ++launches; });
         return launches;
     }
 
@@ -1314,12 +1318,12 @@ private:
             }
             pm::fill_u64(s_, reinterpret_cast<unsigned long long*>(maxbits_.p), 2, 0ull);
             pm::wave_schedule(
-                1, false, nbuf_, ip_, ic_,
+                1, [](int) { return 1; }, ip_, ic_,
                 [&](int out, int cur, int prev) {
                     pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p,
                                              maxbits_.p);
                 },
-                [](int, int, int, int) {});
+                [](int, int, int, int, int) {});
             ++*launches;
             ++iter;
             host_.ensure(sizeof(MB) * 2);
@@ -1362,7 +1366,7 @@ private:
     StepGraphs graphs_;
     DBuf<unsigned long long> mfix_;
     DBuf<real> mass_, src_, phi_[4], gacc_;
-    bool pair_ = false;  // two iterations per launch
+    bool fused_ = false;  // several iterations per launch
     int nbuf_ = 3;       // potential grids in the rotation
     DBuf<unsigned long long> maxbits_;
     DBuf<double> cpart_;
@@ -1722,11 +1726,13 @@ void wave_op(int32_t nx, double dx, double diss, double c, double wdt, int32_t b
     pm::mass_to_source<real>(os.s, nx, m.p, (real)dx, (real)rho_mean, periodic, k.k_f, src.p, periodic ? nullptr : &wm);
     int ip = 0, ic = 1;
     pm::wave_schedule(
-        iters, pm::wave_pair_ok<real>(wm), 4, ip, ic,
+        iters, [&](int r) { return pm::wave_fused_levels<real>(wm, r); }, ip, ic,
         [&](int out, int cur, int prv) {
             pm::wave_iteration<real>(os.s, wm, k, b[out].p, b[cur].p, b[prv].p, src.p, nullptr);
         },
-        [&](int o1, int o2, int cur, int prv) { pm::wave_pair<real>(os.s, wm, k, b[o1].p, b[o2].p, b[cur].p, b[prv].p, src.p); });
+        [&](int lv, int o1, int o2, int cur, int prv) {
+            pm::wave_fused<real>(os.s, wm, k, lv, b[o1].p, b[o2].p, b[cur].p, b[prv].p, src.p);
+        });
     d2h(os.s, prev, b[ip].p, sizeof(real) * ncell);
     d2h(os.s, curr, b[ic].p, sizeof(real) * ncell);
 }

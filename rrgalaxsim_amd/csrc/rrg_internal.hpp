@@ -246,42 +246,44 @@ template <class real>
 void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, real* pn, const real* p,
                     const real* po, const real* src, void* maxbits);
 
-// Two wave_diss_iterations in one launch, (p = phi^n, po = phi^{n-1}) ->
-// (q1 = phi^{n+1}, q2 = phi^{n+2}), bitwise equal to two wave_iteration
-// calls; q1 and q2 must not alias p, po or src.  Available where
-// wave_pair_ok(m): periodic grids with a tuned launch shape.
-template <class real> bool wave_pair_ok(const WaveModel<real>& m);
+// `levels` (2 or 3) wave_diss_iterations in one launch, (p = phi^n,
+// po = phi^{n-1}) -> (qa = phi^{n+levels-1}, qb = phi^{n+levels}), bitwise
+// equal to that many wave_iteration calls; qa and qb must not alias p, po or
+// src.  wave_fused_levels(m, r) is the number of iterations the next launch
+// should run when r remain: 3 or 2 where a launch shape exists (periodic
+// grids of the sizes it lists), else 1 (one wave_iteration).
+template <class real> int wave_fused_levels(const WaveModel<real>& m, int remaining);
 template <class real>
-void wave_pair(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, real* q1, real* q2, const real* p,
-               const real* po, const real* src);
+void wave_fused(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, int levels, real* qa, real* qb,
+                const real* p, const real* po, const real* src);
 
 // The buffer schedule of `iters` unchecked iterations of solve_wave_equation
-// (wave.cu:253-272) over `nbuf` grids, (ip, ic) = (phi^{n-1}, phi^n):
-// two-iteration launches while two remain when `pair`, then single ones.
-// single(out, cur, prev) and pair(out1, out2, cur, prev) issue the launches;
-// (ip, ic) end on the last two levels.  Every output is a grid other than
-// the two inputs, so nbuf = 4 for pairs and 3 otherwise.
-template <class Single, class Pair>
-void wave_schedule(int iters, bool pair, int nbuf, int& ip, int& ic, Single&& single, Pair&& pairf) {
+// (wave.cu:253-272), (ip, ic) = (phi^{n-1}, phi^n): levels(r) iterations
+// per launch while r remain.  single(out, cur, prev) and
+// multi(levels, out1, out2, cur, prev) issue the launches; (ip, ic) end on
+// the last two levels.  Outputs are grids other than the two inputs, so four
+// grids are needed when levels() can exceed 1, three otherwise.
+template <class Levels, class Single, class Multi>
+void wave_schedule(int iters, Levels&& levels, int& ip, int& ic, Single&& single, Multi&& multi) {
     auto free_after = [&](int from) {
         int f = from;
         while (f == ip || f == ic) ++f;
         return f;
     };
-    int it = 0;
-    if (pair && nbuf >= 4) {
-        for (; it + 2 <= iters; it += 2) {
+    while (iters > 0) {
+        const int L = levels(iters);
+        if (L >= 2) {
             const int f1 = free_after(0), f2 = free_after(f1 + 1);
-            pairf(f1, f2, ic, ip);
+            multi(L, f1, f2, ic, ip);
             ip = f1;
             ic = f2;
+        } else {
+            const int f = free_after(0);
+            single(f, ic, ip);
+            ip = ic;
+            ic = f;
         }
-    }
-    for (; it < iters; ++it) {
-        const int f = free_after(0);
-        single(f, ic, ip);
-        ip = ic;
-        ic = f;
+        iters -= L >= 2 ? L : 1;
     }
 }
 

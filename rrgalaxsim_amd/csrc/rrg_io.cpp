@@ -209,7 +209,7 @@ void params_default(rrg_params* p) {
     p->device = 0;
     p->use_graphs = 1;
     p->autotune = 0;
-    p->wave_pair = 1;
+    p->wave_fused = 1;
 }
 
 // validate_dependent_fields (io_json.cpp:170-186).  The CFL step is formed
@@ -312,9 +312,9 @@ void params_parse_json(const std::string& path, rrg_params* p, rrg_ini_params* i
             if (v->kind != JValue::Bool) fail(RRG_ERR_IO, "mi355x.autotune must be true or false");
             p->autotune = v->b ? 1 : 0;
         }
-        if (const JValue* v = m->find("wave_pair")) {
-            if (v->kind != JValue::Bool) fail(RRG_ERR_IO, "mi355x.wave_pair must be true or false");
-            p->wave_pair = v->b ? 1 : 0;
+        if (const JValue* v = m->find("wave_fused")) {
+            if (v->kind != JValue::Bool) fail(RRG_ERR_IO, "mi355x.wave_fused must be true or false");
+            p->wave_fused = v->b ? 1 : 0;
         }
     }
     params_finalize(p);

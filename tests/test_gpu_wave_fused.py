@@ -1,9 +1,10 @@
-"""Two stencil iterations per launch (periodic 64^3, 128^3, FP32 512^3):
-bitwise the same as the oracle's sequence of wave_diss_iteration calls
-(reference wave.cu:106-193, solve_wave_equation wave.cu:230-297), for odd and
-even iteration counts, inside full PM runs (graph-replayed steps, the
-steady-state check splitting the unchecked stretches, the recomputing
-adjoint), and bitwise equal to the same runs with one iteration per launch.
+"""Several stencil iterations per launch (three, or two for a remainder of
+two; periodic 64^3, 128^3, 256^3, FP32 512^3): bitwise the same as the
+oracle's sequence of wave_diss_iteration calls (reference wave.cu:106-193,
+solve_wave_equation wave.cu:230-297) for iteration counts with every
+remainder, inside full PM runs (graph-replayed steps, the steady-state check
+splitting the unchecked stretches, the recomputing adjoint), and bitwise
+equal to the same runs with one iteration per launch.
 """
 import numpy as np
 import pytest
@@ -25,8 +26,9 @@ def _fields(nx, dtype, seed):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("nx,iters", [(64, 1), (64, 2), (64, 7), (128, 8), (128, 5)])
-def test_wave_pair_iterations_bitwise(rrg, orc, dtype, nx, iters):
+@pytest.mark.parametrize("nx,iters", [(64, 1), (64, 2), (64, 3), (64, 7), (64, 8), (128, 8), (128, 6), (128, 4),
+                                     (256, 4), (256, 5)])
+def test_wave_fused_iterations_bitwise(rrg, orc, dtype, nx, iters):
     dx = 16.0 / nx
     prev, curr, mass = _fields(nx, dtype, nx + iters)
     wdt = orc.wave_dt(dx, C, dtype=dtype)
@@ -37,8 +39,8 @@ def test_wave_pair_iterations_bitwise(rrg, orc, dtype, nx, iters):
     assert np.array_equal(gp, rp) and np.array_equal(gc, rc)
 
 
-def test_wave_pair_512_fp32_bitwise(rrg, orc):
-    nx, iters = 512, 3
+def test_wave_fused_512_fp32_bitwise(rrg, orc):
+    nx, iters = 512, 5  # one three-iteration and one two-iteration launch
     dx = 16.0 / nx
     prev, curr, mass = _fields(nx, np.float32, 5)
     wdt = orc.wave_dt(dx, C, dtype=np.float32)
@@ -48,12 +50,12 @@ def test_wave_pair_512_fp32_bitwise(rrg, orc):
     assert np.array_equal(gp, rp) and np.array_equal(gc, rc)
 
 
-def _run(rrg, dtype, *, nx, iters, setup, tol=0.0, check=30, hold=3, steps=3, pair=True, graphs=True, n=20_000):
+def _run(rrg, dtype, *, nx, iters, setup, tol=0.0, check=30, hold=3, steps=3, fused=True, graphs=True, n=20_000):
     pos, vel, m = ics.exp_disk(n, seed=4)
     wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=setup, iterations=iters,
                 tolerance=tol, steady_check_step=check, steady_hold_steps=hold)
     real = "fp32" if dtype == np.float32 else "fp64"
-    p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real=real, wave_pair=pair,
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real=real, wave_fused=fused,
                         graphs=graphs)
     with rrg.Simulation(p, pos, vel, m) as sim:
         sim.precalc()
@@ -64,8 +66,8 @@ def _run(rrg, dtype, *, nx, iters, setup, tol=0.0, check=30, hold=3, steps=3, pa
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("iters,setup", [(7, 9), (8, 12)])
-def test_pm_run_with_pairs_matches_oracle(rrg, orc, dtype, iters, setup):
+@pytest.mark.parametrize("iters,setup", [(7, 9), (8, 12), (6, 10)])
+def test_pm_run_fused_matches_oracle(rrg, orc, dtype, iters, setup):
     (pos, vel, m), st, gr, stats = _run(rrg, dtype, nx=64, iters=iters, setup=setup)
     o = orc.pm_run(pos, vel, m, 3, nx=64, dx=0.25, dt=5e-3, diss_base=0.785, wave_speed=C, setup_iterations=setup,
                    iterations=iters, fixed_e=stats["fixed_exponent"], dtype=dtype)
@@ -74,9 +76,10 @@ def test_pm_run_with_pairs_matches_oracle(rrg, orc, dtype, iters, setup):
     assert np.array_equal(gr["phi"].astype(dtype), o["phi"])
 
 
-def test_pm_steady_check_with_pairs_matches_oracle(rrg, orc):
-    """Checks every 5 iterations: pairs fill the unchecked stretches, the
-    checking iteration is a single launch, the early exit is the oracle's."""
+def test_pm_steady_check_fused_matches_oracle(rrg, orc):
+    """Checks every 5 iterations: fused launches fill the unchecked stretches
+    (3 + 1 after the first check), the checking iteration is a single launch,
+    the early exit is the oracle's."""
     (pos, vel, m), st, gr, stats = _run(rrg, np.float64, nx=64, iters=60, setup=400, tol=2e-3, check=5, hold=2,
                                         steps=2)
     o = orc.pm_run(pos, vel, m, 2, nx=64, dx=0.25, dt=5e-3, diss_base=0.785, wave_speed=C, setup_iterations=400,
@@ -87,19 +90,21 @@ def test_pm_steady_check_with_pairs_matches_oracle(rrg, orc):
 
 
 @pytest.mark.parametrize("graphs", [True, False])
-def test_headline_grid_pair_equals_single(rrg, graphs):
-    """The 128^3 FP32 headline configuration (odd iteration count so the
-    last launch is a single one): pairs on and off give the same bits."""
+@pytest.mark.parametrize("nx,iters", [(128, 23), (256, 5)])
+def test_fused_equals_single(rrg, graphs, nx, iters):
+    """FP32 PM runs on the headline grid (23 = 7 x 3 + 2 iterations per step)
+    and on 256^3 (5 = 3 + 1 + 1): fused launches on and off give the same
+    bits."""
     out = []
-    for pair in (True, False):
-        _, st, gr, _ = _run(rrg, np.float32, nx=128, iters=21, setup=30, steps=2, pair=pair, graphs=graphs)
+    for fused in (True, False):
+        _, st, gr, _ = _run(rrg, np.float32, nx=nx, iters=iters, setup=31, steps=2, fused=fused, graphs=graphs)
         out.append((st, gr))
     (a, ga), (b, gb) = out
     assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
     assert np.array_equal(ga["phi"], gb["phi"])
 
 
-def test_pm_adjoint_with_pairs_equals_single(rrg):
+def test_pm_adjoint_fused_equals_single(rrg):
     """The PM adjoint recomputes forward segments through the same schedule;
     the forward states are bitwise equal and the gradients agree to the
     reordering of the adjoint's FP64 atomic sums (run-to-run, 1e-12)."""
@@ -108,8 +113,8 @@ def test_pm_adjoint_with_pairs_equals_single(rrg):
     gx, gv = rng.standard_normal((2000, 3)), rng.standard_normal((2000, 3))
     wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=11, iterations=5, tolerance=0.0)
     out = []
-    for pair in (True, False):
-        p = rrg.make_params(dt=0.02, grid=dict(nx=64, dx=0.25), wave=wave, wave_pair=pair)
+    for fused in (True, False):
+        p = rrg.make_params(dt=0.02, grid=dict(nx=64, dx=0.25), wave=wave, wave_fused=fused)
         with rrg.Simulation(p, pos, vel, m) as sim:
             sim.precalc()
             sim.adjoint_begin(3)
