@@ -22,6 +22,7 @@
 #include <hip/amd_detail/amd_hip_unsafe_atomics.h>
 
 #include "rrg_internal.hpp"
+#include "wave_fused.hpp"
 
 #include <cmath>
 
@@ -408,6 +409,20 @@ void adj_wave(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* 
     RRG_LAUNCH_CHECK();
 }
 
+template <class real> int adj_wave_levels(int nx, bool periodic, int remaining) {
+    return fused_levels<real>(nx, periodic ? RRG_BC_PERIODIC : RRG_BC_ZERO_DIRICHLET, remaining);
+}
+
+template <class real>
+void adj_wave_fused(hipStream_t s, int nx, const WaveK<real>& k, int levels, real* qa, real* qb, const real* p,
+                    const real* po, real* sbar, bool first) {
+    if (levels < 2 || adj_wave_levels<real>(nx, true, levels) != levels)
+        fail(RRG_ERR_INVALID, "adj_wave_fused: no launch shape for this grid and iteration count");
+    if (first) launch_fused<real, true, true>(s, nx, levels, qa, qb, p, po, nullptr, sbar, k);
+    else launch_fused<real, true, false>(s, nx, levels, qa, qb, p, po, nullptr, sbar, k);
+    RRG_LAUNCH_CHECK();
+}
+
 template <class real>
 void adj_cic(hipStream_t s, int64_t n, const real* x, const real* m, const Geo<real>& g, const real* sbar,
              double alpha, double dt, bool closing, double* Xb, double* Vb, double* Ab, double* gm,
@@ -426,6 +441,9 @@ void adj_cic(hipStream_t s, int64_t n, const real* x, const real* m, const Geo<r
     template void adj_gradient_t<real>(hipStream_t, int, real, bool, const double*, real*);                      \
     template void adj_wave<real>(hipStream_t, int, bool, const WaveK<real>&, real*, const real*, const real*,    \
                                  real*, bool, const real*, const real*, const real*);                            \
+    template int adj_wave_levels<real>(int, bool, int);                                                          \
+    template void adj_wave_fused<real>(hipStream_t, int, const WaveK<real>&, int, real*, real*, const real*,     \
+                                       const real*, real*, bool);                                                \
     template void adj_cic<real>(hipStream_t, int64_t, const real*, const real*, const Geo<real>&, const real*,  \
                                 double, double, bool, double*, double*, double*, double*, const real*);
 

@@ -983,7 +983,7 @@ public:
         }
         zero_async(s_, Ab_.p, Ab_.bytes());
         zero_async(s_, gm_.p, gm_.bytes());
-        for (auto& b : mu_) zero_async(s_, b.p, b.bytes());
+        for (int i = 0; i < nbuf_; ++i) zero_async(s_, mu_[i].p, mu_[i].bytes());
         aip_ = 0;
         aic_ = 1;
         // keep the live state
@@ -1205,7 +1205,7 @@ private:
         if (Xb_.n != size_t(3 * n)) {
             Xb_.alloc(3 * n); Vb_.alloc(3 * n); Ab_.alloc(3 * n); gm_.alloc(n);
             Gb_.alloc(3 * ncell_);
-            for (auto& b : mu_) b.alloc(ncell_);
+            for (int i = 0; i < nbuf_; ++i) mu_[i].alloc(ncell_);
             sbar_.alloc(ncell_);
             gtmp_.alloc(4 * ncell_);
             zero_async(s_, Gb_.p, Gb_.bytes());
@@ -1234,31 +1234,40 @@ private:
         if (iters == 0) {
             zero_async(s_, sbar_.p, sbar_.bytes());
         } else {
-            auto run = [&](int& ip, int& ic) {
-                for (int it = 0; it < iters; ++it) {
-                    const int in = 3 - ip - ic;
-                    pm::adj_wave<real>(s_, nx_, periodic_, k_, mu_[in].p, mu_[ic].p, mu_[ip].p, sbar_.p, it == 0,
-                                       periodic_ ? nullptr : kc_[0].p, kc_[1].p, kc_[2].p);
-                    ip = ic;
-                    ic = in;
-                }
+            // the transposed iterations, several per launch like the forward's
+            auto levels = [&](int r) { return fused_ ? pm::adj_wave_levels<real>(nx_, periodic_, r) : 1; };
+            auto run = [&](int& ip, int& ic, bool launch) {
+                int launches = 0;
+                pm::wave_schedule(
+                    iters, levels, ip, ic,
+                    [&](int out, int cur, int prev) {
+                        if (launch)
+                            pm::adj_wave<real>(s_, nx_, periodic_, k_, mu_[out].p, mu_[cur].p, mu_[prev].p, sbar_.p,
+                                               launches == 0, periodic_ ? nullptr : kc_[0].p, kc_[1].p, kc_[2].p);
+                        ++launches;
+                    },
+                    [&](int lv, int o1, int o2, int cur, int prev) {
+                        if (launch)
+                            pm::adj_wave_fused<real>(s_, nx_, k_, lv, mu_[o1].p, mu_[o2].p, mu_[cur].p, mu_[prev].p,
+                                                     sbar_.p, launches == 0);
+                        ++launches;
+                    });
+                return launches;
             };
             t = timer.start(s_, RRG_STAGE_ADJ_WAVE);
+            int launches = 0;
             if (prm_.use_graphs) {
-                const uint64_t key = (uint64_t(1) << 40) | (uint64_t(iters) << 4) | uint64_t(aip_ * 3 + aic_);
+                const uint64_t key = (uint64_t(1) << 40) | (uint64_t(iters) << 4) | uint64_t(aip_ * 4 + aic_);
+                const int ip0 = aip_, ic0 = aic_;
+                launches = run(aip_, aic_, false);  // the rotation the launches leave
                 graphs_.run(s_, key, [&] {
-                    int ip = aip_, ic = aic_;
-                    run(ip, ic);
+                    int ip = ip0, ic = ic0;
+                    run(ip, ic, true);
                 });
-                for (int i = 0; i < iters % 3; ++i) {
-                    const int in = 3 - aip_ - aic_;
-                    aip_ = aic_;
-                    aic_ = in;
-                }
             } else {
-                run(aip_, aic_);
+                launches = run(aip_, aic_, true);
             }
-            timer.stop(s_, t, iters);
+            timer.stop(s_, t, launches);
         }
         const real dx3 = geo_.dx * geo_.dx * geo_.dx;
         const double alpha = 4 * 3.14159265358979 * (double)k_.k_f / (double)dx3;  // d source / d cell mass
@@ -1385,7 +1394,7 @@ private:
     std::vector<DBuf<real>> ck_;        // (x, v, a, phi_prev, phi_curr) every ck_every_ steps
     DBuf<real> backup_, xs_, phis_;     // live state; x' and phi_I of one segment
     DBuf<double> Xb_, Vb_, Ab_, gm_, Gb_;
-    DBuf<real> mu_[3], sbar_, gtmp_;
+    DBuf<real> mu_[4], sbar_, gtmp_;
     DBuf<real> kc_[4];                  // off-periodic K_MAIN, K_ACTUAL, K_OLD, d source / d mass per cell
     int aip_ = 0, aic_ = 1;             // potential adjoints mu_{k+1}, mu_k
 };

@@ -343,6 +343,14 @@ template <class real> void adj_gradient_t(hipStream_t s, int nx, real dx, bool p
 template <class real>
 void adj_wave(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* pn, const real* p, const real* po,
               real* sbar, bool first, const real* km = nullptr, const real* ka = nullptr, const real* ko = nullptr);
+// `levels` (2 or 3) periodic transposed iterations in one launch, (p = mu_k,
+// po = mu_{k+1}) -> (qa = mu_{k-levels+1}, qb = mu_{k-levels}), with sbar
+// accumulated as by that many adj_wave calls (bitwise).  adj_wave_levels is
+// wave_fused_levels for the transposed stencil.
+template <class real> int adj_wave_levels(int nx, bool periodic, int remaining);
+template <class real>
+void adj_wave_fused(hipStream_t s, int nx, const WaveK<real>& k, int levels, real* qa, real* qb, const real* p,
+                    const real* po, real* sbar, bool first);
 // CIC adjoint at x' (gm += alpha CIC^T sbar, Xb += alpha m dCIC^T sbar; alpha
 // per cell when `alpha_cell` is non-null) and, unless `closing`, the
 // transposed Verlet update of (Xb, Vb, Ab).

@@ -1,0 +1,262 @@
+// Several damped-wave iterations per launch on periodic grids: the forward
+// stencil of wave_diss_iteration (solver/lib/rrgsim_wave/src/wave.cu:106-193)
+// and its transpose (the potential adjoint), shared by pm_kernels.hip and
+// pm_adjoint_kernels.hip.  Included inside namespace rrg::pm of those files.
+#pragma once
+
+#include <type_traits>
+
+#include "rrg_internal.hpp"
+
+namespace rrg {
+namespace pm {
+namespace {
+
+// K iterations per launch (K = 2, 3) on a periodic NX^3 grid, NX = 64 XJ RS.
+//
+// Forward (ADJ false): (phi^n, phi^{n-1}, src) -> (phi^{n+K-1}, phi^{n+K}),
+// every level with the arithmetic of k_wave, so one launch is bitwise K
+// k_wave launches.  Adjoint (ADJ true): the transposed stencil of
+// k_adj_wave (no source term) from (mu_k, mu_{k+1}) to (mu_{k-K+1},
+// mu_{k-K}), with sbar accumulating each iteration's input in the same order
+// as K k_adj_wave launches: sbar = ((sbar + mu_k) + mu_{k-1}) + ... (FIRST:
+// the first term replaces sbar).
+//
+// A workgroup owns TY rows x ZB planes of full x rows and has TY + 2(K-1)
+// row groups of RS waves (XJ cells per lane), rows Y0-(K-1) .. Y0+TY+K-2.
+// Level 1 runs on all of them, marching z from Z0-(K-1); level L runs one
+// plane behind level L-1 on the rows with a (K-L)-row halo, reading level
+// L-1's x/y neighbours from a two-plane LDS ring and its z neighbours from
+// registers (the last three planes of every level stay in registers; level
+// L's "old" term is level L-2 two planes back, the input's own column for
+// L = 2).  The input's x/y neighbours come from the cache.  Loads for the
+// next plane are issued PD planes ahead; the barrier only waits for LDS
+// traffic, so they stay in flight across it.  HBM traffic per cell: 3 reals
+// read and 2 written for K forward iterations (4K for K single launches),
+// 3 read and 3 written for K adjoint iterations (5K), and one launch
+// instead of K.
+template <class real, int XJ> struct PlaneIn {
+    real a[XJ], b[XJ], c[XJ], d[XJ], po[XJ], src[XJ], pn[XJ], sb[XJ];
+};
+
+template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool ADJ, bool FIRST>
+__global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real* __restrict__ qa,
+                                                                           real* __restrict__ qb,
+                                                                           const real* __restrict__ p,
+                                                                           const real* __restrict__ po,
+                                                                           const real* __restrict__ src,
+                                                                           real* __restrict__ sbar, WaveK<real> k) {
+    constexpr int NX = 64 * XJ * RS;
+    constexpr int NY = NX / TY, NB = NY * (NX / ZB);
+    constexpr int ROWS = TY + 2 * K - 2;
+    constexpr int S = ZB + 2 * K - 2;  // level-1 planes Z0-(K-1) .. Z0+ZB+K-2
+    static_assert(K >= 2 && (NX & (NX - 1)) == 0 && NX % TY == 0 && NX % ZB == 0, "shape");
+    __shared__ real lq[K - 1][2][ROWS][NX];
+    int b = blockIdx.x;
+    if constexpr ((NB & 7) == 0) b = (b & 7) * (NB >> 3) + (b >> 3);  // one z slab per XCD
+    const int by = b % NY, bz = b / NY;
+    // the wave index is uniform: row offsets live in scalar registers and every
+    // global access is a scalar base plus the lane's column offset
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+    const int w = wid / RS, seg = wid - w * RS;
+    const int Y0 = by * TY, Z0 = bz * ZB;
+    const int y = (Y0 - (K - 1) + w + NX) & (NX - 1);
+    const int ym = (y - 1) & (NX - 1), yp = (y + 1) & (NX - 1);
+    const bool out_row = w >= K - 1 && w < K - 1 + TY;  // rows Y0 .. Y0+TY-1
+    unsigned x[XJ], xm[XJ], xp[XJ];
+#pragma unroll
+    for (int j = 0; j < XJ; ++j) {
+        x[j] = unsigned(seg * 64 * XJ + 64 * j + l);
+        xm[j] = (x[j] - 1u) & unsigned(NX - 1);
+        xp[j] = (x[j] + 1u) & unsigned(NX - 1);
+    }
+    // offset of row yy of plane z (z wraps)
+    auto row = [](int z, int yy) { return size_t(unsigned((z + NX) & (NX - 1)) * unsigned(NX) + unsigned(yy)) * NX; };
+    const int zs = Z0 - (K - 1);  // level-1 plane of step 0
+    // the inputs of step t: the level-1 plane zs+t, and (adjoint) sbar on the
+    // output plane of step t
+    auto load = [&](PlaneIn<real, XJ>& in, int t) {
+        const int zq = zs + t;
+        const real* __restrict__ r0 = p + row(zq, y);
+        const real* __restrict__ rm = p + row(zq, ym);
+        const real* __restrict__ rp = p + row(zq, yp);
+        const real* __restrict__ r2 = p + row(zq + 2, y);
+        const real* __restrict__ ro = po + row(zq, y);
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) {
+            in.a[j] = r0[xm[j]];
+            in.b[j] = r0[xp[j]];
+            in.c[j] = rm[x[j]];
+            in.d[j] = rp[x[j]];
+            in.po[j] = ro[x[j]];
+            in.pn[j] = r2[x[j]];
+        }
+        if constexpr (!ADJ) {
+            const real* __restrict__ rs = src + row(zq, y);
+#pragma unroll
+            for (int j = 0; j < XJ; ++j) in.src[j] = rs[x[j]];
+        } else if (!FIRST && t >= 2 * (K - 1) && out_row) {
+            const real* __restrict__ rb = sbar + row(Z0 + t - 2 * (K - 1), y);
+#pragma unroll
+            for (int j = 0; j < XJ; ++j) in.sb[j] = rb[x[j]];
+        }
+    };
+    // the input at planes z+1, z, z-1, z-2 of the level-1 plane z; every
+    // level's value of the previous step (h1) and of two steps back (h2); the
+    // source of the level-1 plane of the last K steps (level L at step s is on
+    // the level-1 plane of step s-L+1)
+    real Pv[4][XJ];
+    real h1[K + 1][XJ], h2[K + 1][XJ], sh[K][XJ];
+    {
+        const real* __restrict__ a = p + row(zs - 1, y);
+        const real* __restrict__ c = p + row(zs, y);
+        const real* __restrict__ d = p + row(zs + 1, y);
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) {
+            Pv[3][j] = 0;
+            Pv[2][j] = a[x[j]];
+            Pv[1][j] = c[x[j]];
+            Pv[0][j] = d[x[j]];
+#pragma unroll
+            for (int L = 0; L <= K; ++L) h1[L][j] = h2[L][j] = 0;
+#pragma unroll
+            for (int i = 0; i < K; ++i) sh[i][j] = 0;
+        }
+    }
+    PlaneIn<real, XJ> ring[PD];
+#pragma unroll
+    for (int i = 0; i < PD; ++i) load(ring[i], i);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const PlaneIn<real, XJ> in = ring[s % PD];
+        if (s + PD < S) load(ring[s % PD], s + PD);
+        real nw[K + 1][XJ];
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) {
+            if constexpr (!ADJ) {
+#pragma unroll
+                for (int i = K - 1; i > 0; --i) sh[i][j] = sh[i - 1][j];
+                sh[0][j] = in.src[j];
+            }
+            // level 1 (wave.cu:183-190 order; the adjoint drops the source)
+            const real lap_sum = in.a[j] + in.b[j] + in.c[j] + in.d[j] + Pv[2][j] + Pv[0][j];
+            if constexpr (ADJ) nw[1][j] = lap_sum * k.k_main + Pv[1][j] * k.k_actual + in.po[j] * k.k_old;
+            else nw[1][j] = lap_sum * k.k_main + Pv[1][j] * k.k_actual + in.po[j] * k.k_old + in.src[j];
+            lq[0][s & 1][w][x[j]] = nw[1][j];
+        }
+#pragma unroll
+        for (int L = 2; L <= K; ++L) {
+            // level L is live from step 2(L-1) on, on rows L-1 .. ROWS-L
+            if (s >= 2 * (L - 1) && w >= L - 1 && w < ROWS - (L - 1)) {
+                const int sl = (s - 1) & 1;
+#pragma unroll
+                for (int j = 0; j < XJ; ++j) {
+                    const real lap_sum = lq[L - 2][sl][w][xm[j]] + lq[L - 2][sl][w][xp[j]] +
+                                         lq[L - 2][sl][w - 1][x[j]] + lq[L - 2][sl][w + 1][x[j]] + h2[L - 1][j] +
+                                         nw[L - 1][j];
+                    const real old = L == 2 ? Pv[2][j] : h2[L - 2][j];
+                    if constexpr (ADJ) nw[L][j] = lap_sum * k.k_main + h1[L - 1][j] * k.k_actual + old * k.k_old;
+                    else nw[L][j] = lap_sum * k.k_main + h1[L - 1][j] * k.k_actual + old * k.k_old + sh[L - 1][j];
+                    if (L < K) lq[L - 1][s & 1][w][x[j]] = nw[L][j];
+                }
+                if (L == K && out_row) {
+                    const size_t o = row(Z0 + s - 2 * (K - 1), y);
+#pragma unroll
+                    for (int j = 0; j < XJ; ++j) {
+                        qa[o + x[j]] = h1[K - 1][j];
+                        qb[o + x[j]] = nw[K][j];
+                    }
+                    if constexpr (ADJ) {
+                        // the inputs of the K iterations at this cell: the
+                        // input plane K below the newest, level L (1 .. K-1)
+                        // from K-L steps back
+#pragma unroll
+                        for (int j = 0; j < XJ; ++j) {
+                            real acc = FIRST ? Pv[K][j] : in.sb[j] + Pv[K][j];
+                            if constexpr (K == 3) acc = acc + h2[1][j];
+                            acc = acc + h1[K - 1][j];
+                            sbar[o + x[j]] = acc;
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < XJ; ++j) nw[L][j] = 0;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) {
+#pragma unroll
+            for (int L = 1; L <= K; ++L) {
+                h2[L][j] = h1[L][j];
+                h1[L][j] = nw[L][j];
+            }
+            Pv[3][j] = Pv[2][j];
+            Pv[2][j] = Pv[1][j];
+            Pv[1][j] = Pv[0][j];
+            Pv[0][j] = in.pn[j];
+        }
+        // ring slot s&1 is rewritten at s+2, after every wave passed this barrier
+        if (s + 1 < S) __syncthreads();
+    }
+}
+
+template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool ADJ, bool FIRST>
+void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* po, const real* src, real* sbar,
+                  const WaveK<real>& k) {
+    constexpr int NX = 64 * XJ * RS;
+    k_wave_fused<real, K, XJ, RS, TY, ZB, PD, ADJ, FIRST>
+        <<<(NX / TY) * (NX / ZB), 64 * RS * (TY + 2 * K - 2), 0, s>>>(qa, qb, p, po, src, sbar, k);
+}
+
+// Launch shapes (K, XJ, RS, TY, ZB, PD), from sweeps of graph-replayed
+// 240-iteration chains of the forward kernel on MI355X (time per iteration,
+// FP32 / FP64; one iteration per launch in brackets):
+//   64^3   K=3 (1,1,4,4,2) 1.51 us (2.74) / K=3 (1,1,4,4,1) 2.28 us (3.32)
+//   128^3  K=3 (2,1,8,8,1) 3.51-3.61 us (6.8) / 7.44 us (12.0)
+//   256^3  K=3 (4,1,8,32,1) 26.7 us (38.0) / K=3 (2,2,4,32,1) 71.6 us (115.6)
+//   512^3  K=3 (4,2,4,32,1) 222.8 us (440) / none: FP64 shapes spill registers
+// and for a remainder of two iterations K=2 where it beat two single
+// launches (64^3, 128^3, FP32 512^3).  The adjoint uses the same shapes.
+// Other grids, and non-periodic ones, run one iteration per launch.
+template <class real> int fused_levels(int nx, int bc, int remaining) {
+    constexpr bool f32 = std::is_same_v<real, float>;
+    if (bc != RRG_BC_PERIODIC || remaining < 2) return 1;
+    const bool k3 = nx == 64 || nx == 128 || nx == 256 || (f32 && nx == 512);
+    const bool k2 = nx == 64 || nx == 128 || (f32 && nx == 512);
+    if (remaining >= 3 && k3) return 3;
+    return k2 ? 2 : 1;
+}
+
+// One launch of `levels` iterations with the shape above; the caller checks
+// fused_levels(nx, periodic, levels) == levels.
+template <class real, bool ADJ, bool FIRST>
+void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const real* p, const real* po,
+                  const real* src, real* sbar, const WaveK<real>& k) {
+    constexpr bool f32 = std::is_same_v<real, float>;
+    if (levels == 3) {
+        if (nx == 64) {
+            if constexpr (f32) launch_shape<real, 3, 1, 1, 4, 4, 2, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 3, 1, 1, 4, 4, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+        } else if (nx == 128) {
+            launch_shape<real, 3, 2, 1, 8, 8, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+        } else if (nx == 256) {
+            if constexpr (f32) launch_shape<real, 3, 4, 1, 8, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 3, 2, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+        } else if constexpr (f32) {
+            launch_shape<real, 3, 4, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+        }
+    } else {
+        if (nx == 64) launch_shape<real, 2, 1, 1, 4, 4, 2, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+        else if (nx == 128) {
+            if constexpr (f32) launch_shape<real, 2, 1, 2, 4, 8, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 2, 1, 2, 4, 16, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+        } else if constexpr (f32) {
+            launch_shape<real, 2, 4, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+        }
+    }
+}
+
+}  // namespace
+}  // namespace pm
+}  // namespace rrg
