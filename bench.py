@@ -39,8 +39,10 @@ C = 4574.337022617616
 # 12824.5 KiB fetched and 8192.0 KiB written per launch.  The summary with the
 # commands is profiles/wave_pm1e6_128_hbm_counters.txt.
 WAVE_TRAFFIC_128 = 34_647_301
-# The same for one three-iteration k_wave_fused<float, 3> launch: not yet measured.
-WAVE_FUSED_TRAFFIC_128 = None
+# The same for one three-iteration k_wave_fused<float, 3> launch, same method:
+# 17240.7 KiB fetched (x 1.9995) and 16384.0 KiB written per launch
+# (profiles/wave_fused_pm1e6_128_hbm_counters.txt).
+WAVE_FUSED_TRAFFIC_128 = 52_077_773
 # Which fused launch shapes the library has (pm_kernels.hip, wave_fused_levels):
 # (three iterations, two iterations) per (real, nx) on periodic grids.
 FUSED_SHAPES = {("fp32", 64): (True, True), ("fp32", 128): (True, True), ("fp32", 256): (True, False),
