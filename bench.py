@@ -106,16 +106,21 @@ def pm_bench(rrg, ics, steps, warmup):
     sched = wave_schedule(iters, PM["nx"])
     assert sched[3] + sched[2] + sched[1] == round(wave_launches / steps), (sched, wave_launches, steps)
     fused = sched[3] + sched[2]
-    # fused launch: read phi^n, phi^(n-1), source; write the last two levels
-    # (fp32); single: read the same three, write phi^(n+1)
-    fused_bytes, single_bytes = 20 * ncell, 16 * ncell
-    wave_bytes = fused_bytes if fused else single_bytes
     n = PM["n"]
-    # algorithmic bytes per step (fp32): wave, gradient (4 in, 16 out padded),
-    # source (8 in, 4+4 out, 8 clear), deposit (x,v,a,m in + x,v* out + 8 atomics
-    # of 8 B), gather+kick (x, v, v* in; a, v out; 8 corners of 16 B)
-    step_bytes = (fused * fused_bytes + sched[1] * single_bytes + 20 * ncell + 24 * ncell + n * (40 + 24 + 64) +
-                  n * (36 + 24 + 128))
+    # Algorithmic bytes, fp32, per unit of work of the PM step: the stencil
+    # 16 B per cell-iteration (phi^n, phi^(n-1), source read, phi^(n+1)
+    # written), gradient 16 B per cell, deposit 16 B per particle + 8 B per
+    # cell, fixed point to source 12 B per cell, gather 24 B per particle,
+    # kick/drift 60 B per particle.
+    step_bytes = iters * 16 * ncell + 16 * ncell + (16 * n + 8 * ncell) + 12 * ncell + 24 * n + 60 * n
+    # The least traffic the kernels that ran can have: a fused launch reads
+    # phi^n, phi^(n-1) and the source once and writes its last two levels
+    # (20 B per cell), a single launch 16 B per cell; the other stages as
+    # implemented (gradient 4 in, 16 out padded; source 8 in, 4+4 out, 8
+    # clear; deposit x,v,a,m in, x,v* out, 8 atomics of 8 B; gather + kick
+    # x, v, v* in, a, v out, 8 corners of 16 B).
+    own_bytes = (fused * 20 * ncell + sched[1] * 16 * ncell + 20 * ncell + 24 * ncell + n * (40 + 24 + 64) +
+                 n * (36 + 24 + 128))
     secs = t1 - t0
     # PM force vs an FP64 direct sum over all particles (softening = dx): the
     # method error of the PM model (periodic source, cell-size resolution)
@@ -128,10 +133,13 @@ def pm_bench(rrg, ics, steps, warmup):
         "ms_per_step": secs / steps * 1e3,
         "stage_ms_per_step": {k: v[0] / steps for k, v in tm.items() if v[1]},
         "wave_avg_launch_us": avg_wave_s * 1e6,
+        "wave_ms_per_step": wave_ms / steps,
         "wave_us_per_iteration": wave_ms * 1e3 / (iters * steps),
         "wave_launches_per_step": {"three_iterations": sched[3], "two_iterations": sched[2], "single": sched[1]},
-        "wave_bytes_per_launch": wave_bytes,
+        "wave_iterations_per_launch": 3 if sched[3] else (2 if sched[2] else 1),
+        "wave_fused_launch_bytes": 20 * ncell,
         "step_algorithmic_bytes": step_bytes,
+        "step_own_minimum_bytes": own_bytes,
         "wave_iterations_per_step": stats["last_wave_iterations"],
     }
 
@@ -301,7 +309,8 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2):
     over the same [-8,8]^3 box (20 setup iterations, 200 per step, fp32),
     with several iterations per launch where the library has a launch shape
     and with one per launch; time per iteration and per launch from the stage
-    events, and the launches' algorithmic bytes (fused: 20 B/cell, single:
+    events; rates of 16 B per cell-iteration (the stencil's algorithmic
+    bytes) and of the launches' least traffic (fused: 20 B/cell, single:
     16 B/cell) over the stage time."""
     pos, vel, m = ics.exp_disk(PM["n"], seed=1)
     out = {}
@@ -321,12 +330,13 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2):
             ms, launches = tm["wave"]
             sched = wave_schedule(PM["iterations"], nx, fused=fused)
             assert sum(sched.values()) * steps == launches, (nx, fused, sched, launches)
-            nbytes = ((sched[3] + sched[2]) * 20.0 + sched[1] * 16.0) * nx ** 3 * steps
-            gbs = nbytes / (ms * 1e-3) / 1e9
+            own = ((sched[3] + sched[2]) * 20.0 + sched[1] * 16.0) * nx ** 3 * steps
+            alg = 16.0 * nx ** 3 * PM["iterations"] * steps
             row["fused" if fused else "single"] = {
                 "launches_per_step": {str(k): v for k, v in sched.items() if v}, "us_per_launch": ms * 1e3 / launches,
-                "us_per_iteration": ms * 1e3 / (PM["iterations"] * steps), "achieved_gbs": gbs,
-                "frac": gbs / HBM_PEAK_GBS}
+                "us_per_iteration": ms * 1e3 / (PM["iterations"] * steps),
+                "achieved_gbs": alg / (ms * 1e-3) / 1e9, "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "own_minimum_gbs": own / (ms * 1e-3) / 1e9, "own_minimum_frac": own / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
         out[str(nx)] = row
     return out
 
@@ -521,7 +531,14 @@ def main():
     rrg.lib()
     pm = pm_bench(rrg, ics, args.steps, args.warmup)
     done("pm")
-    achieved = pm["wave_bytes_per_launch"] / (pm["wave_avg_launch_us"] * 1e-6) / 1e9
+    # the stencil's algorithmic rate: 16 B per cell-iteration over the wave
+    # stage's time (all launches of the step); per launch of the dominant
+    # kernel that is 16 B x cells x its iterations
+    cells = PM["nx"] ** 3
+    wave_alg_bytes = 16.0 * cells * PM["iterations"]
+    achieved = wave_alg_bytes / (pm["wave_ms_per_step"] * 1e-3) / 1e9
+    per_launch = pm["wave_iterations_per_launch"]
+    own_min_gbs = pm["wave_fused_launch_bytes"] / (pm["wave_avg_launch_us"] * 1e-6) / 1e9
     line = {
         "metric": METRIC,
         "value": pm["value"],
@@ -538,15 +555,22 @@ def main():
         "config": {"workload": "PM: exponential disk N=1e6 (seed 1), 128^3 grid on [-8,8]^3, CIC + 200 damped-wave "
                                "iterations + gradient + trilinear gather, dt=5e-3, fp32",
                    "n": PM["n"], "nx": PM["nx"], "wave_iterations": PM["iterations"]},
-        "roofline": {"bound": "hbm", "kernel": "k_wave_fused<float, 3>" if pm["wave_launches_per_step"]["three_iterations"]
-                     else "k_wave<float>", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (WAVE_FUSED_TRAFFIC_128 if pm["wave_launches_per_step"]["three_iterations"]
-                                 else WAVE_TRAFFIC_128) if PM["nx"] == 128 else None,
-                     "algorithmic_bytes": pm["wave_bytes_per_launch"]},
+        "roofline": {"bound": "hbm", "kernel": "k_wave_fused<float, 3>" if per_launch == 3 else "k_wave<float>",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (WAVE_FUSED_TRAFFIC_128 if per_launch == 3 else WAVE_TRAFFIC_128)
+                     if PM["nx"] == 128 else None,
+                     "algorithmic_bytes": 16 * cells * per_launch,
+                     "per_unit": "16 B per cell-iteration (phi^n, phi^(n-1), source read; phi^(n+1) written), "
+                                 f"{per_launch} iterations x {cells} cells per launch; the fused launch keeps the "
+                                 "intermediate levels on chip, so its HBM traffic (traffic) is below these bytes",
+                     "own_minimum_bytes": pm["wave_fused_launch_bytes"] if per_launch > 1 else 16 * cells,
+                     "own_minimum_frac": (own_min_gbs if per_launch > 1 else achieved) / HBM_PEAK_GBS},
         "step_roofline": {"algorithmic_bytes": pm["step_algorithmic_bytes"],
                           "achieved_gbs": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9,
-                          "frac": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                          "frac": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "own_minimum_bytes": pm["step_own_minimum_bytes"],
+                          "own_minimum_frac": pm["step_own_minimum_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9 /
+                          HBM_PEAK_GBS},
         "stage_ms_per_step": pm["stage_ms_per_step"],
         "wave_avg_launch_us": pm["wave_avg_launch_us"],
         "wave_us_per_iteration": pm["wave_us_per_iteration"],
