@@ -248,8 +248,8 @@ def ensemble_bench(rrg, ics, steps=100):
         "ms_per_step": (t1 - t0) / steps * 1e3,
         "force_kernel_ms": kern_s * 1e3,
         "fp32_vector_frac": 20 * inter / kern_s / 1e12 / FP32_PEAK_TFLOPS,
-        "launches_per_step": 3,
-        "graph_launches": stats["graph_launches"],
+        "kernel_launches_per_step": 3,
+        "graph_launches_per_step": stats["graph_launches"] / (steps + 2),
         "tuned_shape": {"ib": stats["tile_ib"], "sources_per_item": stats["slice_len"],
                         "workgroups": stats["workgroups"], "items": stats["work_items"],
                         "candidates": stats["tune_candidates"]},

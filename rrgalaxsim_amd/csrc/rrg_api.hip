@@ -386,6 +386,20 @@ public:
 
     void step(int nsteps) override {  // rrgsim_integration_nbody.cuh:50-56
         if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_step before rrg_precalc");
+        // Steps with no host work between them (no stage timing, no adjoint
+        // recording) go kStepsPerGraph to a graph launch: at small N a step's
+        // kernels take less time than launching them.
+        if (prm_.use_graphs && !timer.on() && !recording_) {
+            const double dt = prm_.sim.dt_dynamics;
+            for (; nsteps >= kStepsPerGraph; nsteps -= kStepsPerGraph) {
+                graphs_.run(s_, 4, [&] {
+                    for (int q = 0; q < kStepsPerGraph; ++q) enqueue_kernels(dt);
+                });
+                steps_ += kStepsPerGraph;
+                for (int q = 0; q < kStepsPerGraph; ++q) time_ += dt;
+                grav_actual_ = false;
+            }
+        }
         for (int k = 0; k < nsteps; ++k) {
             enqueue_step(prm_.use_graphs != 0);
             ++steps_;
@@ -729,14 +743,34 @@ private:
         timer.stop(s_, t, 1);
     }
 
+    static constexpr int kStepsPerGraph = 8;
+
+    void predict_kernel(double dt) { nbody::predict<double>(s_, P_.view(), dt, A_.p, B_.p); }
+    void correct_kernel(double dt) {
+        if (ensemble_) nbody::correct_reduce(s_, P_.view(), dt, 0, ns_.p, part_.p);
+        else nbody::correct_reduce(s_, P_.view(), dt, plan_.slices, nullptr, plan_.slices > 1 ? part_.p : nullptr);
+    }
+    // The kernels of one step: predictor, force, corrector.
+    void enqueue_kernels(double dt) {
+        predict_kernel(dt);
+        accel(false);
+        correct_kernel(dt);
+    }
+
     void enqueue_step(bool graph) {
         const double dt = prm_.sim.dt_dynamics;
-        segment(RRG_STAGE_PREDICT, graph, 0, [&] { nbody::predict<double>(s_, P_.view(), dt, A_.p, B_.p); });
-        segment(RRG_STAGE_DIRECT, graph, 1, [&] { accel(false); });
-        segment(RRG_STAGE_CORRECT, graph, 2, [&] {
-            if (ensemble_) nbody::correct_reduce(s_, P_.view(), dt, 0, ns_.p, part_.p);
-            else nbody::correct_reduce(s_, P_.view(), dt, plan_.slices, nullptr, plan_.slices > 1 ? part_.p : nullptr);
-        });
+        auto predict = [&] { predict_kernel(dt); };
+        auto force = [&] { accel(false); };
+        auto correct = [&] { correct_kernel(dt); };
+        if (graph && !timer.on()) {
+            // without stage timing the whole step is one graph launch (small N
+            // is bound by the host's launch rate)
+            graphs_.run(s_, 3, [&] { enqueue_kernels(dt); });
+            return;
+        }
+        segment(RRG_STAGE_PREDICT, graph, 0, predict);
+        segment(RRG_STAGE_DIRECT, graph, 1, force);
+        segment(RRG_STAGE_CORRECT, graph, 2, correct);
     }
 
     void accel(bool reduce_now) {
@@ -1105,6 +1139,23 @@ private:
     // segment launches.
     void enqueue_step(real dt, bool graph, int fixed_iters = -1) {
         const uint64_t b0 = graph_base();
+        if (graph && !timer.on()) {
+            // without stage timing the whole step is one graph launch
+            const int ip0 = ip_, ic0 = ic_;
+            const int iters = prm_.wave.iterations;
+            rotate(iters);
+            graphs_.run(s_, b0 + 5, [&] {
+                pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p);
+                pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f,
+                                          mass_.p, src_.p, layered());
+                int ip = ip0, ic = ic0;  // from the arrangement before the rotation
+                wave_launches(iters, ip, ic);
+                pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
+                pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true);
+            });
+            last_iters_ = iters;
+            return;
+        }
         segment(RRG_STAGE_DEPOSIT, 1, graph, b0 + 0, [&] {
             pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p);
         });

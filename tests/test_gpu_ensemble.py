@@ -89,7 +89,10 @@ def test_autotuned_shape_gives_same_physics(rrg, force):
 
 @pytest.mark.parametrize("ensemble", [False, True])
 @pytest.mark.parametrize("force", ["fp64", "mixed"])
-def test_graph_replay_bitwise_equals_eager(rrg, ensemble, force):
+@pytest.mark.parametrize("timing", [False, True])
+def test_graph_replay_bitwise_equals_eager(rrg, ensemble, force, timing):
+    """Steps replayed from graphs equal eager steps: one graph per step
+    without stage timing, one per stage (3) with it."""
     pos, vel, m, off = _ensemble(SIZES if ensemble else [4096])
     s2 = np.full(len(m), 1e-4)
     states = {}
@@ -97,34 +100,61 @@ def test_graph_replay_bitwise_equals_eager(rrg, ensemble, force):
         p = rrg.make_params(dt=1e-3, force=force, graphs=graphs)
         with rrg.Simulation(p, pos, vel, m, s2, offsets=off if ensemble else None) as sim:
             sim.precalc()
+            sim.set_timing(timing)
             seq = []
             for _ in range(4):
                 sim.calc_step(2)
                 sim.conservation()  # eager work and a synchronisation between replays
                 seq.append(sim.state())
             states[graphs] = (seq, sim.stats()["graph_launches"])
-    assert states[False][1] == 0 and states[True][1] == 24  # 3 segment graphs per step
+    assert states[False][1] == 0 and states[True][1] == (24 if timing else 8)
     for a, b in zip(states[False][0], states[True][0]):
         for k in ("pos", "vel", "acc"):
             assert np.array_equal(a[k], b[k]), k
 
 
-def test_pm_graph_replay_bitwise_equals_eager(rrg):
+@pytest.mark.parametrize("ensemble", [False, True])
+def test_multi_step_graphs_bitwise_equal_eager(rrg, ensemble):
+    """Runs of 19 steps without timing or recording: two graphs of 8 steps
+    and three one-step graphs, bitwise the eager steps."""
+    pos, vel, m, off = _ensemble(SIZES if ensemble else [2048])
+    s2 = np.full(len(m), 1e-4)
+    out = {}
+    for graphs in (False, True):
+        p = rrg.make_params(dt=1e-3, force="mixed", graphs=graphs)
+        with rrg.Simulation(p, pos, vel, m, s2, offsets=off if ensemble else None) as sim:
+            sim.precalc()
+            sim.calc_step(19)
+            st = sim.state()
+            out[graphs] = (st, sim.stats()["graph_launches"], sim.stats()["steps"], sim.stats()["time"])
+    assert out[True][1] == 5 and out[False][1] == 0
+    assert out[True][2] == out[False][2] == 19 and out[True][3] == out[False][3]
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(out[False][0][k], out[True][0][k]), k
+
+
+@pytest.mark.parametrize("nx", [32, 64])
+@pytest.mark.parametrize("timing", [False, True])
+def test_pm_graph_replay_bitwise_equals_eager(rrg, nx, timing):
+    """PM steps from graphs equal eager steps (64^3: fused stencil launches
+    and four potential grids); one graph per step without stage timing, one
+    per stage (5) with it."""
     pos, vel, m = ics.exp_disk(20_000, seed=1)
     wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=4574.337022617616, setup_iterations=30,
                 iterations=7, tolerance=0.0)
     res = {}
     for graphs in (False, True):
-        p = rrg.make_params(dt=5e-3, grid=dict(nx=32, dx=0.5), wave=wave, real="fp32", graphs=graphs)
+        p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real="fp32", graphs=graphs)
         with rrg.Simulation(p, pos, vel, m) as sim:
             sim.precalc()
+            sim.set_timing(timing)
             seq = []
             for _ in range(6):  # crosses a re-sort and all three buffer rotations
                 sim.calc_step(3)
                 sim.conservation()
                 seq.append((sim.state(), sim.grid()["phi"]))
             res[graphs] = (seq, sim.stats()["graph_launches"])
-    assert res[True][1] == 90  # 5 segment graphs per step
+    assert res[True][1] == (90 if timing else 18)
     for (a, pa), (b, pb) in zip(res[False][0], res[True][0]):
         assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
         assert np.array_equal(pa, pb)
