@@ -45,9 +45,9 @@ WAVE_TRAFFIC_128 = 34_647_301
 WAVE_FUSED_TRAFFIC_128 = 52_077_773
 # Which fused launch shapes the library has (pm_kernels.hip, wave_fused_levels):
 # (three iterations, two iterations) per (real, nx) on periodic grids.
-FUSED_SHAPES = {("fp32", 64): (True, True), ("fp32", 128): (True, True), ("fp32", 256): (True, False),
+FUSED_SHAPES = {("fp32", 64): (True, True), ("fp32", 128): (True, True), ("fp32", 256): (True, True),
                 ("fp32", 512): (True, True), ("fp64", 64): (True, True), ("fp64", 128): (True, True),
-                ("fp64", 256): (True, False)}
+                ("fp64", 256): (True, True), ("fp64", 512): (False, True)}
 
 
 def wave_schedule(iters, nx, real="fp32", fused=True):

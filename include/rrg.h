@@ -131,8 +131,8 @@ typedef struct {
     int32_t autotune;       /* 1: time candidate launch shapes at precalc and keep
                                the fastest (cached per process); default 0 */
     int32_t wave_fused;     /* 1 (default): up to three stencil iterations per
-                               launch on periodic 64^3, 128^3, 256^3 grids (and
-                               FP32 512^3), bitwise the same results; 0: one */
+                               launch on periodic 64^3-512^3 grids (two for FP64
+                               512^3), bitwise the same results; 0: one */
 } rrg_params;
 
 typedef struct {

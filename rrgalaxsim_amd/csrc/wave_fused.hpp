@@ -51,6 +51,7 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     constexpr int ROWS = TY + 2 * K - 2;
     constexpr int S = ZB + 2 * K - 2;  // level-1 planes Z0-(K-1) .. Z0+ZB+K-2
     static_assert(K >= 2 && (NX & (NX - 1)) == 0 && NX % TY == 0 && NX % ZB == 0, "shape");
+    static_assert(RS * ROWS <= 16, "a workgroup holds at most 16 waves");
     __shared__ real lq[K - 1][2][ROWS][NX];
     int b = blockIdx.x;
     if constexpr ((NB & 7) == 0) b = (b & 7) * (NB >> 3) + (b >> 3);  // one z slab per XCD
@@ -210,22 +211,23 @@ void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* 
 }
 
 // Launch shapes (K, XJ, RS, TY, ZB, PD), from sweeps of graph-replayed
-// 240-iteration chains of the forward kernel on MI355X (time per iteration,
-// FP32 / FP64; one iteration per launch in brackets):
-//   64^3   K=3 (1,1,4,4,2) 1.51 us (2.74) / K=3 (1,1,4,4,1) 2.28 us (3.32)
-//   128^3  K=3 (2,1,8,8,1) 3.51-3.61 us (6.8) / 7.44 us (12.0)
-//   256^3  K=3 (4,1,8,32,1) 26.7 us (38.0) / K=3 (2,2,4,32,1) 71.6 us (115.6)
-//   512^3  K=3 (4,2,4,32,1) 222.8 us (440) / none: FP64 shapes spill registers
-// and for a remainder of two iterations K=2 where it beat two single
-// launches (64^3, 128^3, FP32 512^3).  The adjoint uses the same shapes.
-// Other grids, and non-periodic ones, run one iteration per launch.
+// 240-iteration chains of the forward kernel on MI355X (time per iteration;
+// one iteration per launch in brackets):
+//          FP32                                FP64
+//   64^3   K=3 (1,1,4,4,2)  1.51 us (2.74)    K=3 (1,1,4,4,1)  2.28 us (3.32)
+//   128^3  K=3 (2,1,8,8,1)  3.51-3.61 (6.8)   K=3 (2,1,8,8,1)  7.26-7.44 (12.0)
+//   256^3  K=3 (4,1,8,32,1) 26.7 (38.0)       K=3 (2,2,4,64,1) 63.3 (115.6)
+//   512^3  K=3 (4,2,4,32,1) 222.8 (440)       K=2 (2,4,2,32,1) 802 (932); K=3
+//                                             shapes spill registers
+// and for a remainder of two iterations the K=2 shapes below.  The adjoint
+// uses the same shapes.  Other grids, and non-periodic ones, run one
+// iteration per launch.
 template <class real> int fused_levels(int nx, int bc, int remaining) {
     constexpr bool f32 = std::is_same_v<real, float>;
     if (bc != RRG_BC_PERIODIC || remaining < 2) return 1;
-    const bool k3 = nx == 64 || nx == 128 || nx == 256 || (f32 && nx == 512);
-    const bool k2 = nx == 64 || nx == 128 || (f32 && nx == 512);
-    if (remaining >= 3 && k3) return 3;
-    return k2 ? 2 : 1;
+    if (nx != 64 && nx != 128 && nx != 256 && nx != 512) return 1;
+    const bool k3 = nx != 512 || f32;
+    return remaining >= 3 && k3 ? 3 : 2;
 }
 
 // One launch of `levels` iterations with the shape above; the caller checks
@@ -242,17 +244,22 @@ void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const r
             launch_shape<real, 3, 2, 1, 8, 8, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 256) {
             if constexpr (f32) launch_shape<real, 3, 4, 1, 8, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 3, 2, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 3, 2, 2, 4, 64, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if constexpr (f32) {
             launch_shape<real, 3, 4, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         }
     } else {
-        if (nx == 64) launch_shape<real, 2, 1, 1, 4, 4, 2, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-        else if (nx == 128) {
+        if (nx == 64) {
+            launch_shape<real, 2, 1, 1, 4, 4, 2, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+        } else if (nx == 128) {
             if constexpr (f32) launch_shape<real, 2, 1, 2, 4, 8, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
             else launch_shape<real, 2, 1, 2, 4, 16, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-        } else if constexpr (f32) {
-            launch_shape<real, 2, 4, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+        } else if (nx == 256) {
+            if constexpr (f32) launch_shape<real, 2, 4, 1, 8, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 2, 2, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+        } else {
+            if constexpr (f32) launch_shape<real, 2, 4, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 2, 2, 4, 2, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         }
     }
 }

@@ -1,5 +1,5 @@
 """Several stencil iterations per launch (three, or two for a remainder of
-two; periodic 64^3, 128^3, 256^3, FP32 512^3): bitwise the same as the
+two; periodic 64^3 - 512^3, two at most for FP64 512^3): bitwise the same as the
 oracle's sequence of wave_diss_iteration calls (reference wave.cu:106-193,
 solve_wave_equation wave.cu:230-297) for iteration counts with every
 remainder, inside full PM runs (graph-replayed steps, the steady-state check
@@ -39,14 +39,17 @@ def test_wave_fused_iterations_bitwise(rrg, orc, dtype, nx, iters):
     assert np.array_equal(gp, rp) and np.array_equal(gc, rc)
 
 
-def test_wave_fused_512_fp32_bitwise(rrg, orc):
-    nx, iters = 512, 5  # one three-iteration and one two-iteration launch
+@pytest.mark.parametrize("dtype,iters", [(np.float32, 5), (np.float64, 3)])
+def test_wave_fused_512_bitwise(rrg, orc, dtype, iters):
+    """FP32: one three- and one two-iteration launch; FP64: one two-iteration
+    launch and a single one."""
+    nx = 512
     dx = 16.0 / nx
-    prev, curr, mass = _fields(nx, np.float32, 5)
-    wdt = orc.wave_dt(dx, C, dtype=np.float32)
-    rm = orc.rho_mean(1.0, nx, dx, dtype=np.float32)
-    rp, rc, _ = orc.solve_wave(nx, dx, 0.785, C, wdt, True, rm, prev, curr, mass, iters, 0.0, dtype=np.float32)
-    gp, gc = rrg.wave_iterate(nx, dx, 0.785, C, wdt, rm, iters, prev, curr, mass, dtype=np.float32)
+    prev, curr, mass = _fields(nx, dtype, 5)
+    wdt = orc.wave_dt(dx, C, dtype=dtype)
+    rm = orc.rho_mean(1.0, nx, dx, dtype=dtype)
+    rp, rc, _ = orc.solve_wave(nx, dx, 0.785, C, wdt, True, rm, prev, curr, mass, iters, 0.0, dtype=dtype)
+    gp, gc = rrg.wave_iterate(nx, dx, 0.785, C, wdt, rm, iters, prev, curr, mass, dtype=dtype)
     assert np.array_equal(gp, rp) and np.array_equal(gc, rc)
 
 
@@ -93,8 +96,7 @@ def test_pm_steady_check_fused_matches_oracle(rrg, orc):
 @pytest.mark.parametrize("nx,iters", [(128, 23), (256, 5)])
 def test_fused_equals_single(rrg, graphs, nx, iters):
     """FP32 PM runs on the headline grid (23 = 7 x 3 + 2 iterations per step)
-    and on 256^3 (5 = 3 + 1 + 1): fused launches on and off give the same
-    bits."""
+    and on 256^3 (5 = 3 + 2): fused launches on and off give the same bits."""
     out = []
     for fused in (True, False):
         _, st, gr, _ = _run(rrg, np.float32, nx=nx, iters=iters, setup=31, steps=2, fused=fused, graphs=graphs)
