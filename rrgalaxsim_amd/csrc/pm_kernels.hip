@@ -80,9 +80,33 @@ __device__ __forceinline__ unsigned cic_fixed(const real r[3], real m, const Geo
 // Predictor fused with the deposit.  Lanes of a wave whose particles share a
 // base cell (runs of equal keys; the particle arrays are kept sorted by cell,
 // so dense regions form long runs) are merged by a segmented inclusive scan
-// over the wave before one lane of each run issues the 8 atomics.  Integer
-// addition is exact, so the grid does not depend on the merging.
+// over the wave before the last lane of each run issues the 8 atomics.
+// Integer addition is exact, so the grid does not depend on the merging.
 constexpr unsigned kNoCell = 0xffffffffu;
+
+// Lane values moved by a DPP pattern (row_shr:n = 0x110+n within rows of 16
+// lanes, row_bcast:15 = 0x142, row_bcast:31 = 0x143); lanes without a source
+// or outside ROWS read 0.
+template <int CTRL, int ROWS> __device__ __forceinline__ int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS> __device__ __forceinline__ long long dpp_i64(long long v) {
+    const unsigned long long u = (unsigned long long)v;
+    const unsigned lo = (unsigned)dpp_i32<CTRL, ROWS>((int)(unsigned)u);
+    const unsigned hi = (unsigned)dpp_i32<CTRL, ROWS>((int)(unsigned)(u >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+// One step of the segmented scan on (q, f), f = "a run head lies in my
+// window": the window to the left joins unless a head is already in mine.
+template <int CTRL, int ROWS> __device__ __forceinline__ void seg_step(long long (&q)[8], int& f) {
+    const int fs = dpp_i32<CTRL, ROWS>(f);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const long long s = dpp_i64<CTRL, ROWS>(q[k]);
+        if (!f) q[k] += s;
+    }
+    f |= fs;
+}
 
 template <class real>
 __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt, int do_predict, Geo<real> g,
@@ -111,8 +135,9 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
     }
     // Bitonic sort of the wave's (key, lane) pairs with shuffles, so equal
     // cells become adjacent lanes even where the particle order has drifted
-    // since the last global sort; each lane then takes the contributions of
-    // the particle it was sorted to.
+    // since the last global sort (without it the deposit takes ~3x longer
+    // on the headline disk); each lane then takes the contributions of the
+    // particle it was sorted to.
     {
         unsigned sk = key;
         int src = lane;
@@ -142,20 +167,13 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
     const unsigned knext = __shfl_down(key, 1, 64);
     int f = (lane == 0 || kprev != key) ? 1 : 0;  // run head
     const bool tail = (lane == 63 || knext != key);
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int fu = __shfl_up(f, off, 64);
-        long long qu[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) qu[k] = __shfl_up(q[k], off, 64);
-        if (lane >= off) {
-            if (!f) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) q[k] += qu[k];
-            }
-            f |= fu;
-        }
-    }
+    // inclusive segmented scan: within rows of 16 lanes, then across rows
+    seg_step<0x111, 0xf>(q, f);
+    seg_step<0x112, 0xf>(q, f);
+    seg_step<0x114, 0xf>(q, f);
+    seg_step<0x118, 0xf>(q, f);
+    seg_step<0x142, 0xa>(q, f);
+    seg_step<0x143, 0xc>(q, f);
     if (tail && key != kNoCell) {
 #pragma unroll
         for (int k = 0; k < 8; ++k)
