@@ -1120,7 +1120,7 @@ public:
     }
 
 private:
-    static constexpr int kSortInterval = 8;  // steps between spatial re-sorts
+    static constexpr int kSortInterval = 16;  // steps between spatial re-sorts
 
     template <class F> void segment(int stage, int64_t launches, bool graph, uint64_t key, F&& f) {
         const int t = timer.start(s_, stage);
