@@ -1,7 +1,8 @@
 // Several damped-wave iterations per launch on periodic grids: the forward
 // stencil of wave_diss_iteration (solver/lib/rrgsim_wave/src/wave.cu:106-193)
 // and its transpose (the potential adjoint), shared by pm_kernels.hip and
-// pm_adjoint_kernels.hip.  Included inside namespace rrg::pm of those files.
+// pm_adjoint_kernels.hip (each translation unit gets its own copy in an
+// unnamed namespace).
 #pragma once
 
 #include <type_traits>
