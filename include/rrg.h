@@ -239,6 +239,15 @@ int rrg_sync(rrg_ctx* ctx);
 int rrg_get_state(rrg_ctx* ctx, double* pos, double* vel, double* acc, double* grav);
 /* Replace positions and velocities (n,3); acceleration is left as is. */
 int rrg_set_state(rrg_ctx* ctx, const double* pos, const double* vel);
+/* rrg_get_state / rrg_set_state with device arrays (FP64 (n,3) rows in the
+ * caller's particle order, device memory of the context's device) instead of
+ * host buffers: the transfers stay on the device, ordered after the work
+ * queued on `stream` (a hipStream_t, NULL = the null stream), and `stream`
+ * waits for them before its later work.  They replace direct access to the
+ * device arrays of ParticlesContext_ (co_particles_context.cuh:17-31) by code
+ * that shares the GPU, such as a torch caller.  Host pointers are rejected. */
+int rrg_get_state_dev(rrg_ctx* ctx, double* pos, double* vel, double* acc, void* stream);
+int rrg_set_state_dev(rrg_ctx* ctx, const double* pos, const double* vel, void* stream);
 /* Energy, momentum and angular momentum by FP64 device reduction, with the
  * definitions of calc_conservation (nbody.cu:246-291). */
 int rrg_get_conservation(rrg_ctx* ctx, rrg_conservation* out);
@@ -260,19 +269,25 @@ int rrg_set_timing(rrg_ctx* ctx, int32_t enabled);
 /* Synchronize the context stream and return the accumulated totals. */
 int rrg_get_timing(rrg_ctx* ctx, rrg_timing* out);
 
-/* ---- adjoint of the direct-sum run (no counterpart in the reference) ----
+/* ---- adjoints of the direct-sum and PM runs (no counterpart in the
+ * reference) ----
  * rrg_adjoint_begin marks the current state (after rrg_precalc) as step 0 of
- * a recorded run; the following rrg_step calls store (x, v, a) checkpoints
- * every `checkpoint_every` steps on the device.  rrg_adjoint then returns the
- * gradients of a loss L(x_T, v_T) of the state after the recorded steps with
- * respect to the initial positions, velocities and the masses, given
- * dL/dx_T and dL/dv_T (either may be NULL = zero).  It recomputes each
- * checkpoint segment forward and runs the transposed (reverse) Verlet step
- * with one vector-Jacobian product of the pair forces per step.  Any output
- * pointer may be NULL.  Single-system direct-sum contexts only. */
+ * a recorded run; the following rrg_step calls store checkpoints every
+ * `checkpoint_every` steps on the device ((x, v, a), and for PM also the two
+ * potential levels).  rrg_adjoint then returns the gradients of a loss
+ * L(x_T, v_T) of the state after the recorded steps with respect to the
+ * initial positions, velocities and the masses, given dL/dx_T and dL/dv_T
+ * (either may be NULL = zero).  It recomputes each checkpoint segment forward
+ * and runs the transposed step (direct sum: reverse Verlet with one
+ * vector-Jacobian product of the pair forces; PM: the reversed grid build,
+ * stencil and CIC).  Any output pointer may be NULL.  Single-system contexts;
+ * PM on periodic and zero-Dirichlet grids. */
 int rrg_adjoint_begin(rrg_ctx* ctx, int32_t checkpoint_every);
 int rrg_adjoint(rrg_ctx* ctx, const double* gx_T, const double* gv_T, double* gx_0, double* gv_0,
                 double* gm);
+/* rrg_adjoint with device arrays, ordered on `stream` as rrg_get_state_dev. */
+int rrg_adjoint_dev(rrg_ctx* ctx, const double* gx_T, const double* gv_T, double* gx_0, double* gv_0, double* gm,
+                    void* stream);
 
 /* ---- stateless operators (one call = one kernel family on device 0) ----
  * Arrays of `real` are double for RRG_REAL_F64 and float for RRG_REAL_F32.

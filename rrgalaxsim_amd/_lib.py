@@ -108,6 +108,9 @@ SIGNATURES = {
     "rrg_sync": (ctypes.c_int, [_vp]),
     "rrg_get_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "rrg_set_state": (ctypes.c_int, [_vp, _vp, _vp]),
+    "rrg_get_state_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "rrg_set_state_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "rrg_adjoint_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rrg_get_conservation": (ctypes.c_int, [_vp, ctypes.POINTER(Conservation)]),
     "rrg_get_grid": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "rrg_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),

@@ -259,12 +259,18 @@ struct Solver {
     virtual void sync() = 0;
     virtual void get_state(double* pos, double* vel, double* acc, double* grav) = 0;
     virtual void set_state(const double* pos, const double* vel) = 0;
+    // The same with device arrays, enqueued on the context stream.
+    virtual void get_state_dev(double* pos, double* vel, double* acc) = 0;
+    virtual void set_state_dev(const double* pos, const double* vel) = 0;
     virtual void conservation(rrg_conservation* out) = 0;
     virtual void conservation_systems(int k, rrg_conservation* out) = 0;
     virtual void get_grid(double* mass, double* phi, double* acc) = 0;
     virtual void stats(rrg_stats* out) = 0;
     virtual void adjoint_begin(int) { fail(RRG_ERR_STATE, "adjoint: supported by direct-sum contexts only"); }
     virtual void adjoint(const double*, const double*, double*, double*, double*) {
+        fail(RRG_ERR_STATE, "adjoint: supported by direct-sum contexts only");
+    }
+    virtual void adjoint_dev(const double*, const double*, double*, double*, double*) {
         fail(RRG_ERR_STATE, "adjoint: supported by direct-sum contexts only");
     }
 };
@@ -423,16 +429,31 @@ public:
         store_checkpoint();
     }
 
-    // Reverse sweep over the recorded steps (see adjoint_kernels.hip).
     void adjoint(const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) override {
+        adjoint_impl(gxT, gvT, gx0, gv0, gm, false);
+    }
+    void adjoint_dev(const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) override {
+        adjoint_impl(gxT, gvT, gx0, gv0, gm, true);
+    }
+
+    // Reverse sweep over the recorded steps (see adjoint_kernels.hip); `dev`:
+    // the seeds and gradients are device arrays.
+    void adjoint_impl(const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm, bool dev) {
         if (!recording_) fail(RRG_ERR_STATE, "rrg_adjoint without rrg_adjoint_begin");
         const int64_t n = P_.n;
         const double dt = prm_.sim.dt_dynamics;
         const bool f32 = nbody::is_fp32_kind(kind_);
         adj_alloc();
-        std::vector<double> zero(3 * n, 0.0);
-        upload_vec<double>(s_, n, gxT ? gxT : zero.data(), Xb_.p);
-        upload_vec<double>(s_, n, gvT ? gvT : zero.data(), Vb_.p);
+        if (dev) {
+            if (gxT) layout::rows_in<double>(s_, n, gxT, Xb_.p, nullptr);
+            else zero_async(s_, Xb_.p, Xb_.bytes());
+            if (gvT) layout::rows_in<double>(s_, n, gvT, Vb_.p, nullptr);
+            else zero_async(s_, Vb_.p, Vb_.bytes());
+        } else {
+            std::vector<double> zero(3 * n, 0.0);
+            upload_vec<double>(s_, n, gxT ? gxT : zero.data(), Xb_.p);
+            upload_vec<double>(s_, n, gvT ? gvT : zero.data(), Vb_.p);
+        }
         zero_async(s_, Ab_.p, Ab_.bytes());
         zero_async(s_, gm_.p, gm_.bytes());
         const int T = int(rec_steps_);
@@ -460,6 +481,12 @@ public:
         // a_0 = F(x_0): the closing product at the initial positions
         reverse_step(ck_[0].p, 0.0, true);
         if (f32) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
+        if (dev) {
+            if (gx0) layout::rows_out<double>(s_, n, Xb_.p, gx0, nullptr);
+            if (gv0) layout::rows_out<double>(s_, n, Vb_.p, gv0, nullptr);
+            if (gm) layout::scalar_out<double>(s_, n, gm_.p, gm, nullptr);
+            return;
+        }
         if (gx0) download_vec<double>(s_, n, Xb_.p, gx0);
         if (gv0) download_vec<double>(s_, n, Vb_.p, gv0);
         if (gm) download_scalar<double>(s_, n, gm_.p, gm);
@@ -485,7 +512,24 @@ public:
     void set_state(const double* pos, const double* vel) override {
         if (pos) upload_vec<double>(s_, P_.n, pos, P_.X.p);
         if (vel) upload_vec<double>(s_, P_.n, vel, P_.V.p);
-        if (pos && nbody::is_fp32_kind(kind_)) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
+        state_changed(pos != nullptr);
+    }
+
+    void get_state_dev(double* pos, double* vel, double* acc) override {
+        const int64_t n = P_.n;
+        if (pos) layout::rows_out<double>(s_, n, P_.X.p, pos, nullptr);
+        if (vel) layout::rows_out<double>(s_, n, P_.V.p, vel, nullptr);
+        if (acc) layout::rows_out<double>(s_, n, P_.A.p, acc, nullptr);
+    }
+
+    void set_state_dev(const double* pos, const double* vel) override {
+        if (pos) layout::rows_in<double>(s_, P_.n, pos, P_.X.p, nullptr);
+        if (vel) layout::rows_in<double>(s_, P_.n, vel, P_.V.p, nullptr);
+        state_changed(pos != nullptr);
+    }
+
+    void state_changed(bool moved) {
+        if (moved && nbody::is_fp32_kind(kind_)) nbody::pack_mixed(s_, P_.view(), A_.p, B_.p);
         grav_actual_ = false;
         recording_ = false;  // a recorded run no longer describes this state
     }
@@ -1004,11 +1048,24 @@ public:
     // then reversed; the precalc build closes the sweep.  The live state is
     // restored afterwards.
     void adjoint(const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) override {
+        adjoint_impl(gxT, gvT, gx0, gv0, gm, false);
+    }
+    void adjoint_dev(const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) override {
+        adjoint_impl(gxT, gvT, gx0, gv0, gm, true);
+    }
+
+    void adjoint_impl(const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm, bool dev) {
         if (!recording_) fail(RRG_ERR_STATE, "rrg_adjoint without rrg_adjoint_begin");
         const int64_t n = P_.n;
         adj_alloc();
-        const std::vector<int> id = ids();
-        {
+        std::vector<int> id;
+        if (dev) {
+            if (gxT) layout::rows_in<double>(s_, n, gxT, Xb_.p, id_.p);
+            else zero_async(s_, Xb_.p, Xb_.bytes());
+            if (gvT) layout::rows_in<double>(s_, n, gvT, Vb_.p, id_.p);
+            else zero_async(s_, Vb_.p, Vb_.bytes());
+        } else {
+            id = ids();
             std::vector<double> tmp(3 * n, 0.0), zero(3 * n, 0.0);
             if (gxT) permute_vec_in(n, id, gxT, tmp.data());
             upload_vec<double>(s_, n, gxT ? tmp.data() : zero.data(), Xb_.p);
@@ -1045,6 +1102,12 @@ public:
         ic_ = ic0;
         load_checkpoint(backup_);
         last_iters_ = it0;
+        if (dev) {
+            if (gx0) layout::rows_out<double>(s_, n, Xb_.p, gx0, id_.p);
+            if (gv0) layout::rows_out<double>(s_, n, Vb_.p, gv0, id_.p);
+            if (gm) layout::scalar_out<double>(s_, n, gm_.p, gm, id_.p);
+            return;
+        }
         std::vector<double> tmp(3 * n);
         if (gx0) { download_vec<double>(s_, n, Xb_.p, tmp.data()); unpermute_vec(n, id, tmp.data(), gx0); }
         if (gv0) { download_vec<double>(s_, n, Vb_.p, tmp.data()); unpermute_vec(n, id, tmp.data(), gv0); }
@@ -1082,6 +1145,21 @@ public:
         std::vector<double> tmp(3 * n);
         if (pos) { permute_vec_in(n, id, pos, tmp.data()); upload_vec<real>(s_, n, tmp.data(), P_.X.p); }
         if (vel) { permute_vec_in(n, id, vel, tmp.data()); upload_vec<real>(s_, n, tmp.data(), P_.V.p); }
+        grav_actual_ = acc_actual_ = false;
+        fresh_ = recording_ = false;
+    }
+
+    void get_state_dev(double* pos, double* vel, double* acc) override {
+        const int64_t n = P_.n;
+        if (pos) layout::rows_out<real>(s_, n, P_.X.p, pos, id_.p);
+        if (vel) layout::rows_out<real>(s_, n, P_.V.p, vel, id_.p);
+        if (acc) layout::rows_out<real>(s_, n, P_.A.p, acc, id_.p);
+    }
+
+    void set_state_dev(const double* pos, const double* vel) override {
+        const int64_t n = P_.n;
+        if (pos) layout::rows_in<real>(s_, n, pos, P_.X.p, id_.p);
+        if (vel) layout::rows_in<real>(s_, n, vel, P_.V.p, id_.p);
         grav_actual_ = acc_actual_ = false;
         fresh_ = recording_ = false;
     }
@@ -1636,6 +1714,75 @@ RRG_API int rrg_get_state(rrg_ctx* ctx, double* pos, double* vel, double* acc, d
 
 RRG_API int rrg_set_state(rrg_ctx* ctx, const double* pos, const double* vel) {
     RRG_CTX_CALL(ctx, ctx->solver->set_state(pos, vel));
+}
+
+namespace {
+
+// Device arrays of the caller: each must be device memory of the context's
+// device; the work is ordered after `stream`'s queued work and `stream` waits
+// for it before its own later work.
+void require_device_array(const void* p, int device, const char* what) {
+    if (!p) return;
+    hipPointerAttribute_t a{};
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(RRG_ERR_INVALID, std::string(what) + ": not device memory");
+    }
+    if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged)
+        fail(RRG_ERR_INVALID, std::string(what) + ": not device memory");
+    if (a.device != device) fail(RRG_ERR_INVALID, std::string(what) + ": memory of another device");
+}
+
+class StreamJoin {
+public:
+    StreamJoin(hipStream_t caller, hipStream_t own) : caller_(caller), own_(own) {
+        RRG_HIP(hipEventCreateWithFlags(&e_, hipEventDisableTiming));
+        RRG_HIP(hipEventRecord(e_, caller_));
+        RRG_HIP(hipStreamWaitEvent(own_, e_, 0));
+    }
+    ~StreamJoin() { (void)hipEventDestroy(e_); }
+    void done() {
+        RRG_HIP(hipEventRecord(e_, own_));
+        RRG_HIP(hipStreamWaitEvent(caller_, e_, 0));
+    }
+
+private:
+    hipStream_t caller_, own_;
+    hipEvent_t e_ = nullptr;
+};
+
+}  // namespace
+
+RRG_API int rrg_get_state_dev(rrg_ctx* ctx, double* pos, double* vel, double* acc, void* stream) {
+    RRG_CTX_CALL(ctx, {
+        for (const void* q : {(const void*)pos, (const void*)vel, (const void*)acc})
+            require_device_array(q, ctx->device, "rrg_get_state_dev");
+        StreamJoin j(static_cast<hipStream_t>(stream), ctx->stream);
+        ctx->solver->get_state_dev(pos, vel, acc);
+        j.done();
+    });
+}
+
+RRG_API int rrg_set_state_dev(rrg_ctx* ctx, const double* pos, const double* vel, void* stream) {
+    RRG_CTX_CALL(ctx, {
+        require_device_array(pos, ctx->device, "rrg_set_state_dev");
+        require_device_array(vel, ctx->device, "rrg_set_state_dev");
+        StreamJoin j(static_cast<hipStream_t>(stream), ctx->stream);
+        ctx->solver->set_state_dev(pos, vel);
+        j.done();
+    });
+}
+
+RRG_API int rrg_adjoint_dev(rrg_ctx* ctx, const double* gx_T, const double* gv_T, double* gx_0, double* gv_0,
+                            double* gm, void* stream) {
+    RRG_CTX_CALL(ctx, {
+        for (const void* q : {(const void*)gx_T, (const void*)gv_T, (const void*)gx_0, (const void*)gv_0, (const void*)gm})
+            require_device_array(q, ctx->device, "rrg_adjoint_dev");
+        StreamJoin j(static_cast<hipStream_t>(stream), ctx->stream);
+        ctx->solver->adjoint_dev(gx_T, gv_T, gx_0, gv_0, gm);
+        j.done();
+    });
 }
 
 RRG_API int rrg_get_conservation(rrg_ctx* ctx, rrg_conservation* out) {

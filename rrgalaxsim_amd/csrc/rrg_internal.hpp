@@ -166,6 +166,14 @@ void pack_pos(hipStream_t s, int64_t n, const double* X, const double* M, const 
 }  // namespace adjoint
 
 // ------------------------------------------------------------ particle-mesh
+namespace layout {
+// Caller rows (n,3) FP64, in caller order, to and from the state's SoA
+// arrays; id (device, may be null = identity) maps slot s to caller index.
+template <class real> void rows_in(hipStream_t s, int64_t n, const double* rows, real* soa, const int* id);
+template <class real> void rows_out(hipStream_t s, int64_t n, const real* soa, double* rows, const int* id);
+template <class real> void scalar_out(hipStream_t s, int64_t n, const real* v, double* out, const int* id);
+}  // namespace layout
+
 namespace pm {
 
 // Grid geometry in `real`, derived like GridInfo (io_json.cpp:28-40).

@@ -117,6 +117,21 @@ class Simulation:
         v = None if vel is None else L.f64(vel, (self.n, 3))
         L.check(L.lib().rrg_set_state(self._h, L.ptr(p), L.ptr(v)))
 
+    def get_state_dev(self, pos=0, vel=0, acc=0, stream=0) -> None:
+        """Device-array form of state(): FP64 (n,3) rows at the given device
+        addresses (0 = skip), ordered after the work queued on `stream`."""
+        L.check(L.lib().rrg_get_state_dev(self._h, pos or None, vel or None, acc or None, stream or None))
+
+    def set_state_dev(self, pos=0, vel=0, stream=0) -> None:
+        """Device-array form of set_state()."""
+        L.check(L.lib().rrg_set_state_dev(self._h, pos or None, vel or None, stream or None))
+
+    def adjoint_dev(self, gx_T=0, gv_T=0, gx_0=0, gv_0=0, gm=0, stream=0) -> None:
+        """Device-array form of adjoint(): seeds and gradients at device
+        addresses (0 = none / skip)."""
+        L.check(L.lib().rrg_adjoint_dev(self._h, gx_T or None, gv_T or None, gx_0 or None, gv_0 or None,
+                                        gm or None, stream or None))
+
     def conservation(self) -> dict:
         c = L.Conservation()
         L.check(L.lib().rrg_get_conservation(self._h, ctypes.byref(c)))
