@@ -43,25 +43,10 @@ WAVE_TRAFFIC_128 = 34_647_301
 # 17240.7 KiB fetched (x 1.9995) and 16384.0 KiB written per launch
 # (profiles/wave_fused_pm1e6_128_hbm_counters.txt).
 WAVE_FUSED_TRAFFIC_128 = 52_077_773
-# Which fused launch shapes the library has (pm_kernels.hip, wave_fused_levels):
-# (three iterations, two iterations) per (real, nx) on periodic grids.
-FUSED_SHAPES = {("fp32", 64): (True, True), ("fp32", 128): (True, True), ("fp32", 256): (True, True),
-                ("fp32", 512): (True, True), ("fp64", 64): (True, True), ("fp64", 128): (True, True),
-                ("fp64", 256): (True, True), ("fp64", 512): (False, True)}
-
-
-def wave_schedule(iters, nx, real="fp32", fused=True):
-    """Launches of one unchecked solve of `iters` iterations, by iterations per
-    launch: the library's rule (as many three-iteration launches as fit, then
-    a two-iteration one where that shape exists, else single launches)."""
-    k3, k2 = FUSED_SHAPES.get((real, nx), (False, False)) if fused else (False, False)
-    n = {3: 0, 2: 0, 1: 0}
-    r = iters
-    while r > 0:
-        lv = 3 if (r >= 3 and k3) else (2 if (r >= 2 and k2) else 1)
-        n[lv] += 1
-        r -= lv
-    return n
+def wave_schedule(rrg, iters, nx, real="fp32", fused=True):
+    """Launches of one unchecked solve of `iters` iterations on a periodic
+    grid, by iterations per launch, as the library splits them."""
+    return rrg.wave_schedule(nx, iters, real=real, fused=fused)
 
 PM = dict(n=1_000_000, nx=128, dx=0.125, dt=5e-3, iterations=200, setup_iterations=2000, diss_base=0.785,
           diss_extra=0.01)
@@ -106,7 +91,7 @@ def pm_bench(rrg, ics, steps, warmup):
     wave_ms, wave_launches = tm["wave"]
     avg_wave_s = wave_ms / 1e3 / max(wave_launches, 1)
     iters = PM["iterations"]
-    sched = wave_schedule(iters, PM["nx"])
+    sched = wave_schedule(rrg, iters, PM["nx"])
     assert sched[3] + sched[2] + sched[1] == round(wave_launches / steps), (sched, wave_launches, steps)
     fused = sched[3] + sched[2]
     n = PM["n"]
@@ -336,7 +321,7 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2):
                 sim.calc_step(steps)
                 tm = sim.timing()
             ms, launches = tm["wave"]
-            sched = wave_schedule(PM["iterations"], nx, fused=fused)
+            sched = wave_schedule(rrg, PM["iterations"], nx, fused=fused)
             assert sum(sched.values()) * steps == launches, (nx, fused, sched, launches)
             own = ((sched[3] + sched[2]) * 20.0 + sched[1] * 16.0) * nx ** 3 * steps
             alg = 16.0 * nx ** 3 * PM["iterations"] * steps

@@ -262,6 +262,12 @@ int rrg_get_grid(rrg_ctx* ctx, double* mass, double* phi, double* acc);
  * single system gives the same as rrg_get_conservation). */
 int rrg_get_conservation_systems(rrg_ctx* ctx, int32_t k, rrg_conservation* out);
 int rrg_get_stats(rrg_ctx* ctx, rrg_stats* out);
+/* How a context of this real type, grid size and boundary type splits
+ * `iterations` unchecked stencil iterations into launches (no counterpart in
+ * the reference): launches[0..2] = launches running 1, 2 and 3 iterations;
+ * `fused` is the wave_fused option. */
+int rrg_wave_schedule(int32_t real_type, int32_t nx, int32_t boundary, int32_t iterations, int32_t fused,
+                      int32_t* launches);
 /* Wait for all work queued on the context stream. */
 int rrg_synchronize(rrg_ctx* ctx);
 /* Turn per-stage event timing on (1) or off (0); either resets the totals. */

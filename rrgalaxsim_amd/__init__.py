@@ -8,8 +8,9 @@ conditions and the ``torch.autograd.Function`` binding of the adjoint.
 from . import _lib
 from ._lib import RRGError, lib
 from .solver import (Simulation, cic_deposit, default_params, direct_accel, direct_grav, gather, gradient,
-                     make_params, ngp_bin, parse_params_json, read_ini, steady_err, wave_iterate)
+                     make_params, ngp_bin, parse_params_json, read_ini, steady_err, wave_iterate, wave_schedule)
 
 __all__ = ["RRGError", "Simulation", "cic_deposit", "default_params", "direct_accel", "direct_grav", "gather",
            "gradient", "lib", "make_params", "ngp_bin", "parse_params_json", "read_ini", "steady_err", "wave_iterate",
+           "wave_schedule",
            "_lib"]

@@ -109,6 +109,7 @@ SIGNATURES = {
     "rrg_get_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "rrg_set_state": (ctypes.c_int, [_vp, _vp, _vp]),
     "rrg_get_state_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "rrg_wave_schedule": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32, _vp]),
     "rrg_set_state_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "rrg_adjoint_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rrg_get_conservation": (ctypes.c_int, [_vp, ctypes.POINTER(Conservation)]),

@@ -1785,6 +1785,26 @@ RRG_API int rrg_adjoint_dev(rrg_ctx* ctx, const double* gx_T, const double* gv_T
     });
 }
 
+RRG_API int rrg_wave_schedule(int32_t real_type, int32_t nx, int32_t boundary, int32_t iterations, int32_t fused,
+                              int32_t* launches) {
+    return guarded([&] {
+        require(launches != nullptr && nx >= 1 && iterations >= 0, "rrg_wave_schedule: bad arguments");
+        require(real_type == RRG_REAL_F32 || real_type == RRG_REAL_F64, "rrg_wave_schedule: bad real_type");
+        launches[0] = launches[1] = launches[2] = 0;
+        auto levels = [&](int r) {
+            if (!fused) return 1;
+            return real_type == RRG_REAL_F32 ? pm::wave_fused_levels<float>(pm::make_model<float>(nx, 1, 0, 0, 0, 1, 1,
+                                                                                                  boundary), r)
+                                             : pm::wave_fused_levels<double>(pm::make_model<double>(nx, 1, 0, 0, 0, 1,
+                                                                                                    1, boundary), r);
+        };
+        int ip = 0, ic = 1;
+        pm::wave_schedule(
+            iterations, levels, ip, ic, [&](int, int, int) { ++launches[0]; },
+            [&](int lv, int, int, int, int) { ++launches[lv - 1]; });
+    });
+}
+
 RRG_API int rrg_get_conservation(rrg_ctx* ctx, rrg_conservation* out) {
     RRG_CTX_CALL(ctx, {
         require(out != nullptr, "rrg_get_conservation: null output");
