@@ -62,11 +62,11 @@ def _device_sync():
         pass
 
 
-def pm_bench(rrg, ics, steps, warmup):
+def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     pos, vel, m = ics.exp_disk(PM["n"], seed=1)
     wave = dict(diss_base=PM["diss_base"], diss_extra=PM["diss_extra"], wave_speed=C,
                 setup_iterations=PM["setup_iterations"], iterations=PM["iterations"], tolerance=0.0)
-    p = rrg.make_params(dt=PM["dt"], grid=dict(nx=PM["nx"], dx=PM["dx"], bc_frac=0.0), wave=wave, real="fp32")
+    p = rrg.make_params(dt=PM["dt"], grid=dict(nx=PM["nx"], dx=PM["dx"], bc_frac=0.0), wave=wave, real=real)
     sim = rrg.Simulation(p, pos, vel, m)
     sim.precalc()
     sim.sync_particles_and_grid()
@@ -91,7 +91,7 @@ def pm_bench(rrg, ics, steps, warmup):
     wave_ms, wave_launches = tm["wave"]
     avg_wave_s = wave_ms / 1e3 / max(wave_launches, 1)
     iters = PM["iterations"]
-    sched = wave_schedule(rrg, iters, PM["nx"])
+    sched = wave_schedule(rrg, iters, PM["nx"], real=real)
     assert sched[3] + sched[2] + sched[1] == round(wave_launches / steps), (sched, wave_launches, steps)
     fused = sched[3] + sched[2]
     n = PM["n"]
@@ -110,13 +110,16 @@ def pm_bench(rrg, ics, steps, warmup):
     own_bytes = (fused * 20 * ncell + sched[1] * 16 * ncell + 20 * ncell + 24 * ncell + n * (40 + 24 + 64) +
                  n * (36 + 24 + 128))
     secs = t1 - t0
-    # PM force vs an FP64 direct sum over all particles (softening = dx): the
-    # method error of the PM model (periodic source, cell-size resolution)
-    idx, ref = _fp64_sample_forces(pos, m, np.full(n, PM["dx"] ** 2), nsample=256)
-    err = np.linalg.norm(acc0[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
-    return {
-        "pm_force_err_vs_fp64_direct": {"median": float(np.median(err)), "p99": float(np.percentile(err, 99)),
-                                        "sample": "256 particles (seed 5), t=0, direct sum softened by dx"},
+    out = {}
+    if force_err:
+        # PM force vs an FP64 direct sum over all particles (softening = dx):
+        # the method error of the PM model (periodic source, cell-size
+        # resolution)
+        idx, ref = _fp64_sample_forces(pos, m, np.full(n, PM["dx"] ** 2), nsample=256)
+        err = np.linalg.norm(acc0[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        out["pm_force_err_vs_fp64_direct"] = {"median": float(np.median(err)), "p99": float(np.percentile(err, 99)),
+                                              "sample": "256 particles (seed 5), t=0, direct sum softened by dx"}
+    return out | {
         "value": n * steps / secs / 1e6,
         "ms_per_step": secs / steps * 1e3,
         "stage_ms_per_step": {k: v[0] / steps for k, v in tm.items() if v[1]},
@@ -130,6 +133,16 @@ def pm_bench(rrg, ics, steps, warmup):
         "step_own_minimum_bytes": own_bytes,
         "wave_iterations_per_step": stats["last_wave_iterations"],
     }
+
+
+def pm_fp64_bench(rrg, ics, steps=10, warmup=2):
+    """The headline PM workload with FP64 particles and grid (the reference's
+    default precision): throughput, step time and stage breakdown."""
+    r = pm_bench(rrg, ics, steps, warmup, real="fp64", force_err=False)
+    return {"value": r["value"], "unit": "Mparticle-steps/s", "ms_per_step": r["ms_per_step"],
+            "stage_ms_per_step": r["stage_ms_per_step"], "wave_us_per_iteration": r["wave_us_per_iteration"],
+            "wave_launches_per_step": r["wave_launches_per_step"],
+            "config": {"workload": "as the headline, FP64 state", "timed_steps": steps}}
 
 
 def _fp64_sample_forces(pos, m, s2, nsample=512, chunk=1):
@@ -506,7 +519,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-pass", action="store_true", help="short PM run + calibration copy for rocprofv3 --pmc")
     ap.add_argument("--skip", default="", help="comma-separated legs to leave out: direct, c1, scaling, ensemble, "
-                                              "adjoint, adjoint_pm, cpu")
+                                              "adjoint, adjoint_pm, pm_fp64, cpu")
     args = ap.parse_args()
     skip = set(x for x in args.skip.split(",") if x)
     t_start = time.perf_counter()
@@ -581,7 +594,8 @@ def main():
                 ("scaling", "wave_scaling", lambda: wave_scaling(rrg, ics)),
                 ("ensemble", "ensemble", lambda: ensemble_bench(rrg, ics)),
                 ("adjoint", "adjoint", lambda: adjoint_bench(rrg, ics)),
-                ("adjoint_pm", "adjoint_pm", lambda: pm_adjoint_bench(rrg, ics))]
+                ("adjoint_pm", "adjoint_pm", lambda: pm_adjoint_bench(rrg, ics)),
+                ("pm_fp64", "pm_fp64", lambda: pm_fp64_bench(rrg, ics))]
         for leg, key, fn in legs:
             if leg in skip:
                 continue
