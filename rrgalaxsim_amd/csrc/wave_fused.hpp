@@ -30,9 +30,13 @@ namespace {
 // L-1's x/y neighbours from a two-plane LDS ring and its z neighbours from
 // registers (the last three planes of every level stay in registers; level
 // L's "old" term is level L-2 two planes back, the input's own column for
-// L = 2).  The input's x/y neighbours come from the cache.  Loads for the
-// next plane are issued PD planes ahead; the barrier only waits for LDS
-// traffic, so they stay in flight across it.  HBM traffic per cell: 3 reals
+// L = 2).  With one wave per row (RS = 1) the wave holds the whole x row, so
+// x neighbours come from its own registers by DPP wave rotates, and the LDS
+// ring only serves y neighbours.  The input's y neighbours come from the
+// cache, or (YL) from an LDS ring of the input plane for all but the two edge
+// row waves.  Loads for the next plane are issued PD planes ahead; the barrier
+// only waits for LDS traffic, so they stay in flight across it.  HBM traffic
+// per cell: 3 reals
 // read and 2 written for K forward iterations (4K for K single launches),
 // 3 read and 3 written for K adjoint iterations (5K), and one launch
 // instead of K.
@@ -40,7 +44,36 @@ template <class real, int XJ> struct PlaneIn {
     real a[XJ], b[XJ], c[XJ], d[XJ], po[XJ], src[XJ], pn[XJ], sb[XJ];
 };
 
-template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool ADJ, bool FIRST>
+// v of lane l-1 (CTRL 0x13C, wave_ror:1) or of lane l+1 (0x134, wave_rol:1),
+// rotating over the 64 lanes of the wave.
+template <int CTRL> __device__ __forceinline__ float wave_rot(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL> __device__ __forceinline__ double wave_rot(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, int(unsigned(u)), CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, int(unsigned(u >> 32)), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The x-1 and x+1 neighbours of a periodic x row held by one wave as XJ
+// chunks of 64 lanes (x = 64 j + lane).
+template <class real, int XJ>
+__device__ __forceinline__ void row_neighbours(const real (&r)[XJ], real (&m)[XJ], real (&p)[XJ], int lane) {
+    real rr[XJ], rl[XJ];
+#pragma unroll
+    for (int j = 0; j < XJ; ++j) {
+        rr[j] = wave_rot<0x13C>(r[j]);
+        rl[j] = wave_rot<0x134>(r[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < XJ; ++j) {
+        m[j] = lane == 0 ? rr[(j + XJ - 1) % XJ] : rr[j];
+        p[j] = lane == 63 ? rl[(j + 1) % XJ] : rl[j];
+    }
+}
+
+template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool ADJ, bool FIRST>
 __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real* __restrict__ qa,
                                                                            real* __restrict__ qb,
                                                                            const real* __restrict__ p,
@@ -54,6 +87,7 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     static_assert(K >= 2 && (NX & (NX - 1)) == 0 && NX % TY == 0 && NX % ZB == 0, "shape");
     static_assert(RS * ROWS <= 16, "a workgroup holds at most 16 waves");
     __shared__ real lq[K - 1][2][ROWS][NX];
+    __shared__ real l0[YL ? 2 : 1][YL ? ROWS : 1][NX];  // (YL) the input plane, for y neighbours
     int b = blockIdx.x;
     if constexpr ((NB & 7) == 0) b = (b & 7) * (NB >> 3) + (b >> 3);  // one z slab per XCD
     const int by = b % NY, bz = b / NY;
@@ -65,6 +99,7 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     const int y = (Y0 - (K - 1) + w + NX) & (NX - 1);
     const int ym = (y - 1) & (NX - 1), yp = (y + 1) & (NX - 1);
     const bool out_row = w >= K - 1 && w < K - 1 + TY;  // rows Y0 .. Y0+TY-1
+    const bool edge = !YL || w == 0 || w == ROWS - 1;    // loads the input's y neighbours
     unsigned x[XJ], xm[XJ], xp[XJ];
 #pragma unroll
     for (int j = 0; j < XJ; ++j) {
@@ -84,12 +119,22 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
         const real* __restrict__ rp = p + row(zq, yp);
         const real* __restrict__ r2 = p + row(zq + 2, y);
         const real* __restrict__ ro = po + row(zq, y);
+        if constexpr (RS > 1) {
+#pragma unroll
+            for (int j = 0; j < XJ; ++j) {
+                in.a[j] = r0[xm[j]];
+                in.b[j] = r0[xp[j]];
+            }
+        }
+        if (edge) {
+#pragma unroll
+            for (int j = 0; j < XJ; ++j) {
+                in.c[j] = rm[x[j]];
+                in.d[j] = rp[x[j]];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < XJ; ++j) {
-            in.a[j] = r0[xm[j]];
-            in.b[j] = r0[xp[j]];
-            in.c[j] = rm[x[j]];
-            in.d[j] = rp[x[j]];
             in.po[j] = ro[x[j]];
             in.pn[j] = r2[x[j]];
         }
@@ -128,10 +173,28 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     PlaneIn<real, XJ> ring[PD];
 #pragma unroll
     for (int i = 0; i < PD; ++i) load(ring[i], i);
+    if constexpr (YL) {  // the input plane of step 0, read at step 0 from slot 1
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) l0[1][w][x[j]] = Pv[1][j];
+        __syncthreads();
+    }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        const PlaneIn<real, XJ> in = ring[s % PD];
+        PlaneIn<real, XJ> in = ring[s % PD];
         if (s + PD < S) load(ring[s % PD], s + PD);
+        if constexpr (YL) {
+            if (!edge) {
+#pragma unroll
+                for (int j = 0; j < XJ; ++j) {
+                    in.c[j] = l0[(s - 1) & 1][w - 1][x[j]];
+                    in.d[j] = l0[(s - 1) & 1][w + 1][x[j]];
+                }
+            }
+            // the input plane of step s+1; slot s&1 was last read at step s-1
+#pragma unroll
+            for (int j = 0; j < XJ; ++j) l0[s & 1][w][x[j]] = Pv[0][j];
+        }
+        if constexpr (RS == 1) row_neighbours<real, XJ>(Pv[1], in.a, in.b, l);
         real nw[K + 1][XJ];
 #pragma unroll
         for (int j = 0; j < XJ; ++j) {
@@ -151,11 +214,21 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
             // level L is live from step 2(L-1) on, on rows L-1 .. ROWS-L
             if (s >= 2 * (L - 1) && w >= L - 1 && w < ROWS - (L - 1)) {
                 const int sl = (s - 1) & 1;
+                // x neighbours of level L-1 on this row: its previous step
+                real bm[XJ], bp[XJ];
+                if constexpr (RS == 1) {
+                    row_neighbours<real, XJ>(h1[L - 1], bm, bp, l);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < XJ; ++j) {
+                        bm[j] = lq[L - 2][sl][w][xm[j]];
+                        bp[j] = lq[L - 2][sl][w][xp[j]];
+                    }
+                }
 #pragma unroll
                 for (int j = 0; j < XJ; ++j) {
-                    const real lap_sum = lq[L - 2][sl][w][xm[j]] + lq[L - 2][sl][w][xp[j]] +
-                                         lq[L - 2][sl][w - 1][x[j]] + lq[L - 2][sl][w + 1][x[j]] + h2[L - 1][j] +
-                                         nw[L - 1][j];
+                    const real lap_sum = bm[j] + bp[j] + lq[L - 2][sl][w - 1][x[j]] + lq[L - 2][sl][w + 1][x[j]] +
+                                         h2[L - 1][j] + nw[L - 1][j];
                     const real old = L == 2 ? Pv[2][j] : h2[L - 2][j];
                     if constexpr (ADJ) nw[L][j] = lap_sum * k.k_main + h1[L - 1][j] * k.k_actual + old * k.k_old;
                     else nw[L][j] = lap_sum * k.k_main + h1[L - 1][j] * k.k_actual + old * k.k_old + sh[L - 1][j];
@@ -203,26 +276,27 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     }
 }
 
-template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool ADJ, bool FIRST>
+template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool ADJ, bool FIRST>
 void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* po, const real* src, real* sbar,
                   const WaveK<real>& k) {
     constexpr int NX = 64 * XJ * RS;
-    k_wave_fused<real, K, XJ, RS, TY, ZB, PD, ADJ, FIRST>
+    k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, ADJ, FIRST>
         <<<(NX / TY) * (NX / ZB), 64 * RS * (TY + 2 * K - 2), 0, s>>>(qa, qb, p, po, src, sbar, k);
 }
 
-// Launch shapes (K, XJ, RS, TY, ZB, PD), from sweeps of graph-replayed
+// Launch shapes (K, XJ, RS, TY, ZB, PD, YL), from sweeps of graph-replayed
 // 240-iteration chains of the forward kernel on MI355X (time per iteration;
 // one iteration per launch in brackets):
-//          FP32                                FP64
-//   64^3   K=3 (1,1,4,4,2)  1.51 us (2.74)    K=3 (1,1,4,4,1)  2.28 us (3.32)
-//   128^3  K=3 (2,1,8,8,1)  3.51-3.61 (6.8)   K=3 (2,1,8,8,1)  7.26-7.44 (12.0)
-//   256^3  K=3 (4,1,8,32,1) 26.7 (38.0)       K=3 (2,2,4,64,1) 63.3 (115.6)
-//   512^3  K=3 (4,2,4,32,1) 222.8 (440)       K=2 (2,4,2,32,1) 802 (932); K=3
-//                                             shapes spill registers
-// and for a remainder of two iterations the K=2 shapes below.  The adjoint
-// uses the same shapes.  Other grids, and non-periodic ones, run one
-// iteration per launch.
+//          FP32                                  FP64
+//   64^3   K=3 (1,1,4,4,1,0)  1.39 us (2.74)     K=3 (1,1,4,4,1,0)  2.05 us (3.32)
+//   128^3  K=3 (2,1,8,8,1,0)  3.52-3.56 (6.8)    K=3 (2,1,8,8,1,1)  6.68 (12.0)
+//   256^3  K=3 (4,1,8,32,1,1) 24.2 (38.0)        K=3 (2,2,4,64,1,0) 63.3 (115.6)
+//   512^3  K=3 (4,2,4,32,1,0) 222.8 (440)        K=2 (2,4,2,32,1,0) 802 (932); K=3
+//                                                shapes spill registers
+// and for a remainder of two iterations the K=2 shapes below.  The LDS ring
+// of the input plane (YL) pays where a plane is large (256^3 FP32, 128^3
+// FP64) and costs 5-7% on the small FP32 grids.  The adjoint uses the same
+// shapes.  Other grids, and non-periodic ones, run one iteration per launch.
 template <class real> int fused_levels(int nx, int bc, int remaining) {
     constexpr bool f32 = std::is_same_v<real, float>;
     if (bc != RRG_BC_PERIODIC || remaining < 2) return 1;
@@ -239,28 +313,28 @@ void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const r
     constexpr bool f32 = std::is_same_v<real, float>;
     if (levels == 3) {
         if (nx == 64) {
-            if constexpr (f32) launch_shape<real, 3, 1, 1, 4, 4, 2, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 3, 1, 1, 4, 4, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 1, 1, 4, 4, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 128) {
-            launch_shape<real, 3, 2, 1, 8, 8, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 2, 1, 8, 8, 1, !f32, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 256) {
-            if constexpr (f32) launch_shape<real, 3, 4, 1, 8, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 3, 2, 2, 4, 64, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            if constexpr (f32) launch_shape<real, 3, 4, 1, 8, 32, 1, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 3, 2, 2, 4, 64, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if constexpr (f32) {
-            launch_shape<real, 3, 4, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 4, 2, 4, 32, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         }
     } else {
         if (nx == 64) {
-            launch_shape<real, 2, 1, 1, 4, 4, 2, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            if constexpr (f32) launch_shape<real, 2, 1, 1, 4, 4, 2, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 2, 1, 1, 4, 4, 1, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 128) {
-            if constexpr (f32) launch_shape<real, 2, 1, 2, 4, 8, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 2, 1, 2, 4, 16, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            if constexpr (f32) launch_shape<real, 2, 1, 2, 4, 8, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 2, 2, 1, 8, 8, 1, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 256) {
-            if constexpr (f32) launch_shape<real, 2, 4, 1, 8, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 2, 2, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            if constexpr (f32) launch_shape<real, 2, 4, 1, 8, 32, 1, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 2, 2, 2, 4, 32, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else {
-            if constexpr (f32) launch_shape<real, 2, 4, 2, 4, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 2, 2, 4, 2, 32, 1, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            if constexpr (f32) launch_shape<real, 2, 4, 2, 4, 32, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else launch_shape<real, 2, 2, 4, 2, 32, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         }
     }
 }
