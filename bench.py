@@ -40,9 +40,9 @@ C = 4574.337022617616
 # commands is profiles/wave_pm1e6_128_hbm_counters.txt.
 WAVE_TRAFFIC_128 = 34_647_301
 # The same for one three-iteration k_wave_fused<float, 3> launch, same method:
-# 17240.7 KiB fetched (x 1.9995) and 16384.0 KiB written per launch
+# 17186.2 KiB fetched (x 1.9995) and 16384.0 KiB written per launch
 # (profiles/wave_fused_pm1e6_128_hbm_counters.txt).
-WAVE_FUSED_TRAFFIC_128 = 52_077_773
+WAVE_FUSED_TRAFFIC_128 = 51_965_754
 def wave_schedule(rrg, iters, nx, real="fp32", fused=True):
     """Launches of one unchecked solve of `iters` iterations on a periodic
     grid, by iterations per launch, as the library splits them."""
@@ -478,18 +478,20 @@ def cpu_baseline(ics, target_s=10.0):
                       f"workload, {k} PM steps of 200 wave iterations ({k * t:.1f} s), minus precalc"}
 
 
-def profile_pass(rrg, ics, steps):
+def profile_pass(rrg, ics, steps, graphs=True):
     """Short run for hardware-counter passes: `steps` PM steps of the headline
     workload (setup iterations cut to 200), then one device-to-device copy of
     exactly 128 MiB through the HIP runtime (its copy kernel reads and writes
     128 MiB), whose FETCH_SIZE / WRITE_SIZE per dispatch calibrate the
-    counters' units."""
+    counters' units.  graphs=False launches every kernel eagerly, for kernel
+    tracers that do not follow graph launches."""
     import ctypes
 
     pos, vel, m = ics.exp_disk(PM["n"], seed=1)
     wave = dict(diss_base=PM["diss_base"], diss_extra=PM["diss_extra"], wave_speed=C, setup_iterations=200,
                 iterations=PM["iterations"], tolerance=0.0)
-    p = rrg.make_params(dt=PM["dt"], grid=dict(nx=PM["nx"], dx=PM["dx"], bc_frac=0.0), wave=wave, real="fp32")
+    p = rrg.make_params(dt=PM["dt"], grid=dict(nx=PM["nx"], dx=PM["dx"], bc_frac=0.0), wave=wave, real="fp32",
+                        graphs=graphs)
     with rrg.Simulation(p, pos, vel, m) as sim:
         sim.precalc()
         sim.calc_step(steps)
@@ -518,6 +520,7 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-pass", action="store_true", help="short PM run + calibration copy for rocprofv3 --pmc")
+    ap.add_argument("--eager", action="store_true", help="with --profile-pass: launch kernels eagerly, not from graphs")
     ap.add_argument("--skip", default="", help="comma-separated legs to leave out: direct, c1, scaling, ensemble, "
                                               "adjoint, adjoint_pm, pm_fp64, cpu")
     args = ap.parse_args()
@@ -531,7 +534,7 @@ def main():
         from rrgalaxsim_amd import ics
 
         rrg.lib()
-        profile_pass(rrg, ics, args.steps)
+        profile_pass(rrg, ics, args.steps, graphs=not args.eager)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the hot path has no "
