@@ -30,9 +30,12 @@ namespace {
 // L-1's x/y neighbours from a two-plane LDS ring and its z neighbours from
 // registers (the last three planes of every level stay in registers; level
 // L's "old" term is level L-2 two planes back, the input's own column for
-// L = 2).  With one wave per row (RS = 1) the wave holds the whole x row, so
-// x neighbours come from its own registers by DPP wave rotates, and the LDS
-// ring only serves y neighbours.  The input's y neighbours come from the
+// L = 2).  With one wave per row (RS = 1) the wave holds the whole x row and
+// takes x neighbours from registers: with XJ consecutive cells per lane (XC)
+// they are the lane's own cells and one DPP wave rotate at each end, and rows
+// move with wide loads and stores; with XJ chunks of 64 lanes, a DPP wave
+// rotate and a lane select per chunk.  The LDS ring then only serves y
+// neighbours.  The input's y neighbours come from the
 // cache, or (YL) from an LDS ring of the input plane for all but the two edge
 // row waves.  Loads for the next plane are issued PD planes ahead; the barrier
 // only waits for LDS traffic, so they stay in flight across it.  HBM traffic
@@ -56,24 +59,37 @@ template <int CTRL> __device__ __forceinline__ double wave_rot(double v) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// The x-1 and x+1 neighbours of a periodic x row held by one wave as XJ
-// chunks of 64 lanes (x = 64 j + lane).
-template <class real, int XJ>
+// The x-1 and x+1 neighbours of a periodic x row held by one wave.  XC: XJ
+// consecutive cells per lane (x = XJ lane + j): the lane's own cells, and one
+// wave rotate at each end (lane 0's x-1 is lane 63's last cell, x = NX-1;
+// lane 63's x+1 is lane 0's first, x = 0).  Otherwise XJ chunks of 64 lanes
+// (x = 64 j + lane): each chunk rotated, the end lanes taking the
+// neighbouring chunk's.
+template <class real, int XJ, bool XC>
 __device__ __forceinline__ void row_neighbours(const real (&r)[XJ], real (&m)[XJ], real (&p)[XJ], int lane) {
-    real rr[XJ], rl[XJ];
+    if constexpr (XC) {
+        m[0] = wave_rot<0x13C>(r[XJ - 1]);
 #pragma unroll
-    for (int j = 0; j < XJ; ++j) {
-        rr[j] = wave_rot<0x13C>(r[j]);
-        rl[j] = wave_rot<0x134>(r[j]);
-    }
+        for (int j = 1; j < XJ; ++j) m[j] = r[j - 1];
 #pragma unroll
-    for (int j = 0; j < XJ; ++j) {
-        m[j] = lane == 0 ? rr[(j + XJ - 1) % XJ] : rr[j];
-        p[j] = lane == 63 ? rl[(j + 1) % XJ] : rl[j];
+        for (int j = 0; j < XJ - 1; ++j) p[j] = r[j + 1];
+        p[XJ - 1] = wave_rot<0x134>(r[0]);
+    } else {
+        real rr[XJ], rl[XJ];
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) {
+            rr[j] = wave_rot<0x13C>(r[j]);
+            rl[j] = wave_rot<0x134>(r[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < XJ; ++j) {
+            m[j] = lane == 0 ? rr[(j + XJ - 1) % XJ] : rr[j];
+            p[j] = lane == 63 ? rl[(j + 1) % XJ] : rl[j];
+        }
     }
 }
 
-template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool ADJ, bool FIRST>
+template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST>
 __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real* __restrict__ qa,
                                                                            real* __restrict__ qb,
                                                                            const real* __restrict__ p,
@@ -86,6 +102,7 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     constexpr int S = ZB + 2 * K - 2;  // level-1 planes Z0-(K-1) .. Z0+ZB+K-2
     static_assert(K >= 2 && (NX & (NX - 1)) == 0 && NX % TY == 0 && NX % ZB == 0, "shape");
     static_assert(RS * ROWS <= 16, "a workgroup holds at most 16 waves");
+    static_assert(!XC || RS == 1, "consecutive cells per lane need one wave per row");
     __shared__ real lq[K - 1][2][ROWS][NX];
     __shared__ real l0[YL ? 2 : 1][YL ? ROWS : 1][NX];  // (YL) the input plane, for y neighbours
     int b = blockIdx.x;
@@ -100,10 +117,11 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     const int ym = (y - 1) & (NX - 1), yp = (y + 1) & (NX - 1);
     const bool out_row = w >= K - 1 && w < K - 1 + TY;  // rows Y0 .. Y0+TY-1
     const bool edge = !YL || w == 0 || w == ROWS - 1;    // loads the input's y neighbours
+    // XC: XJ consecutive cells per lane; otherwise XJ chunks of 64 lanes
     unsigned x[XJ], xm[XJ], xp[XJ];
 #pragma unroll
     for (int j = 0; j < XJ; ++j) {
-        x[j] = unsigned(seg * 64 * XJ + 64 * j + l);
+        x[j] = XC ? unsigned(XJ * l + j) : unsigned(seg * 64 * XJ + 64 * j + l);
         xm[j] = (x[j] - 1u) & unsigned(NX - 1);
         xp[j] = (x[j] + 1u) & unsigned(NX - 1);
     }
@@ -194,7 +212,7 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
 #pragma unroll
             for (int j = 0; j < XJ; ++j) l0[s & 1][w][x[j]] = Pv[0][j];
         }
-        if constexpr (RS == 1) row_neighbours<real, XJ>(Pv[1], in.a, in.b, l);
+        if constexpr (RS == 1) row_neighbours<real, XJ, XC>(Pv[1], in.a, in.b, l);
         real nw[K + 1][XJ];
 #pragma unroll
         for (int j = 0; j < XJ; ++j) {
@@ -217,7 +235,7 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
                 // x neighbours of level L-1 on this row: its previous step
                 real bm[XJ], bp[XJ];
                 if constexpr (RS == 1) {
-                    row_neighbours<real, XJ>(h1[L - 1], bm, bp, l);
+                    row_neighbours<real, XJ, XC>(h1[L - 1], bm, bp, l);
                 } else {
 #pragma unroll
                     for (int j = 0; j < XJ; ++j) {
@@ -276,27 +294,29 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     }
 }
 
-template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool ADJ, bool FIRST>
+template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST>
 void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* po, const real* src, real* sbar,
                   const WaveK<real>& k) {
     constexpr int NX = 64 * XJ * RS;
-    k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, ADJ, FIRST>
+    k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST>
         <<<(NX / TY) * (NX / ZB), 64 * RS * (TY + 2 * K - 2), 0, s>>>(qa, qb, p, po, src, sbar, k);
 }
 
-// Launch shapes (K, XJ, RS, TY, ZB, PD, YL), from sweeps of graph-replayed
-// 240-iteration chains of the forward kernel on MI355X (time per iteration;
-// one iteration per launch in brackets):
-//          FP32                                  FP64
-//   64^3   K=3 (1,1,4,4,1,0)  1.39 us (2.74)     K=3 (1,1,4,4,1,0)  2.05 us (3.32)
-//   128^3  K=3 (2,1,8,8,1,0)  3.52-3.56 (6.8)    K=3 (2,1,8,8,1,1)  6.68 (12.0)
-//   256^3  K=3 (4,1,8,32,1,1) 24.2 (38.0)        K=3 (2,2,4,64,1,0) 63.3 (115.6)
-//   512^3  K=3 (4,2,4,32,1,0) 222.8 (440)        K=2 (2,4,2,32,1,0) 802 (932); K=3
-//                                                shapes spill registers
+// Launch shapes (K, XJ, RS, TY, ZB, PD, YL, XC), from sweeps of
+// graph-replayed 240-iteration chains of the forward kernel on MI355X (time
+// per iteration; one iteration per launch in brackets):
+//          FP32                                    FP64
+//   64^3   K=3 (1,1,4,4,1,0,1)  1.39 us (2.74)     K=3 (1,1,4,4,1,0,1)  2.07 us (3.32)
+//   128^3  K=3 (2,1,8,8,1,0,0)  3.50-3.68 (6.8)    K=3 (2,1,8,8,1,1,1)  5.71 (12.0)
+//   256^3  K=3 (4,1,8,32,1,1,1) 23.4 (38.0)        K=3 (2,2,4,64,1,0,0) 63.0 (115.6)
+//   512^3  K=3 (4,2,4,32,1,0,0) 218 (440)          K=2 (2,4,2,32,1,0,0) 790 (932); K=3
+//                                                  shapes spill registers
 // and for a remainder of two iterations the K=2 shapes below.  The LDS ring
 // of the input plane (YL) pays where a plane is large (256^3 FP32, 128^3
-// FP64) and costs 5-7% on the small FP32 grids.  The adjoint uses the same
-// shapes.  Other grids, and non-periodic ones, run one iteration per launch.
+// FP64) and costs 5-7% on the small FP32 grids; consecutive cells per lane
+// (XC) gain 11-15% at 128^3 FP64 and 2.5% at 256^3 FP32 and lose 2-4% at
+// 128^3 FP32.  The adjoint uses the same shapes.  Other grids, and
+// non-periodic ones, run one iteration per launch.
 template <class real> int fused_levels(int nx, int bc, int remaining) {
     constexpr bool f32 = std::is_same_v<real, float>;
     if (bc != RRG_BC_PERIODIC || remaining < 2) return 1;
@@ -313,28 +333,38 @@ void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const r
     constexpr bool f32 = std::is_same_v<real, float>;
     if (levels == 3) {
         if (nx == 64) {
-            launch_shape<real, 3, 1, 1, 4, 4, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 1, 1, 4, 4, 1, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 128) {
-            launch_shape<real, 3, 2, 1, 8, 8, 1, !f32, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 2, 1, 8, 8, 1, !f32, !f32, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 256) {
-            if constexpr (f32) launch_shape<real, 3, 4, 1, 8, 32, 1, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 3, 2, 2, 4, 64, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            if constexpr (f32)
+                launch_shape<real, 3, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else
+                launch_shape<real, 3, 2, 2, 4, 64, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if constexpr (f32) {
-            launch_shape<real, 3, 4, 2, 4, 32, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 4, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         }
     } else {
         if (nx == 64) {
-            if constexpr (f32) launch_shape<real, 2, 1, 1, 4, 4, 2, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 2, 1, 1, 4, 4, 1, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            if constexpr (f32)
+                launch_shape<real, 2, 1, 1, 4, 4, 2, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else
+                launch_shape<real, 2, 1, 1, 4, 4, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 128) {
-            if constexpr (f32) launch_shape<real, 2, 1, 2, 4, 8, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 2, 2, 1, 8, 8, 1, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            if constexpr (f32)
+                launch_shape<real, 2, 1, 2, 4, 8, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else
+                launch_shape<real, 2, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 256) {
-            if constexpr (f32) launch_shape<real, 2, 4, 1, 8, 32, 1, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 2, 2, 2, 4, 32, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            if constexpr (f32)
+                launch_shape<real, 2, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else
+                launch_shape<real, 2, 2, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else {
-            if constexpr (f32) launch_shape<real, 2, 4, 2, 4, 32, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
-            else launch_shape<real, 2, 2, 4, 2, 32, 1, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            if constexpr (f32)
+                launch_shape<real, 2, 4, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            else
+                launch_shape<real, 2, 2, 4, 2, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         }
     }
 }
