@@ -151,3 +151,52 @@ def test_pm_adjoint_fused_matches_oracle(rrg, orc, iters, setup):
                                   setup_iterations=setup, iterations=iters, fixed_e=e)
     rel = lambda a, b: np.max(np.abs(a - b)) / np.max(np.abs(b))  # noqa: E731
     assert rel(gx0, ox0) < 1e-9 and rel(gv0, ov0) < 1e-9 and rel(gm, om) < 1e-9
+
+
+@pytest.mark.parametrize("iters,setup", [(5, 7), (6, 8)])
+def test_pm_adjoint_fused_128_matches_oracle(rrg, orc, iters, setup):
+    """128^3 FP64: the transposed stencil's three- and two-iteration shapes
+    (consecutive cells per lane, input plane in LDS) against the oracle's
+    reverse sweep."""
+    n, steps = 3000, 4
+    pos, vel, m = ics.exp_disk(n, seed=11)
+    rng = np.random.default_rng(12)
+    gx, gv = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    wave = dict(diss_base=0.785, diss_extra=0.0, wave_speed=C, setup_iterations=setup, iterations=iters, tolerance=0.0)
+    p = rrg.make_params(dt=0.02, grid=dict(nx=128, dx=0.125), wave=wave)
+    with rrg.Simulation(p, pos, vel, m) as sim:
+        sim.precalc()
+        sim.adjoint_begin(2)
+        sim.calc_step(steps)
+        gx0, gv0, gm = sim.adjoint(gx, gv)
+        e = sim.stats()["fixed_exponent"]
+    ox0, ov0, om = orc.pm_adjoint(pos, vel, m, steps, gx, gv, nx=128, dx=0.125, dt=0.02, diss_base=0.785,
+                                  wave_speed=C, setup_iterations=setup, iterations=iters, fixed_e=e)
+    rel = lambda a, b: np.max(np.abs(a - b)) / np.max(np.abs(b))  # noqa: E731
+    assert rel(gx0, ox0) < 1e-9 and rel(gv0, ov0) < 1e-9 and rel(gm, om) < 1e-9
+
+
+@pytest.mark.parametrize("nx", [64, 128, 256])
+def test_pm_adjoint_fused_equals_single_fp32(rrg, nx):
+    """FP32 transposed stencil at every fused FP32 shape (three- and
+    two-iteration launches): forward states bitwise equal to one iteration
+    per launch, gradients equal up to the ordering of the adjoint's FP64
+    atomic sums."""
+    n = 4000
+    pos, vel, m = ics.exp_disk(n, seed=3)
+    rng = np.random.default_rng(4)
+    gx, gv = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=7, iterations=5, tolerance=0.0)
+    out = []
+    for fused in (True, False):
+        p = rrg.make_params(dt=0.02, grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real="fp32", wave_fused=fused)
+        with rrg.Simulation(p, pos, vel, m) as sim:
+            sim.precalc()
+            sim.adjoint_begin(2)
+            sim.calc_step(4)
+            st = sim.state()
+            out.append((st["pos"], st["vel"]) + tuple(sim.adjoint(gx, gv)))
+    a, b = out
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for ga, gb in zip(a[2:], b[2:]):
+        assert np.max(np.abs(ga - gb)) <= 1e-5 * np.max(np.abs(gb)), (nx, np.max(np.abs(ga - gb)) / np.max(np.abs(gb)))
