@@ -156,27 +156,38 @@ def pm_fp64_bench(rrg, ics, steps=10, warmup=2):
             "config": {"workload": "as the headline, FP64 state", "timed_steps": steps}}
 
 
-def _fp64_sample_forces(pos, m, s2, nsample=512, chunk=1):
+def _fp64_sample_forces(pos, m, s2, nsample=512):
     """Host FP64 direct sum a_i = sum_j m_j d_ij (r^2 + (s2_i + s2_j)/2)^(-3/2), d_ij = x_j - x_i,
-    for a seeded sample of targets (the reference kernel's formula, nbody.cu:19-65)."""
+    for a seeded sample of targets (the reference kernel's formula, nbody.cu:19-65); one target at
+    a time over all sources, targets spread over host threads (numpy releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     idx = np.sort(np.random.default_rng(5).choice(len(m), nsample, replace=False))
     ref = np.empty((nsample, 3))
     x, y, z = (np.ascontiguousarray(pos[:, c]) for c in range(3))
-    for a in range(0, nsample, chunk):
-        t = idx[a:a + chunk]
-        dx, dy, dz = x - x[t, None], y - y[t, None], z - z[t, None]
-        q = dx * dx + dy * dy + dz * dz + 0.5 * (s2[t, None] + s2)
+
+    def one(a):
+        t = idx[a]
+        dx, dy, dz = x - x[t], y - y[t], z - z[t]
+        q = dx * dx + dy * dy + dz * dz + 0.5 * (s2[t] + s2)
         w = m / (q * np.sqrt(q))
-        ref[a:a + chunk] = np.stack([(w * dx).sum(1), (w * dy).sum(1), (w * dz).sum(1)], axis=1)
+        ref[a] = ((w * dx).sum(), (w * dy).sum(), (w * dz).sum())
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        list(ex.map(one, range(nsample)))
     return idx, ref
 
 
-def direct_bench(rrg, ics, steps=3):
+def direct_bench(rrg, ics, steps=20):
+    """Config 3: `steps` steps of each force kind; the first captures the
+    stage graphs, the others are timed (with stage events, for the kernel
+    time); force errors at t=0 and the energy drift over the run."""
     pos, vel, m = ics.disk_bulge()
     n = len(m)
     s2 = np.full(n, 0.005 ** 2)
-    out, acc0, energy0 = {}, {}, {}
-    for force, k in (("mixed", steps), ("fp32", steps), ("fp64", 1), ("fp64_exact", 1)):
+    out, acc0, energy0, energy1 = {}, {}, {}, {}
+    k = steps - 1
+    for force in ("mixed", "fp32", "fp64", "fp64_exact"):
         p = rrg.make_params(dt=1e-3, force=force)
         sim = rrg.Simulation(p, pos, vel, m, s2)
         sim.precalc()
@@ -192,6 +203,7 @@ def direct_bench(rrg, ics, steps=3):
         sim.synchronize()
         t1 = time.perf_counter()
         tm = sim.timing()
+        energy1[force] = sim.conservation()
         sim.close()
         kern_s = tm["direct"][0] / 1e3 / k
         inter = float(n) * n
@@ -201,27 +213,30 @@ def direct_bench(rrg, ics, steps=3):
             "kernel_ms": kern_s * 1e3,
             "tflops_20_per_interaction": 20 * inter / kern_s / 1e12,
         }
-    idx, ref = _fp64_sample_forces(pos, m, s2)
+    idx, ref = _fp64_sample_forces(pos, m, s2, nsample=4096)
     e_ref = energy0["fp64_exact"]["Ek"] + energy0["fp64_exact"]["Ep"]
     for force in acc0:
         err = np.linalg.norm(acc0[force][idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
         e = energy0[force]["Ek"] + energy0[force]["Ep"]
+        e1 = energy1[force]["Ek"] + energy1[force]["Ep"]
         out[force]["force_rel_err_median"] = float(np.median(err))
         out[force]["force_rel_err_p99"] = float(np.percentile(err, 99))
         out[force]["force_rel_err_max"] = float(np.max(err))
         out[force]["energy_rel_err_vs_fp64_exact"] = abs(e - e_ref) / abs(e_ref)
+        out[force]["energy_drift"] = abs(e1 / e - 1.0)
     for f32 in ("mixed", "fp32"):
         for f64 in ("fp64", "fp64_exact"):
             out[f"{f32}_over_{f64}"] = (out[f32]["kernel_ginteractions_per_s"] /
                                         out[f64]["kernel_ginteractions_per_s"])
     out["config"] = {"workload": "disk N=196608 + Plummer bulge N=65536 (M=0.2, a=0.2), eps=0.005, dt=1e-3",
-                     "n": n, "timed_steps": {"mixed": steps, "fp32": steps, "fp64": 1, "fp64_exact": 1},
+                     "n": n, "steps": steps, "timed_steps": k,
                      "kinds": {"fp64": "FP64 pairs, device-library rsqrt", "fp64_exact": "FP64 pairs, IEEE 1./sqrt "
                                "(the reference kernel's arithmetic)", "mixed": "FP32 pairs on hi/lo positions + "
                                "two-sum accumulation", "fp32": "FP32 pairs on FP32 positions + two-sum accumulation"},
                      "force_error": f"|a - a_ref| / |a_ref| at t=0 on {len(idx)} sampled particles (seed 5); a_ref = "
                                     "FP64 direct sum over all N sources on the host (numpy)",
-                     "energy_error": "|E - E_ref| / |E_ref| of Ek + Ep at t=0, E_ref from the fp64_exact kernel"}
+                     "energy_error": "|E - E_ref| / |E_ref| of Ek + Ep at t=0, E_ref from the fp64_exact kernel",
+                     "energy_drift": f"|E(t_{steps}) / E(0) - 1| over the {steps} steps"}
     return out
 
 
