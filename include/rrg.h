@@ -130,9 +130,11 @@ typedef struct {
     int32_t use_graphs;     /* 1 (default): replay captured hipGraphs for steps */
     int32_t autotune;       /* 1: time candidate launch shapes at precalc and keep
                                the fastest (cached per process); default 0 */
-    int32_t wave_fused;     /* 1 (default): up to three stencil iterations per
-                               launch on periodic 64^3-512^3 grids (two for FP64
-                               512^3), bitwise the same results; 0: one */
+    int32_t wave_fused;     /* 1 (default): several stencil iterations per
+                               launch on periodic 64^3-512^3 grids (up to four
+                               at 128^3 and FP32 256^3, three elsewhere, two
+                               for FP64 512^3), bitwise the same results;
+                               0: one */
 } rrg_params;
 
 typedef struct {
@@ -264,8 +266,8 @@ int rrg_get_conservation_systems(rrg_ctx* ctx, int32_t k, rrg_conservation* out)
 int rrg_get_stats(rrg_ctx* ctx, rrg_stats* out);
 /* How a context of this real type, grid size and boundary type splits
  * `iterations` unchecked stencil iterations into launches (no counterpart in
- * the reference): launches[0..2] = launches running 1, 2 and 3 iterations;
- * `fused` is the wave_fused option. */
+ * the reference): launches[0..3] = launches running 1, 2, 3 and 4
+ * iterations; `fused` is the wave_fused option. */
 int rrg_wave_schedule(int32_t real_type, int32_t nx, int32_t boundary, int32_t iterations, int32_t fused,
                       int32_t* launches);
 /* Wait for all work queued on the context stream. */

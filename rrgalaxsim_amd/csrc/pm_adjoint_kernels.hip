@@ -410,7 +410,7 @@ void adj_wave(hipStream_t s, int nx, bool periodic, const WaveK<real>& k, real* 
 }
 
 template <class real> int adj_wave_levels(int nx, bool periodic, int remaining) {
-    return fused_levels<real>(nx, periodic ? RRG_BC_PERIODIC : RRG_BC_ZERO_DIRICHLET, remaining);
+    return fused_levels<real>(nx, periodic ? RRG_BC_PERIODIC : RRG_BC_ZERO_DIRICHLET, remaining, 3);
 }
 
 template <class real>

@@ -1790,7 +1790,7 @@ RRG_API int rrg_wave_schedule(int32_t real_type, int32_t nx, int32_t boundary, i
     return guarded([&] {
         require(launches != nullptr && nx >= 1 && iterations >= 0, "rrg_wave_schedule: bad arguments");
         require(real_type == RRG_REAL_F32 || real_type == RRG_REAL_F64, "rrg_wave_schedule: bad real_type");
-        launches[0] = launches[1] = launches[2] = 0;
+        launches[0] = launches[1] = launches[2] = launches[3] = 0;
         auto levels = [&](int r) {
             if (!fused) return 1;
             return real_type == RRG_REAL_F32 ? pm::wave_fused_levels<float>(pm::make_model<float>(nx, 1, 0, 0, 0, 1, 1,

@@ -13,7 +13,8 @@ namespace rrg {
 namespace pm {
 namespace {
 
-// K iterations per launch (K = 2, 3) on a periodic NX^3 grid, NX = 64 XJ RS.
+// K iterations per launch (K = 2, 3, 4; the adjoint at most 3) on a periodic
+// NX^3 grid, NX = 64 XJ RS.
 //
 // Forward (ADJ false): (phi^n, phi^{n-1}, src) -> (phi^{n+K-1}, phi^{n+K}),
 // every level with the arithmetic of k_wave, so one launch is bitwise K
@@ -103,6 +104,7 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     static_assert(K >= 2 && (NX & (NX - 1)) == 0 && NX % TY == 0 && NX % ZB == 0, "shape");
     static_assert(RS * ROWS <= 16, "a workgroup holds at most 16 waves");
     static_assert(!XC || RS == 1, "consecutive cells per lane need one wave per row");
+    static_assert(!ADJ || K <= 3, "the adjoint's s-bar order is written for at most three levels");
     __shared__ real lq[K - 1][2][ROWS][NX];
     __shared__ real l0[YL ? 2 : 1][YL ? ROWS : 1][NX];  // (YL) the input plane, for y neighbours
     int b = blockIdx.x;
@@ -307,22 +309,34 @@ void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* 
 // per iteration; one iteration per launch in brackets):
 //          FP32                                    FP64
 //   64^3   K=3 (1,1,4,4,1,0,1)  1.39 us (2.74)     K=3 (1,1,4,4,1,0,1)  2.07 us (3.32)
-//   128^3  K=3 (2,1,8,8,1,0,0)  3.50-3.68 (6.8)    K=3 (2,1,8,8,1,1,1)  5.71 (12.0)
-//   256^3  K=3 (4,1,8,32,1,1,1) 23.4 (38.0)        K=3 (2,2,4,64,1,0,0) 63.0 (115.6)
+//   128^3  K=4 (2,1,8,8,2,1,1)  3.20 (6.8)         K=4 (2,1,8,8,1,1,1)  5.15-5.40 (12.0)
+//   256^3  K=4 (4,1,8,32,1,1,1) 18.3 (38.0)        K=3 (2,2,4,64,1,0,0) 63.0 (115.6)
 //   512^3  K=3 (4,2,4,32,1,0,0) 218 (440)          K=2 (2,4,2,32,1,0,0) 790 (932); K=3
 //                                                  shapes spill registers
-// and for a remainder of two iterations the K=2 shapes below.  The LDS ring
-// of the input plane (YL) pays where a plane is large (256^3 FP32, 128^3
-// FP64) and costs 5-7% on the small FP32 grids; consecutive cells per lane
-// (XC) gain 11-15% at 128^3 FP64 and 2.5% at 256^3 FP32 and lose 2-4% at
-// 128^3 FP32.  The adjoint uses the same shapes.  Other grids, and
-// non-periodic ones, run one iteration per launch.
-template <class real> int fused_levels(int nx, int bc, int remaining) {
+// and for shorter remainders the K=3 and K=2 shapes below.  The LDS ring of
+// the input plane (YL) and consecutive cells per lane (XC) pay on the larger
+// planes and, together, at 128^3 FP32; at 128^3 FP32 K=3 with neither was
+// 3.5-3.8 us, with both 3.4, and K=4 with both and a two-plane prefetch 3.2.
+// K=5 gains nothing over K=4 (128^3 FP32 3.25 us, 256^3 FP32 19.8 us).  The
+// adjoint uses the K<=3 shapes.  Other grids, and non-periodic ones, run one
+// iteration per launch.
+template <class real> int fused_max(int nx, int bc) {
     constexpr bool f32 = std::is_same_v<real, float>;
-    if (bc != RRG_BC_PERIODIC || remaining < 2) return 1;
-    if (nx != 64 && nx != 128 && nx != 256 && nx != 512) return 1;
-    const bool k3 = nx != 512 || f32;
-    return remaining >= 3 && k3 ? 3 : 2;
+    if (bc != RRG_BC_PERIODIC) return 1;
+    if (nx == 128 || (nx == 256 && f32)) return 4;
+    if (nx == 64 || nx == 256 || (nx == 512 && f32)) return 3;
+    return nx == 512 ? 2 : 1;
+}
+
+// Iterations of the next launch of an unchecked stretch of `remaining`
+// iterations, at most `cap`.  Where four fit, a tail of five or six
+// iterations runs as 3+2 and 3+3 (cheaper than 4+1 and 4+2).
+template <class real> int fused_levels(int nx, int bc, int remaining, int cap = 4) {
+    const int m = fused_max<real>(nx, bc) < cap ? fused_max<real>(nx, bc) : cap;
+    if (m < 2 || remaining < 2) return 1;
+    if (m == 2 || remaining == 2) return 2;
+    if (m == 3 || remaining == 3 || remaining == 5 || remaining == 6) return 3;
+    return 4;
 }
 
 // One launch of `levels` iterations with the shape above; the caller checks
@@ -331,11 +345,24 @@ template <class real, bool ADJ, bool FIRST>
 void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const real* p, const real* po,
                   const real* src, real* sbar, const WaveK<real>& k) {
     constexpr bool f32 = std::is_same_v<real, float>;
+    if constexpr (!ADJ) {
+        if (levels == 4) {
+            if (nx == 128) {
+                if constexpr (f32)
+                    launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                else
+                    launch_shape<real, 4, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            } else if constexpr (f32) {
+                launch_shape<real, 4, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            }
+            return;
+        }
+    }
     if (levels == 3) {
         if (nx == 64) {
             launch_shape<real, 3, 1, 1, 4, 4, 1, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 128) {
-            launch_shape<real, 3, 2, 1, 8, 8, 1, !f32, !f32, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 2, 1, 8, 8, f32 ? 2 : 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if (nx == 256) {
             if constexpr (f32)
                 launch_shape<real, 3, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);

@@ -26,10 +26,10 @@ def default_params() -> L.Params:
 def wave_schedule(nx, iterations, real="fp32", boundary="periodic", fused=True) -> dict:
     """{iterations per launch: launches} for `iterations` unchecked stencil
     iterations of a context with this grid (the library's own rule)."""
-    out = (ctypes.c_int32 * 3)()
+    out = (ctypes.c_int32 * 4)()
     L.check(L.lib().rrg_wave_schedule({"fp64": L.REAL_F64, "fp32": L.REAL_F32}[real], int(nx), L.BOUNDARIES[boundary],
                                       int(iterations), 1 if fused else 0, out))
-    return {1: out[0], 2: out[1], 3: out[2]}
+    return {1: out[0], 2: out[1], 3: out[2], 4: out[3]}
 
 
 def make_params(*, dt, time_max=None, dt_save=None, grid=None, wave=None, real="fp64", force="fp64",

@@ -176,18 +176,23 @@ def test_ensemble_offsets_validated_before_device_work(rrg):
 
 
 @pytest.mark.parametrize("real,nx,boundary,iters,fused,expect", [
-    ("fp32", 128, "periodic", 200, True, {1: 0, 2: 1, 3: 66}),
-    ("fp32", 256, "periodic", 5, True, {1: 0, 2: 1, 3: 1}),
-    ("fp64", 512, "periodic", 7, True, {1: 1, 2: 3, 3: 0}),
-    ("fp64", 64, "periodic", 4, True, {1: 1, 2: 0, 3: 1}),
-    ("fp32", 96, "periodic", 10, True, {1: 10, 2: 0, 3: 0}),
-    ("fp32", 128, "zero_dirichlet", 10, True, {1: 10, 2: 0, 3: 0}),
-    ("fp32", 128, "periodic", 10, False, {1: 10, 2: 0, 3: 0}),
-    ("fp32", 128, "periodic", 0, True, {1: 0, 2: 0, 3: 0}),
+    ("fp32", 128, "periodic", 200, True, {1: 0, 2: 0, 3: 0, 4: 50}),
+    ("fp64", 128, "periodic", 9, True, {1: 0, 2: 1, 3: 1, 4: 1}),
+    ("fp32", 256, "periodic", 7, True, {1: 0, 2: 0, 3: 1, 4: 1}),
+    ("fp32", 256, "periodic", 6, True, {1: 0, 2: 0, 3: 2, 4: 0}),
+    ("fp32", 256, "periodic", 5, True, {1: 0, 2: 1, 3: 1, 4: 0}),
+    ("fp64", 256, "periodic", 200, True, {1: 0, 2: 1, 3: 66, 4: 0}),
+    ("fp32", 512, "periodic", 200, True, {1: 0, 2: 1, 3: 66, 4: 0}),
+    ("fp64", 512, "periodic", 7, True, {1: 1, 2: 3, 3: 0, 4: 0}),
+    ("fp64", 64, "periodic", 4, True, {1: 1, 2: 0, 3: 1, 4: 0}),
+    ("fp32", 96, "periodic", 10, True, {1: 10, 2: 0, 3: 0, 4: 0}),
+    ("fp32", 128, "zero_dirichlet", 10, True, {1: 10, 2: 0, 3: 0, 4: 0}),
+    ("fp32", 128, "periodic", 10, False, {1: 10, 2: 0, 3: 0, 4: 0}),
+    ("fp32", 128, "periodic", 0, True, {1: 0, 2: 0, 3: 0, 4: 0}),
 ])
 def test_wave_schedule(rrg, real, nx, boundary, iters, fused, expect):
     """The library's split of unchecked stencil iterations into launches of
-    three, two and one iterations (no device needed)."""
+    four, three, two and one iterations (no device needed)."""
     got = rrg.wave_schedule(nx, iters, real=real, boundary=boundary, fused=fused)
     assert got == expect
     assert sum(k * v for k, v in got.items()) == iters

@@ -1,5 +1,6 @@
-"""Several stencil iterations per launch (three, or two for a remainder of
-two; periodic 64^3 - 512^3, two at most for FP64 512^3): bitwise the same as the
+"""Several stencil iterations per launch (up to four at 128^3 and FP32 256^3,
+three on the other periodic grids, two at most for FP64 512^3; shorter
+launches for the remainders): bitwise the same as the
 oracle's sequence of wave_diss_iteration calls (reference wave.cu:106-193,
 solve_wave_equation wave.cu:230-297) for iteration counts with every
 remainder, inside full PM runs (graph-replayed steps, the steady-state check
@@ -27,7 +28,7 @@ def _fields(nx, dtype, seed):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("nx,iters", [(64, 1), (64, 2), (64, 3), (64, 7), (64, 8), (128, 8), (128, 6), (128, 4),
-                                     (256, 4), (256, 5)])
+                                     (128, 7), (128, 9), (256, 4), (256, 5), (256, 7)])
 def test_wave_fused_iterations_bitwise(rrg, orc, dtype, nx, iters):
     dx = 16.0 / nx
     prev, curr, mass = _fields(nx, dtype, nx + iters)
@@ -93,7 +94,7 @@ def test_pm_steady_check_fused_matches_oracle(rrg, orc):
 
 
 @pytest.mark.parametrize("graphs", [True, False])
-@pytest.mark.parametrize("nx,iters", [(128, 23), (256, 5)])
+@pytest.mark.parametrize("nx,iters", [(128, 23), (256, 5), (256, 9)])
 def test_fused_equals_single(rrg, graphs, nx, iters):
     """FP32 PM runs on the headline grid (23 = 7 x 3 + 2 iterations per step)
     and on 256^3 (5 = 3 + 2): fused launches on and off give the same bits."""
