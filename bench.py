@@ -40,9 +40,15 @@ C = 4574.337022617616
 # commands is profiles/wave_pm1e6_128_hbm_counters.txt.
 WAVE_TRAFFIC_128 = 34_647_301
 # The same for one three-iteration k_wave_fused<float, 3> launch, same method:
-# 17186.2 KiB fetched (x 1.9995) and 16384.0 KiB written per launch
-# (profiles/wave_fused_pm1e6_128_hbm_counters.txt).
+# 17186.2 KiB fetched (x 1.9995) and 16384.0 KiB written per launch (before
+# the 128^3 shape took four iterations per launch).
 WAVE_FUSED_TRAFFIC_128 = 51_965_754
+# One four-iteration k_wave_fused<float, 4> launch, same method: 17889.2 KiB
+# fetched (x 1.9995) and 16384.0 KiB written per launch
+# (profiles/wave_fused_pm1e6_128_hbm_counters.txt).
+WAVE_FUSED4_TRAFFIC_128 = 53_405_138
+# HBM bytes per launch by iterations per launch, for the roofline's traffic
+WAVE_TRAFFIC_BY_LEVELS_128 = {1: WAVE_TRAFFIC_128, 3: WAVE_FUSED_TRAFFIC_128, 4: WAVE_FUSED4_TRAFFIC_128}
 def wave_schedule(rrg, iters, nx, real="fp32", fused=True):
     """Launches of one unchecked solve of `iters` iterations on a periodic
     grid, by iterations per launch, as the library splits them."""
@@ -101,8 +107,8 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     avg_wave_s = wave_ms / 1e3 / max(wave_launches, 1)
     iters = PM["iterations"]
     sched = wave_schedule(rrg, iters, PM["nx"], real=real)
-    assert sched[3] + sched[2] + sched[1] == round(wave_launches / steps), (sched, wave_launches, steps)
-    fused = sched[3] + sched[2]
+    assert sum(sched.values()) == round(wave_launches / steps), (sched, wave_launches, steps)
+    fused = sched[4] + sched[3] + sched[2]
     n = PM["n"]
     # Algorithmic bytes, fp32, per unit of work of the PM step: the stencil
     # 16 B per cell-iteration (phi^n, phi^(n-1), source read, phi^(n+1)
@@ -136,8 +142,9 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
         "wave_avg_launch_us": avg_wave_s * 1e6,
         "wave_ms_per_step": wave_ms / steps,
         "wave_us_per_iteration": wave_ms * 1e3 / (iters * steps),
-        "wave_launches_per_step": {"three_iterations": sched[3], "two_iterations": sched[2], "single": sched[1]},
-        "wave_iterations_per_launch": 3 if sched[3] else (2 if sched[2] else 1),
+        "wave_launches_per_step": {"four_iterations": sched[4], "three_iterations": sched[3],
+                                   "two_iterations": sched[2], "single": sched[1]},
+        "wave_iterations_per_launch": max(k for k, v in sched.items() if v),
         "wave_fused_launch_bytes": 20 * ncell,
         "step_algorithmic_bytes": step_bytes,
         "step_own_minimum_bytes": own_bytes,
@@ -362,7 +369,7 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2):
             ms, launches = tm["wave"]
             sched = wave_schedule(rrg, PM["iterations"], nx, fused=fused)
             assert sum(sched.values()) * steps == launches, (nx, fused, sched, launches)
-            own = ((sched[3] + sched[2]) * 20.0 + sched[1] * 16.0) * nx ** 3 * steps
+            own = ((sched[4] + sched[3] + sched[2]) * 20.0 + sched[1] * 16.0) * nx ** 3 * steps
             alg = 16.0 * nx ** 3 * PM["iterations"] * steps
             row["fused" if fused else "single"] = {
                 "launches_per_step": {str(k): v for k, v in sched.items() if v}, "us_per_launch": ms * 1e3 / launches,
@@ -596,10 +603,10 @@ def main():
         "config": {"workload": "PM: exponential disk N=1e6 (seed 1), 128^3 grid on [-8,8]^3, CIC + 200 damped-wave "
                                "iterations + gradient + trilinear gather, dt=5e-3, fp32",
                    "n": PM["n"], "nx": PM["nx"], "wave_iterations": PM["iterations"]},
-        "roofline": {"bound": "hbm", "kernel": "k_wave_fused<float, 3>" if per_launch == 3 else "k_wave<float>",
+        "roofline": {"bound": "hbm",
+                     "kernel": f"k_wave_fused<float, {per_launch}>" if per_launch > 1 else "k_wave<float>",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (WAVE_FUSED_TRAFFIC_128 if per_launch == 3 else WAVE_TRAFFIC_128)
-                     if PM["nx"] == 128 else None,
+                     "traffic": WAVE_TRAFFIC_BY_LEVELS_128.get(per_launch) if PM["nx"] == 128 else None,
                      "algorithmic_bytes": 16 * cells * per_launch,
                      "per_unit": "16 B per cell-iteration (phi^n, phi^(n-1), source read; phi^(n+1) written), "
                                  f"{per_launch} iterations x {cells} cells per launch; the fused launch keeps the "
