@@ -311,8 +311,9 @@ void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* 
 //   64^3   K=3 (1,1,4,4,1,0,1)  1.39 us (2.74)     K=3 (1,1,4,4,1,0,1)  2.07 us (3.32)
 //   128^3  K=4 (2,1,8,8,2,1,1)  3.20 (6.8)         K=4 (2,1,8,8,1,1,1)  5.15-5.40 (12.0)
 //   256^3  K=4 (4,1,8,32,1,1,1) 18.3 (38.0)        K=3 (2,2,4,64,1,0,0) 63.0 (115.6)
-//   512^3  K=3 (4,2,4,32,1,0,0) 218 (440)          K=2 (2,4,2,32,1,0,0) 790 (932); K=3
-//                                                  shapes spill registers
+//   512^3  K=3 (8,1,4,32,1,1,1) 202 (440)          K=2 (2,4,2,32,1,0,0) 790 (932); K=3
+//                                                  and one-wave-per-row shapes
+//                                                  spill registers
 // and for shorter remainders the K=3 and K=2 shapes below.  The LDS ring of
 // the input plane (YL) and consecutive cells per lane (XC) pay on the larger
 // planes and, together, at 128^3 FP32; at 128^3 FP32 K=3 with neither was
@@ -369,7 +370,7 @@ void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const r
             else
                 launch_shape<real, 3, 2, 2, 4, 64, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if constexpr (f32) {
-            launch_shape<real, 3, 4, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 8, 1, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         }
     } else {
         if (nx == 64) {
@@ -389,7 +390,7 @@ void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const r
                 launch_shape<real, 2, 2, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else {
             if constexpr (f32)
-                launch_shape<real, 2, 4, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 2, 8, 1, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
             else
                 launch_shape<real, 2, 2, 4, 2, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         }

@@ -177,7 +177,7 @@ def test_pm_adjoint_fused_128_matches_oracle(rrg, orc, iters, setup):
     assert rel(gx0, ox0) < 1e-9 and rel(gv0, ov0) < 1e-9 and rel(gm, om) < 1e-9
 
 
-@pytest.mark.parametrize("nx", [64, 128, 256])
+@pytest.mark.parametrize("nx", [64, 128, 256, 512])
 def test_pm_adjoint_fused_equals_single_fp32(rrg, nx):
     """FP32 transposed stencil at every fused FP32 shape (three- and
     two-iteration launches): forward states bitwise equal to one iteration
