@@ -35,8 +35,9 @@ namespace {
 // takes x neighbours from registers: with XJ consecutive cells per lane (XC)
 // they are the lane's own cells and one DPP wave rotate at each end, and rows
 // move with wide loads and stores; with XJ chunks of 64 lanes, a DPP wave
-// rotate and a lane select per chunk.  The LDS ring then only serves y
-// neighbours.  The input's y neighbours come from the
+// rotate and a lane select per chunk.  With several waves per row and XC
+// (and YL), only the two cells beyond a wave's segment come from the LDS
+// rows; otherwise all x neighbours do, and the input's from the cache.  The input's y neighbours come from the
 // cache, or (YL) from an LDS ring of the input plane for all but the two edge
 // row waves.  Loads for the next plane are issued PD planes ahead; the barrier
 // only waits for LDS traffic, so they stay in flight across it.  HBM traffic
@@ -60,14 +61,17 @@ template <int CTRL> __device__ __forceinline__ double wave_rot(double v) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// The x-1 and x+1 neighbours of a periodic x row held by one wave.  XC: XJ
-// consecutive cells per lane (x = XJ lane + j): the lane's own cells, and one
-// wave rotate at each end (lane 0's x-1 is lane 63's last cell, x = NX-1;
-// lane 63's x+1 is lane 0's first, x = 0).  Otherwise XJ chunks of 64 lanes
-// (x = 64 j + lane): each chunk rotated, the end lanes taking the
-// neighbouring chunk's.
-template <class real, int XJ, bool XC>
-__device__ __forceinline__ void row_neighbours(const real (&r)[XJ], real (&m)[XJ], real (&p)[XJ], int lane) {
+// The x-1 and x+1 neighbours of the cells a wave holds of a periodic x row.
+// XC: XJ consecutive cells per lane (x = x0 + XJ lane + j): the lane's own
+// cells, and one wave rotate at each end; with one wave per row (RS = 1)
+// lane 0's x-1 is lane 63's last cell (x = NX-1) and lane 63's x+1 lane 0's
+// first (x = 0), with RS waves per row those two cells belong to the
+// neighbouring segments and come from `row`, the whole row in LDS, at x =
+// xlo and xhi.  Otherwise (RS = 1 only) XJ chunks of 64 lanes (x = 64 j +
+// lane): each chunk rotated, the end lanes taking the neighbouring chunk's.
+template <class real, int XJ, bool XC, int RS>
+__device__ __forceinline__ void row_neighbours(const real (&r)[XJ], real (&m)[XJ], real (&p)[XJ], int lane,
+                                               const real* row, unsigned xlo, unsigned xhi) {
     if constexpr (XC) {
         m[0] = wave_rot<0x13C>(r[XJ - 1]);
 #pragma unroll
@@ -75,7 +79,13 @@ __device__ __forceinline__ void row_neighbours(const real (&r)[XJ], real (&m)[XJ
 #pragma unroll
         for (int j = 0; j < XJ - 1; ++j) p[j] = r[j + 1];
         p[XJ - 1] = wave_rot<0x134>(r[0]);
+        if constexpr (RS > 1) {
+            const real lo = row[xlo], hi = row[xhi];
+            if (lane == 0) m[0] = lo;
+            if (lane == 63) p[XJ - 1] = hi;
+        }
     } else {
+        static_assert(RS == 1, "chunked rows take their x neighbours from registers with one wave per row only");
         real rr[XJ], rl[XJ];
 #pragma unroll
         for (int j = 0; j < XJ; ++j) {
@@ -103,7 +113,7 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     constexpr int S = ZB + 2 * K - 2;  // level-1 planes Z0-(K-1) .. Z0+ZB+K-2
     static_assert(K >= 2 && (NX & (NX - 1)) == 0 && NX % TY == 0 && NX % ZB == 0, "shape");
     static_assert(RS * ROWS <= 16, "a workgroup holds at most 16 waves");
-    static_assert(!XC || RS == 1, "consecutive cells per lane need one wave per row");
+    static_assert(!XC || RS == 1 || YL, "segment ends of several waves per row come from the input plane in LDS");
     static_assert(!ADJ || K <= 3, "the adjoint's s-bar order is written for at most three levels");
     __shared__ real lq[K - 1][2][ROWS][NX];
     __shared__ real l0[YL ? 2 : 1][YL ? ROWS : 1][NX];  // (YL) the input plane, for y neighbours
@@ -123,10 +133,13 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     unsigned x[XJ], xm[XJ], xp[XJ];
 #pragma unroll
     for (int j = 0; j < XJ; ++j) {
-        x[j] = XC ? unsigned(XJ * l + j) : unsigned(seg * 64 * XJ + 64 * j + l);
+        x[j] = XC ? unsigned(seg * 64 * XJ + XJ * l + j) : unsigned(seg * 64 * XJ + 64 * j + l);
         xm[j] = (x[j] - 1u) & unsigned(NX - 1);
         xp[j] = (x[j] + 1u) & unsigned(NX - 1);
     }
+    // the cells beyond this wave's segment of the row (several waves per row)
+    const unsigned xlo = unsigned(seg * 64 * XJ - 1) & unsigned(NX - 1);
+    const unsigned xhi = unsigned((seg + 1) * 64 * XJ) & unsigned(NX - 1);
     // offset of row yy of plane z (z wraps)
     auto row = [](int z, int yy) { return size_t(unsigned((z + NX) & (NX - 1)) * unsigned(NX) + unsigned(yy)) * NX; };
     const int zs = Z0 - (K - 1);  // level-1 plane of step 0
@@ -139,7 +152,7 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
         const real* __restrict__ rp = p + row(zq, yp);
         const real* __restrict__ r2 = p + row(zq + 2, y);
         const real* __restrict__ ro = po + row(zq, y);
-        if constexpr (RS > 1) {
+        if constexpr (RS > 1 && !XC) {
 #pragma unroll
             for (int j = 0; j < XJ; ++j) {
                 in.a[j] = r0[xm[j]];
@@ -214,7 +227,11 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
 #pragma unroll
             for (int j = 0; j < XJ; ++j) l0[s & 1][w][x[j]] = Pv[0][j];
         }
-        if constexpr (RS == 1) row_neighbours<real, XJ, XC>(Pv[1], in.a, in.b, l);
+        if constexpr (RS == 1 || XC) {
+            const real* in_row = nullptr;  // the input plane's row w (YL), for segment ends
+            if constexpr (YL) in_row = &l0[(s - 1) & 1][w][0];
+            row_neighbours<real, XJ, XC, RS>(Pv[1], in.a, in.b, l, in_row, xlo, xhi);
+        }
         real nw[K + 1][XJ];
 #pragma unroll
         for (int j = 0; j < XJ; ++j) {
@@ -236,8 +253,8 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
                 const int sl = (s - 1) & 1;
                 // x neighbours of level L-1 on this row: its previous step
                 real bm[XJ], bp[XJ];
-                if constexpr (RS == 1) {
-                    row_neighbours<real, XJ, XC>(h1[L - 1], bm, bp, l);
+                if constexpr (RS == 1 || XC) {
+                    row_neighbours<real, XJ, XC, RS>(h1[L - 1], bm, bp, l, &lq[L - 2][sl][w][0], xlo, xhi);
                 } else {
 #pragma unroll
                     for (int j = 0; j < XJ; ++j) {
@@ -310,10 +327,9 @@ void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* 
 //          FP32                                    FP64
 //   64^3   K=3 (1,1,4,4,1,0,1)  1.39 us (2.74)     K=3 (1,1,4,4,1,0,1)  2.07 us (3.32)
 //   128^3  K=4 (2,1,8,8,2,1,1)  3.20 (6.8)         K=4 (2,1,8,8,1,1,1)  5.15-5.40 (12.0)
-//   256^3  K=4 (4,1,8,32,1,1,1) 18.3 (38.0)        K=3 (2,2,4,64,1,0,0) 63.0 (115.6)
-//   512^3  K=3 (8,1,4,32,1,1,1) 202 (440)          K=2 (2,4,2,32,1,0,0) 790 (932); K=3
-//                                                  and one-wave-per-row shapes
-//                                                  spill registers
+//   256^3  K=4 (4,1,8,32,1,1,1) 18.3 (38.0)        K=3 (2,2,4,64,1,1,1) 45.5 (115.6)
+//   512^3  K=3 (4,2,4,32,1,1,1) 200-202 (440)      K=2 (4,2,4,32,1,1,1) 578 (932); K=3
+//                                                  shapes spill registers
 // and for shorter remainders the K=3 and K=2 shapes below.  The LDS ring of
 // the input plane (YL) and consecutive cells per lane (XC) pay on the larger
 // planes and, together, at 128^3 FP32; at 128^3 FP32 K=3 with neither was
@@ -368,9 +384,9 @@ void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const r
             if constexpr (f32)
                 launch_shape<real, 3, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
             else
-                launch_shape<real, 3, 2, 2, 4, 64, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 3, 2, 2, 4, 64, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         } else if constexpr (f32) {
-            launch_shape<real, 3, 8, 1, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         }
     } else {
         if (nx == 64) {
@@ -392,7 +408,7 @@ void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const r
             if constexpr (f32)
                 launch_shape<real, 2, 8, 1, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
             else
-                launch_shape<real, 2, 2, 4, 2, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 2, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
         }
     }
 }

@@ -177,12 +177,13 @@ def test_pm_adjoint_fused_128_matches_oracle(rrg, orc, iters, setup):
     assert rel(gx0, ox0) < 1e-9 and rel(gv0, ov0) < 1e-9 and rel(gm, om) < 1e-9
 
 
-@pytest.mark.parametrize("nx", [64, 128, 256, 512])
-def test_pm_adjoint_fused_equals_single_fp32(rrg, nx):
-    """FP32 transposed stencil at every fused FP32 shape (three- and
-    two-iteration launches): forward states bitwise equal to one iteration
-    per launch, gradients equal up to the ordering of the adjoint's FP64
-    atomic sums."""
+@pytest.mark.parametrize("nx,real", [(64, "fp32"), (128, "fp32"), (256, "fp32"), (512, "fp32"), (256, "fp64"),
+                                     (512, "fp64")])
+def test_pm_adjoint_fused_equals_single_large(rrg, nx, real):
+    """The transposed stencil at the fused FP32 shapes and the FP64 shapes with
+    several waves per row (three- and two-iteration launches): forward states
+    bitwise equal to one iteration per launch, gradients equal up to the
+    ordering of the adjoint's FP64 atomic sums."""
     n = 4000
     pos, vel, m = ics.exp_disk(n, seed=3)
     rng = np.random.default_rng(4)
@@ -190,7 +191,7 @@ def test_pm_adjoint_fused_equals_single_fp32(rrg, nx):
     wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=7, iterations=5, tolerance=0.0)
     out = []
     for fused in (True, False):
-        p = rrg.make_params(dt=0.02, grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real="fp32", wave_fused=fused)
+        p = rrg.make_params(dt=0.02, grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real=real, wave_fused=fused)
         with rrg.Simulation(p, pos, vel, m) as sim:
             sim.precalc()
             sim.adjoint_begin(2)
@@ -199,5 +200,6 @@ def test_pm_adjoint_fused_equals_single_fp32(rrg, nx):
             out.append((st["pos"], st["vel"]) + tuple(sim.adjoint(gx, gv)))
     a, b = out
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    tol = 1e-5 if real == "fp32" else 1e-12
     for ga, gb in zip(a[2:], b[2:]):
-        assert np.max(np.abs(ga - gb)) <= 1e-5 * np.max(np.abs(gb)), (nx, np.max(np.abs(ga - gb)) / np.max(np.abs(gb)))
+        assert np.max(np.abs(ga - gb)) <= tol * np.max(np.abs(gb)), (nx, real, np.max(np.abs(ga - gb)) / np.max(np.abs(gb)))
