@@ -550,33 +550,47 @@ __global__ __launch_bounds__(256) void k_com_sum(PState<real> p, double scale_x,
 }
 
 // ----------------------------------------------------------------- gradient
-template <class real>
+// Each thread marches ZB planes of one (x, y) column, keeping phi at z-1, z
+// and z+1 in registers; the x and y neighbours come from the cache.  Every
+// cell uses calc_acceleration_field_'s differences (one-sided on the faces).
+template <class real, int ZB>
 __global__ __launch_bounds__(256) void k_gradient(int NX, real coef, const real* __restrict__ phi,
                                                   typename Vec4<real>::T* __restrict__ acc) {
     int bx, by, bz;
     xcd_block(gridDim.x, gridDim.y, gridDim.z, bx, by, bz);
     const int ix = bx * 64 + threadIdx.x;
     const int iy = by * 4 + threadIdx.y;
-    const int iz = bz;
-    if (ix >= NX || iy >= NX) return;
+    const int z0 = bz * ZB;
+    if (ix >= NX || iy >= NX || z0 >= NX) return;
     const size_t sy = size_t(NX), sz = size_t(NX) * NX;
-    const size_t c = size_t(ix) + iy * sy + iz * sz;
-    real d[3];
-    if (ix == 0) d[0] = phi[c + 1] - phi[c];
-    else if (ix == NX - 1) d[0] = phi[c] - phi[c - 1];
-    else d[0] = (real)(0.5 * (double)(phi[c + 1] - phi[c - 1]));
-    if (iy == 0) d[1] = phi[c + sy] - phi[c];
-    else if (iy == NX - 1) d[1] = phi[c] - phi[c - sy];
-    else d[1] = (real)(0.5 * (double)(phi[c + sy] - phi[c - sy]));
-    if (iz == 0) d[2] = phi[c + sz] - phi[c];
-    else if (iz == NX - 1) d[2] = phi[c] - phi[c - sz];
-    else d[2] = (real)(0.5 * (double)(phi[c + sz] - phi[c - sz]));
-    typename Vec4<real>::T o;
-    o.x = d[0] * coef;
-    o.y = d[1] * coef;
-    o.z = d[2] * coef;
-    o.w = 0;
-    acc[c] = o;
+    const size_t col = size_t(ix) + iy * sy;
+    real pm = z0 > 0 ? phi[col + (z0 - 1) * sz] : real(0);
+    real pc = phi[col + z0 * sz];
+#pragma unroll 2
+    for (int t = 0; t < ZB; ++t) {
+        const int iz = z0 + t;
+        if (iz >= NX) break;
+        const size_t c = col + iz * sz;
+        const real pp = iz + 1 < NX ? phi[c + sz] : real(0);
+        real d[3];
+        if (ix == 0) d[0] = phi[c + 1] - pc;
+        else if (ix == NX - 1) d[0] = pc - phi[c - 1];
+        else d[0] = (real)(0.5 * (double)(phi[c + 1] - phi[c - 1]));
+        if (iy == 0) d[1] = phi[c + sy] - pc;
+        else if (iy == NX - 1) d[1] = pc - phi[c - sy];
+        else d[1] = (real)(0.5 * (double)(phi[c + sy] - phi[c - sy]));
+        if (iz == 0) d[2] = pp - pc;
+        else if (iz == NX - 1) d[2] = pc - pm;
+        else d[2] = (real)(0.5 * (double)(pp - pm));
+        typename Vec4<real>::T o;
+        o.x = d[0] * coef;
+        o.y = d[1] * coef;
+        o.z = d[2] * coef;
+        o.w = 0;
+        acc[c] = o;
+        pm = pc;
+        pc = pp;
+    }
 }
 
 // ------------------------------------------------------------ gather + kick
@@ -893,8 +907,15 @@ void cell_coefficients(hipStream_t s, const WaveModel<real>& m, real* km, real* 
 
 template <class real> void gradient(hipStream_t s, int nx, real dx, const real* phi, real* acc4) {
     const real coef = (real)(-1. / (double)dx);  // rm_p2mesh.cu:284
-    k_gradient<real><<<grid3d(nx), dim3(64, 4, 1), 0, s>>>(nx, coef, phi,
-                                                           reinterpret_cast<typename Vec4<real>::T*>(acc4));
+    auto* out = reinterpret_cast<typename Vec4<real>::T*>(acc4);
+    const dim3 blk(64, 4, 1);
+    if (nx >= 128) {
+        const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned((nx + 7) / 8));
+        k_gradient<real, 8><<<grid, blk, 0, s>>>(nx, coef, phi, out);
+    } else {
+        const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned((nx + 1) / 2));
+        k_gradient<real, 2><<<grid, blk, 0, s>>>(nx, coef, phi, out);
+    }
     RRG_LAUNCH_CHECK();
 }
 
