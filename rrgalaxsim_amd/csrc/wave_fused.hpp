@@ -28,23 +28,26 @@ namespace {
 // row groups of RS waves (XJ cells per lane), rows Y0-(K-1) .. Y0+TY+K-2.
 // Level 1 runs on all of them, marching z from Z0-(K-1); level L runs one
 // plane behind level L-1 on the rows with a (K-L)-row halo, reading level
-// L-1's x/y neighbours from a two-plane LDS ring and its z neighbours from
+// L-1's y neighbours from a two-plane LDS ring and its z neighbours from
 // registers (the last three planes of every level stay in registers; level
 // L's "old" term is level L-2 two planes back, the input's own column for
-// L = 2).  With one wave per row (RS = 1) the wave holds the whole x row and
-// takes x neighbours from registers: with XJ consecutive cells per lane (XC)
-// they are the lane's own cells and one DPP wave rotate at each end, and rows
-// move with wide loads and stores; with XJ chunks of 64 lanes, a DPP wave
-// rotate and a lane select per chunk.  With several waves per row and XC
-// (and YL), only the two cells beyond a wave's segment come from the LDS
-// rows; otherwise all x neighbours do, and the input's from the cache.  The input's y neighbours come from the
-// cache, or (YL) from an LDS ring of the input plane for all but the two edge
-// row waves.  Loads for the next plane are issued PD planes ahead; the barrier
-// only waits for LDS traffic, so they stay in flight across it.  HBM traffic
-// per cell: 3 reals
-// read and 2 written for K forward iterations (4K for K single launches),
-// 3 read and 3 written for K adjoint iterations (5K), and one launch
-// instead of K.
+// L = 2).
+//
+// x neighbours: with one wave per row (RS = 1) the wave holds the whole x
+// row and takes them from registers: with XJ consecutive cells per lane
+// (XC) they are the lane's own cells and one DPP wave rotate at each end, and
+// rows move with wide loads and stores; with XJ chunks of 64 lanes, a DPP
+// wave rotate and a lane select per chunk.  With several waves per row and
+// XC (which needs YL), only the two cells beyond a wave's segment come from
+// the LDS rows.  Otherwise the levels' x neighbours come from the LDS ring
+// and the input's from the cache.
+//
+// The input's y neighbours come from the cache, or (YL) from an LDS ring of
+// the input plane for all but the two edge row waves.  Loads for the next
+// plane are issued PD planes ahead; the barrier only waits for LDS traffic,
+// so they stay in flight across it.  HBM traffic per cell: 3 reals read and
+// 2 written for K forward iterations (4K for K single launches), 3 read and
+// 3 written for K adjoint iterations (5K), and one launch instead of K.
 template <class real, int XJ> struct PlaneIn {
     real a[XJ], b[XJ], c[XJ], d[XJ], po[XJ], src[XJ], pn[XJ], sb[XJ];
 };
