@@ -342,14 +342,15 @@ def adjoint_bench(rrg, ics, n=65_536, steps=200, every=20):
                                    "loss = mean |x_T - x_T(target masses)|^2", "n": n, "steps": steps}}
 
 
-def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2):
+def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2, real="fp32"):
     """The stencil across grid sizes: the headline disk on 64^3 - 512^3 grids
-    over the same [-8,8]^3 box (20 setup iterations, 200 per step, fp32),
-    with several iterations per launch where the library has a launch shape
-    and with one per launch; time per iteration and per launch from the stage
-    events; rates of 16 B per cell-iteration (the stencil's algorithmic
-    bytes) and of the launches' least traffic (fused: 20 B/cell, single:
-    16 B/cell) over the stage time."""
+    over the same [-8,8]^3 box (20 setup iterations, 200 per step, `real`
+    state), with several iterations per launch where the library has a launch
+    shape and with one per launch; time per iteration and per launch from the
+    stage events; rates of 4 reals per cell-iteration (the stencil's
+    algorithmic bytes: 16 B in FP32, 32 B in FP64) and of the launches' least
+    traffic (fused: 5 reals per cell, single: 4) over the stage time."""
+    rb = 4 if real == "fp32" else 8
     pos, vel, m = ics.exp_disk(PM["n"], seed=1)
     out = {}
     for nx in sizes:
@@ -357,7 +358,7 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2):
         for fused in (True, False):
             wave = dict(diss_base=PM["diss_base"], diss_extra=PM["diss_extra"], wave_speed=C, setup_iterations=20,
                         iterations=PM["iterations"], tolerance=0.0)
-            p = rrg.make_params(dt=PM["dt"], grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real="fp32", wave_fused=fused)
+            p = rrg.make_params(dt=PM["dt"], grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real=real, wave_fused=fused)
             with rrg.Simulation(p, pos, vel, m) as sim:
                 sim.precalc()
                 sim.set_timing(True)
@@ -367,10 +368,10 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2):
                 sim.calc_step(steps)
                 tm = sim.timing()
             ms, launches = tm["wave"]
-            sched = wave_schedule(rrg, PM["iterations"], nx, fused=fused)
+            sched = wave_schedule(rrg, PM["iterations"], nx, real=real, fused=fused)
             assert sum(sched.values()) * steps == launches, (nx, fused, sched, launches)
-            own = ((sched[4] + sched[3] + sched[2]) * 20.0 + sched[1] * 16.0) * nx ** 3 * steps
-            alg = 16.0 * nx ** 3 * PM["iterations"] * steps
+            own = ((sched[4] + sched[3] + sched[2]) * 5.0 + sched[1] * 4.0) * rb * nx ** 3 * steps
+            alg = 4.0 * rb * nx ** 3 * PM["iterations"] * steps
             row["fused" if fused else "single"] = {
                 "launches_per_step": {str(k): v for k, v in sched.items() if v}, "us_per_launch": ms * 1e3 / launches,
                 "us_per_iteration": ms * 1e3 / (PM["iterations"] * steps),
@@ -629,6 +630,7 @@ def main():
     if not args.no_direct:
         legs = [("direct", "direct_sum", lambda: direct_bench(rrg, ics)), ("c1", "direct_sum_c1", lambda: c1_bench(rrg, ics)),
                 ("scaling", "wave_scaling", lambda: wave_scaling(rrg, ics)),
+                ("scaling", "wave_scaling_fp64", lambda: wave_scaling(rrg, ics, real="fp64")),
                 ("ensemble", "ensemble", lambda: ensemble_bench(rrg, ics)),
                 ("adjoint", "adjoint", lambda: adjoint_bench(rrg, ics)),
                 ("adjoint_pm", "adjoint_pm", lambda: pm_adjoint_bench(rrg, ics)),
