@@ -93,6 +93,22 @@ def test_pm_steady_check_fused_matches_oracle(rrg, orc):
     assert np.array_equal(st["pos"], o["pos"]) and np.array_equal(gr["phi"], o["phi"])
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("check", [5, 9])
+def test_pm_steady_check_fused_128_matches_oracle(rrg, orc, dtype, check):
+    """128^3: the unchecked stretches between checks run four-iteration
+    launches (4, and 4 + 4), then the single checking iteration; the early
+    exit and the states are the oracle's, bitwise."""
+    (pos, vel, m), st, gr, stats = _run(rrg, dtype, nx=128, iters=60, setup=400, tol=2e-3, check=check, hold=2,
+                                        steps=2)
+    o = orc.pm_run(pos, vel, m, 2, nx=128, dx=0.125, dt=5e-3, diss_base=0.785, wave_speed=C, setup_iterations=400,
+                   iterations=60, tolerance=2e-3, check_step=check, hold=2, fixed_e=stats["fixed_exponent"],
+                   dtype=dtype)
+    assert o["iterations"][0] < 400
+    assert stats["last_wave_iterations"] == int(o["iterations"][-1])
+    assert np.array_equal(st["pos"].astype(dtype), o["pos"]) and np.array_equal(gr["phi"].astype(dtype), o["phi"])
+
+
 @pytest.mark.parametrize("graphs", [True, False])
 @pytest.mark.parametrize("nx,iters", [(128, 23), (256, 5), (256, 9)])
 def test_fused_equals_single(rrg, graphs, nx, iters):
