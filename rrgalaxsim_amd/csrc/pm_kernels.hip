@@ -80,9 +80,14 @@ __device__ __forceinline__ unsigned cic_fixed(const real r[3], real m, const Geo
 // Predictor fused with the deposit.  Lanes of a wave whose particles share a
 // base cell (runs of equal keys; the particle arrays are kept sorted by cell,
 // so dense regions form long runs) are merged by a segmented inclusive scan
-// over the wave before the last lane of each run issues the 8 atomics.
+// over the wave; the last lane of each run then adds its 8 corner sums into a
+// workgroup table in LDS when its base cell is interior and at most
+// kDepositWindow-1 cells above the workgroup's smallest key (corner rows
+// (dy, dz) x offsets 0 .. kDepositWindow), and the workgroup issues one
+// global atomic per nonzero entry; other runs issue their 8 atomics directly.
 // Integer addition is exact, so the grid does not depend on the merging.
 constexpr unsigned kNoCell = 0xffffffffu;
+constexpr int kDepositWindow = 256;
 
 // Lane values moved by a DPP pattern (row_shr:n = 0x110+n within rows of 16
 // lanes, row_bcast:15 = 0x142, row_bcast:31 = 0x143); lanes without a source
@@ -111,10 +116,14 @@ template <int CTRL, int ROWS> __device__ __forceinline__ void seg_step(long long
 template <class real>
 __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt, int do_predict, Geo<real> g,
                                                          double scale, unsigned long long* __restrict__ mfix) {
+    __shared__ unsigned long long tab[4][kDepositWindow + 1];
+    __shared__ unsigned kmin_s;
+    for (int e = threadIdx.x; e < 4 * (kDepositWindow + 1); e += blockDim.x) (&tab[0][0])[e] = 0ull;
+    if (threadIdx.x == 0) kmin_s = kNoCell;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t n = p.n;
-    unsigned key = kNoCell;
+    unsigned key = kNoCell;  // lanes past n keep it: every thread reaches the barriers below
     long long q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     unsigned c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (i < n) {
@@ -137,8 +146,11 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
     // cells become adjacent lanes even where the particle order has drifted
     // since the last global sort (without it the deposit takes ~3x longer
     // on the headline disk); each lane then takes the contributions of the
-    // particle it was sorted to.
-    {
+    // particle it was sorted to.  Ties keep lane order, so a wave whose keys
+    // are already in order (most of them just after a global sort) would be
+    // sorted to itself and skips it.
+    const unsigned kin = __shfl_up(key, 1, 64);
+    if (!__all(lane == 0 || kin <= key)) {
         unsigned sk = key;
         int src = lane;
 #pragma unroll
@@ -163,6 +175,8 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
         }
         key = sk;
     }
+    __syncthreads();                          // table cleared, kmin_s set
+    if (lane == 0) atomicMin(&kmin_s, key);  // the wave's smallest key, its keys being in order
     const unsigned kprev = __shfl_up(key, 1, 64);
     const unsigned knext = __shfl_down(key, 1, 64);
     int f = (lane == 0 || kprev != key) ? 1 : 0;  // run head
@@ -174,10 +188,32 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
     seg_step<0x118, 0xf>(q, f);
     seg_step<0x142, 0xa>(q, f);
     seg_step<0x143, 0xc>(q, f);
+    __syncthreads();  // kmin_s final
+    const unsigned kmin = kmin_s;
+    const unsigned sy = unsigned(g.nx), sz = unsigned(g.nx) * unsigned(g.nx);
     if (tail && key != kNoCell) {
+        const unsigned off = key - kmin;  // key >= kmin
+        const bool interior = c[1] == c[0] + 1u && c[2] == c[0] + sy && c[4] == c[0] + sz;
+        if (interior && off < unsigned(kDepositWindow)) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (q[k] != 0) atomicAdd(mfix + c[k], (unsigned long long)q[k]);
+            for (int k = 0; k < 8; ++k)  // corner k: x offset k&1, row (dy, dz) = k>>1
+                if (q[k] != 0) atomicAdd(&tab[k >> 1][off + (k & 1)], (unsigned long long)q[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (q[k] != 0) atomicAdd(mfix + c[k], (unsigned long long)q[k]);
+        }
+    }
+    __syncthreads();
+    if (kmin != kNoCell) {
+        for (int e = threadIdx.x; e < 4 * (kDepositWindow + 1); e += blockDim.x) {
+            const unsigned long long v = (&tab[0][0])[e];
+            if (v != 0ull) {
+                const unsigned row = unsigned(e) / unsigned(kDepositWindow + 1);
+                const unsigned t = unsigned(e) - row * unsigned(kDepositWindow + 1);
+                atomicAdd(mfix + (kmin + t + (row & 1u) * sy + (row >> 1) * sz), v);
+            }
+        }
     }
 }
 
