@@ -335,6 +335,22 @@ int rrg_op_ngp_bin(int32_t real_type, int64_t n, const void* pos, int32_t nx, do
 int rrg_op_steady_err(int32_t real_type, int64_t ncell, const void* u_new, const void* u_old,
                       double* err);
 
+/* rrg_op_direct_accel without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipStream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * pos (n,3), mass (n), soft2 (n) and acc (n,3) are FP64 device arrays of the
+ * current device; the work is queued on `stream` (a hipStream_t, NULL = the
+ * null stream) and the call returns without waiting for it.  The result is
+ * bitwise that of rrg_op_direct_accel.  The working storage (source records,
+ * per-slice partial sums) lives in a fixed 64 MiB device area that the
+ * library holds on every device it runs on until the process ends, so n is
+ * limited to what fits: every n <= 65535 for every force kind, and every
+ * n <= 250000 for the FP64 kinds; a larger n returns RRG_ERR_INVALID with the
+ * bytes it would need.  Calls share that area per device: calls whose work overlaps
+ * on one device (other streams, other threads) race, so serialise them. */
+int rrg_op_direct_accel_noalloc(int32_t force_kind, int64_t n, const double* pos, const double* mass,
+                                const double* soft2, double* acc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

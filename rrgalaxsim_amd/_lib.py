@@ -126,6 +126,7 @@ SIGNATURES = {
     "rrg_op_gather": (ctypes.c_int, [_i32, _i64, _vp, _i32, _f64, _f64, _vp, _vp]),
     "rrg_op_ngp_bin": (ctypes.c_int, [_i32, _i64, _vp, _i32, _f64, _f64, _vp, _vp, _vp, _vp]),
     "rrg_op_steady_err": (ctypes.c_int, [_i32, _i64, _vp, _vp, ctypes.POINTER(_f64)]),
+    "rrg_op_direct_accel_noalloc": (ctypes.c_int, [_i32, _i64, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -2124,3 +2124,86 @@ RRG_API int rrg_op_steady_err(int32_t real_type, int64_t ncell, const void* u_ne
         else steady_op<float>(ncell, u_new, u_old, err);
     });
 }
+
+// ------------------------------------------------ allocation-free operator
+// rrg_op_direct_accel for callers that cannot allocate device memory (code
+// under stream capture, serving loops): device arrays in and out, the
+// caller's stream, and a fixed scratch area that lives in a file-scope device
+// array for the life of the process.  The array's name in host code is a
+// host-side stand-in; its device address on each device comes from
+// hipGetSymbolAddress, taken once per device with that device current, and
+// only that address is handed to kernels.
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;  // per device
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on device `dev` (the current device).
+unsigned char* scratch_address(int dev) {
+    require(dev >= 0 && dev < kMaxScratchDevices, "rrg_op_direct_accel_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        RRG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "rrg_op_direct_accel_noalloc: no device address for the scratch area");
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// The evaluation of direct_op with the same launch plan and kernels, so the
+// result is bitwise the same.  Softening is always taken per pair:
+// h_i + h_j with h = 0.5 * soft2 in the kernel's precision rounds exactly as
+// the uniform-softening constant h + h does, so no host read of soft2 is
+// needed.  Every slice partial is written before the reduction reads it, so
+// nothing in the scratch area has to start at zero.
+void direct_noalloc(int32_t kind, int64_t n, const double* pos, const double* mass, const double* soft2, double* acc,
+                    hipStream_t s) {
+    require(n > 0 && pos && mass && soft2 && acc, "rrg_op_direct_accel_noalloc: bad arguments");
+    require(valid_force_kind(kind), "rrg_op_direct_accel_noalloc: unknown force_kind");
+    int dev = 0;
+    RRG_HIP(hipGetDevice(&dev));
+    require_device_array(pos, dev, "rrg_op_direct_accel_noalloc: pos");
+    require_device_array(mass, dev, "rrg_op_direct_accel_noalloc: mass");
+    require_device_array(soft2, dev, "rrg_op_direct_accel_noalloc: soft2");
+    require_device_array(acc, dev, "rrg_op_direct_accel_noalloc: acc");
+    const nbody::Plan pl = nbody::plan_direct(n, kind);
+    const bool f32 = nbody::is_fp32_kind(kind);
+    const size_t v3 = round256(sizeof(double) * 3 * size_t(n));
+    const size_t rec = f32 ? round256(sizeof(float4) * size_t(n)) : 0;
+    const size_t part = round256(sizeof(double) * 3 * size_t(n) * size_t(pl.slices));
+    const size_t need = 2 * v3 + 2 * rec + part;
+    if (need > kNoAllocScratchBytes)
+        fail(RRG_ERR_INVALID, "rrg_op_direct_accel_noalloc: n = " + std::to_string(n) + " needs " +
+                                  std::to_string(need) + " bytes of scratch, more than the " +
+                                  std::to_string(kNoAllocScratchBytes) + " reserved per device");
+    unsigned char* base = scratch_address(dev);
+    double* X = reinterpret_cast<double*>(base);
+    double* R = reinterpret_cast<double*>(base + v3);
+    double* P = reinterpret_cast<double*>(base + 2 * v3);
+    float4* A = f32 ? reinterpret_cast<float4*>(base + 2 * v3 + part) : nullptr;
+    float4* B = f32 ? reinterpret_cast<float4*>(base + 2 * v3 + part + rec) : nullptr;
+    const PState<double> p{X, nullptr, nullptr, nullptr, const_cast<double*>(mass), const_cast<double*>(soft2),
+                           nullptr, n};
+    nbody::Force f;
+    f.kind = kind;
+    f.uniform = false;
+    layout::rows_in<double>(s, n, pos, X, nullptr);
+    if (f32) nbody::pack_mixed(s, p, A, B);
+    nbody::accel(s, pl, f, p, A, B, P);
+    nbody::reduce_slices(s, n, 3, pl.slices, nullptr, P, R);
+    layout::rows_out<double>(s, n, R, acc, nullptr);
+}
+
+}  // namespace
+
+RRG_API int rrg_op_direct_accel_noalloc(int32_t force_kind, int64_t n, const double* pos, const double* mass,
+                                        const double* soft2, double* acc, void* stream) {
+    return guarded([&] { direct_noalloc(force_kind, n, pos, mass, soft2, acc, static_cast<hipStream_t>(stream)); });
+}
