@@ -32,23 +32,29 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 C = 4574.337022617616
-# HBM bytes per k_wave<float> launch on the 128^3 headline grid, measured with
-# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE in separate runs of
-# `bench.py --profile-pass`, each scaled by what the same counter reads for a
-# 128 MiB device-to-device copy (FETCH_SIZE x 1.9996, WRITE_SIZE x 1.0):
-# 12824.5 KiB fetched and 8192.0 KiB written per launch.  The summary with the
-# commands is profiles/wave_pm1e6_128_hbm_counters.txt.
-WAVE_TRAFFIC_128 = 34_647_301
-# The same for one three-iteration k_wave_fused<float, 3> launch, same method:
-# 17186.2 KiB fetched (x 1.9995) and 16384.0 KiB written per launch (before
-# the 128^3 shape took four iterations per launch).
-WAVE_FUSED_TRAFFIC_128 = 51_965_754
-# One four-iteration k_wave_fused<float, 4> launch, same method: 17889.2 KiB
-# fetched (x 1.9995) and 16384.0 KiB written per launch
-# (profiles/wave_fused_pm1e6_128_hbm_counters.txt).
-WAVE_FUSED4_TRAFFIC_128 = 53_405_138
-# HBM bytes per launch by iterations per launch, for the roofline's traffic
-WAVE_TRAFFIC_BY_LEVELS_128 = {1: WAVE_TRAFFIC_128, 3: WAVE_FUSED_TRAFFIC_128, 4: WAVE_FUSED4_TRAFFIC_128}
+PROFILES = os.path.join(ROOT, "profiles")
+# rocprofv3 --pmc summaries holding the measured HBM bytes per launch of the
+# headline stencil launches, by iterations per launch (FETCH_SIZE and
+# WRITE_SIZE per dispatch, each scaled by what the counter reads for a 128 MiB
+# device-to-device copy); each file states the commands that made it.
+TRAFFIC_FILES = {4: "wave_fused_pm1e6_128_hbm_counters.txt", 1: "wave_pm1e6_128_hbm_counters.txt"}
+
+
+def profiled_traffic(levels):
+    """(bytes per launch, file) from the summary's `traffic_bytes_per_launch = B` line, or (None, None)."""
+    name = TRAFFIC_FILES.get(levels)
+    if name is None:
+        return None, None
+    try:
+        with open(os.path.join(PROFILES, name)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return int(line.replace(":", "=", 1).split("=")[1].split()[0]), "profiles/" + name
+    except OSError:
+        pass
+    return None, None
+
+
 def wave_schedule(rrg, iters, nx, real="fp32", fused=True):
     """Launches of one unchecked solve of `iters` iterations on a periodic
     grid, by iterations per launch, as the library splits them."""
@@ -110,20 +116,24 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     assert sum(sched.values()) == round(wave_launches / steps), (sched, wave_launches, steps)
     fused = sched[4] + sched[3] + sched[2]
     n = PM["n"]
-    # Algorithmic bytes, fp32, per unit of work of the PM step: the stencil
-    # 16 B per cell-iteration (phi^n, phi^(n-1), source read, phi^(n+1)
+    # Effective bytes, fp32, per unit of work of the PM step (what the step
+    # would move with one stencil iteration per launch): the stencil 16 B
+    # per cell-iteration (phi^n, phi^(n-1), source read, phi^(n+1)
     # written), gradient 16 B per cell, deposit 16 B per particle + 8 B per
     # cell, fixed point to source 12 B per cell, gather 24 B per particle,
     # kick/drift 60 B per particle.
     step_bytes = iters * 16 * ncell + 16 * ncell + (16 * n + 8 * ncell) + 12 * ncell + 24 * n + 60 * n
-    # The least traffic the kernels that ran can have: a fused launch reads
-    # phi^n, phi^(n-1) and the source once and writes its last two levels
-    # (20 B per cell), a single launch 16 B per cell; the other stages as
-    # implemented (gradient 4 in, 16 out padded; source 8 in, 4+4 out, 8
-    # clear; deposit x,v,a,m in, x,v* out, 8 atomics of 8 B; gather + kick
-    # x, v, v* in, a, v out, 8 corners of 16 B).
-    own_bytes = (fused * 20 * ncell + sched[1] * 16 * ncell + 20 * ncell + 24 * ncell + n * (40 + 24 + 64) +
-                 n * (36 + 24 + 128))
+    # Compulsory HBM bytes of the kernels that run, each grid and particle
+    # array moved once: a fused stencil launch reads phi^n, phi^(n-1) and the
+    # source and writes its last two levels (20 B per cell), a single launch
+    # 16 B per cell; predict + deposit x, v, a, m in and x, v* out (64 B per
+    # particle) and the int64 grid read and written (16 B per cell); fixed
+    # point to source 8 B in, cell mass and source out, the int64 grid
+    # cleared (24 B per cell); gradient 4 B in, 16 B out (padded) per cell;
+    # gather + kick x, v, v* in, a, v out (60 B per particle) and the padded
+    # gradient grid read (16 B per cell).  The re-sort every 16 steps is not
+    # counted.
+    own_bytes = (fused * 20 * ncell + sched[1] * 16 * ncell + (16 + 24 + 20 + 16) * ncell + (64 + 60) * n)
     secs = t1 - t0
     out = {}
     if force_err:
@@ -146,8 +156,8 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
                                    "two_iterations": sched[2], "single": sched[1]},
         "wave_iterations_per_launch": max(k for k, v in sched.items() if v),
         "wave_fused_launch_bytes": 20 * ncell,
-        "step_algorithmic_bytes": step_bytes,
-        "step_own_minimum_bytes": own_bytes,
+        "step_effective_bytes": step_bytes,
+        "step_compulsory_bytes": own_bytes,
         "wave_iterations_per_step": stats["last_wave_iterations"],
     }
 
@@ -330,7 +340,8 @@ def adjoint_bench(rrg, ics, n=65_536, steps=200, every=20):
             g = sim.adjoint(2 * (xT - target) / n)
             t3 = time.perf_counter()
         out[force] = {"t_forward_s": t1 - t0, "t_adjoint_s": t3 - t2, "adjoint_over_forward": (t3 - t2) / (t1 - t0),
-                      "loss": loss, "grads": g}
+                      "reverse_ginteractions_per_s": float(n) * n * steps / (t3 - t2) / 1e9, "loss": loss,
+                      "grads": g}
     ref = out["fp64"]["grads"]
     errs = {}
     for name, a, b in zip(("x0", "v0", "m"), out["mixed"]["grads"], ref):
@@ -347,9 +358,10 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2, real="fp32"):
     over the same [-8,8]^3 box (20 setup iterations, 200 per step, `real`
     state), with several iterations per launch where the library has a launch
     shape and with one per launch; time per iteration and per launch from the
-    stage events; rates of 4 reals per cell-iteration (the stencil's
-    algorithmic bytes: 16 B in FP32, 32 B in FP64) and of the launches' least
-    traffic (fused: 5 reals per cell, single: 4) over the stage time."""
+    stage events; `frac` = the launches' compulsory bytes (fused: 5 reals per
+    cell, single: 4) over the stage time against HBM peak, and
+    `effective_gbs` = 4 reals per cell-iteration (16 B in FP32, 32 B in FP64)
+    over the same time."""
     rb = 4 if real == "fp32" else 8
     pos, vel, m = ics.exp_disk(PM["n"], seed=1)
     out = {}
@@ -375,8 +387,8 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2, real="fp32"):
             row["fused" if fused else "single"] = {
                 "launches_per_step": {str(k): v for k, v in sched.items() if v}, "us_per_launch": ms * 1e3 / launches,
                 "us_per_iteration": ms * 1e3 / (PM["iterations"] * steps),
-                "achieved_gbs": alg / (ms * 1e-3) / 1e9, "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                "own_minimum_gbs": own / (ms * 1e-3) / 1e9, "own_minimum_frac": own / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                "compulsory_gbs": own / (ms * 1e-3) / 1e9, "frac": own / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "effective_gbs": alg / (ms * 1e-3) / 1e9}
         out[str(nx)] = row
     return out
 
@@ -453,7 +465,8 @@ def pm_adjoint_bench(rrg, ics, n=262_144, steps=200, every=20):
             g = sim.adjoint(seed)
             t4 = time.perf_counter()
             r = {"t_forward_s": t2 - t0, "t_forward_steps_s": t2 - t1, "t_adjoint_s": t4 - t3,
-                 "adjoint_over_forward": (t4 - t3) / (t2 - t0), "loss": loss, "grads": g}
+                 "adjoint_over_forward": (t4 - t3) / (t2 - t0), "reversed_steps_per_s": steps / (t4 - t3),
+                 "loss": loss, "grads": g}
             if real == "fp64":
                 seed64 = seed
             else:
@@ -484,17 +497,27 @@ def pm_adjoint_bench(rrg, ics, n=262_144, steps=200, every=20):
                                    "x_T(target masses)|^2", "n": n, "steps": steps}}
 
 
-def cpu_baseline(ics, target_s=10.0):
-    """The CPU oracle (FP32 restatement) on a bounded sample of the headline
-    workload: the full 1e6-particle / 128^3 problem, K PM steps (each with the
-    200 wave iterations; K sized for about 10 s of CPU work), timed as
-    (precalc + K steps) - (precalc only)."""
+def _cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def _oracle():
     import oracle
 
-    pos, vel, m = ics.exp_disk(PM["n"], seed=1)
-    kw = dict(nx=PM["nx"], dx=PM["dx"], dt=PM["dt"], diss_base=PM["diss_base"], wave_speed=C, setup_iterations=0,
-              iterations=PM["iterations"], tolerance=0.0, fixed_e=60, dtype=np.float32)
     oracle.lib()
+    return oracle
+
+
+def cpu_baseline(ics, target_s=10.0, real="fp32"):
+    """The CPU oracle on a bounded sample of the headline workload: the full
+    1e6-particle / 128^3 problem, K PM steps (each with the 200 wave
+    iterations; K sized for about `target_s` of CPU work), timed as
+    (precalc + K steps) - (precalc only)."""
+    oracle = _oracle()
+    pos, vel, m = ics.exp_disk(PM["n"], seed=1)
+    dtype = np.float32 if real == "fp32" else np.float64
+    kw = dict(nx=PM["nx"], dx=PM["dx"], dt=PM["dt"], diss_base=PM["diss_base"], wave_speed=C, setup_iterations=0,
+              iterations=PM["iterations"], tolerance=0.0, fixed_e=60, dtype=dtype)
 
     def per_step(k):
         t0 = time.perf_counter()
@@ -506,10 +529,109 @@ def cpu_baseline(ics, target_s=10.0):
 
     k = int(min(100, max(2, target_s / max(per_step(2), 1e-3))))
     t = per_step(k)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": PM["n"] / t / 1e6, "unit": "Mparticle-steps/s", "cores": cores, "kind": "port",
-            "sample": f"oracle/rrg_oracle.cpp (fp32, OpenMP over cells/particles; deposit sequential) on the full "
-                      f"workload, {k} PM steps of 200 wave iterations ({k * t:.1f} s), minus precalc"}
+    return {"value": PM["n"] / t / 1e6, "unit": "Mparticle-steps/s", "cores": _cores(), "kind": "port",
+            "sample": f"oracle/rrg_oracle.cpp ({real}, OpenMP over cells and particles, deposit into per-thread "
+                      f"grids) on the full workload, {k} PM steps of 200 wave iterations ({k * t:.1f} s), minus "
+                      "precalc"}
+
+
+def cpu_baseline_c1(ics):
+    """The oracle (FP64, IEEE 1/sqrt as nbody.cu:50-53) on the whole C1 run:
+    precalc + 100 steps of Plummer N=4096, N^2 interactions per evaluation."""
+    oracle = _oracle()
+    n, steps = 4096, 100
+    pos, vel, m = ics.plummer(n, seed=0)
+    t0 = time.perf_counter()
+    oracle.nbody_run(pos, vel, m, np.full(n, 0.01 ** 2), 1e-3, steps)
+    t = time.perf_counter() - t0
+    return {"value": float(n) * n * (steps + 1) / t / 1e9, "unit": "Ginteractions/s", "cores": _cores(),
+            "kind": "port", "sample": f"oracle nbody_run, the whole workload (101 force evaluations), {t:.2f} s"}
+
+
+def cpu_baseline_direct(ics, ntarget=4096):
+    """The oracle's FP64 pair sum (nbody.cu:19-65 arithmetic) for a seeded
+    sample of targets against all N sources of the disk + bulge workload."""
+    oracle = _oracle()
+    pos, _, m = ics.disk_bulge()
+    n = len(m)
+    idx = np.sort(np.random.default_rng(5).choice(n, ntarget, replace=False))
+    t0 = time.perf_counter()
+    oracle.accel_subset(pos, m, np.full(n, 0.005 ** 2), idx)
+    t = time.perf_counter() - t0
+    return {"value": float(ntarget) * n / t / 1e9, "unit": "Ginteractions/s", "cores": _cores(), "kind": "port",
+            "sample": f"oracle accel_subset, {ntarget} targets (seed 5) x all {n} sources, FP64, {t:.2f} s"}
+
+
+def cpu_baseline_ensemble(ics, per_system=256):
+    """The oracle's FP64 pair sum over the 64-galaxy ensemble: a seeded sample
+    of targets in every system against all of that system's sources."""
+    oracle = _oracle()
+    sizes = ics.ensemble_sizes(64, seed=2)
+    rng = np.random.default_rng(2)
+    parts = [ics.plummer(int(nn), rng=rng) if g % 2 == 0 else ics.exp_disk(int(nn), rng=rng)
+             for g, nn in enumerate(sizes)]
+    srng = np.random.default_rng(9)
+    inter = 0.0
+    t = 0.0
+    for (pos, _, m) in parts:
+        idx = np.sort(srng.choice(len(m), per_system, replace=False))
+        t0 = time.perf_counter()
+        oracle.accel_subset(pos, m, np.full(len(m), 0.01 ** 2), idx)
+        t += time.perf_counter() - t0
+        inter += float(per_system) * len(m)
+    return {"value": inter / t / 1e9, "unit": "Ginteractions/s", "cores": _cores(), "kind": "port",
+            "sample": f"oracle accel_subset, {per_system} targets per system (seed 9) x all its sources, 64 systems, "
+                      f"FP64, {t:.2f} s"}
+
+
+def cpu_baseline_adjoint(ics, n=4096, steps=20):
+    """The oracle's FP64 reverse-mode gradient of the direct-sum run (forward
+    run with stored positions, then the reversed steps), Plummer N=4096, 20
+    steps; the rate counts N^2 pair interactions per reversed step."""
+    oracle = _oracle()
+    pos, vel, m = ics.plummer(n, seed=0)
+    g = np.random.default_rng(3).standard_normal((n, 3)) / n
+    t0 = time.perf_counter()
+    oracle.nbody_adjoint(pos, vel, m, np.full(n, 0.01 ** 2), 1e-3, steps, g)
+    t = time.perf_counter() - t0
+    return {"value": float(n) * n * steps / t / 1e9, "unit": "Ginteractions/s of the reverse sweep (N^2 per "
+            "reversed step, forward recomputation included)", "cores": _cores(), "kind": "port",
+            "sample": f"oracle nbody_adjoint, Plummer N={n}, {steps} steps, FP64, {t:.2f} s"}
+
+
+def cpu_baseline_adjoint_pm(ics, n=262_144):
+    """The oracle's FP64 reverse-mode gradient of the PM run at the workload's
+    size (64^3 cells, N=262,144, 200 wave iterations per step): the time per
+    reversed step from runs of 3 and 1 steps (200 setup iterations)."""
+    oracle = _oracle()
+    pos, vel, m = ics.exp_disk(n, seed=1)
+    g = np.random.default_rng(3).standard_normal((n, 3)) / n
+    kw = dict(nx=64, dx=0.25, dt=5e-3, diss_base=PM["diss_base"], wave_speed=C, setup_iterations=200,
+              iterations=200, fixed_e=60)
+    t0 = time.perf_counter()
+    oracle.pm_adjoint(pos, vel, m, 1, g, **kw)
+    t1 = time.perf_counter()
+    oracle.pm_adjoint(pos, vel, m, 3, g, **kw)
+    t2 = time.perf_counter()
+    per = ((t2 - t1) - (t1 - t0)) / 2
+    return {"value": 1.0 / per, "unit": "reversed PM steps/s (forward recomputation included)", "cores": _cores(),
+            "kind": "port", "sample": f"oracle pm_adjoint, 64^3, N={n}, (t(3 steps) - t(1 step)) / 2 = {per:.2f} s"}
+
+
+def profile_pass_direct(rrg, ics, steps, graphs=True):
+    """Short run for counter passes over the direct-sum kernels: the disk +
+    bulge workload (N = 262,144, eps = 0.005), precalc + `steps` steps for
+    each force kind (one acceleration launch per step, plus the potential
+    and acceleration at precalc)."""
+    pos, vel, m = ics.disk_bulge()
+    s2 = np.full(len(m), 0.005 ** 2)
+    for force in ("mixed", "fp32", "fp64", "fp64_exact"):
+        p = rrg.make_params(dt=1e-3, force=force, graphs=graphs)
+        with rrg.Simulation(p, pos, vel, m, s2) as sim:
+            sim.precalc()
+            sim.calc_step(steps)
+            sim.synchronize()
+        print(json.dumps({"profile_pass": "direct", "force": force, "steps": steps}), flush=True)
 
 
 def profile_pass(rrg, ics, steps, graphs=True):
@@ -555,6 +677,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-pass", action="store_true", help="short PM run + calibration copy for rocprofv3 --pmc")
     ap.add_argument("--eager", action="store_true", help="with --profile-pass: launch kernels eagerly, not from graphs")
+    ap.add_argument("--workload", default="pm", choices=("pm", "direct"),
+                    help="with --profile-pass: the headline PM run (pm) or the direct-sum kinds on the disk + bulge "
+                         "workload (direct)")
     ap.add_argument("--skip", default="", help="comma-separated legs to leave out: direct, c1, scaling, ensemble, "
                                               "adjoint, adjoint_pm, pm_fp64, cpu")
     args = ap.parse_args()
@@ -568,7 +693,10 @@ def main():
         from rrgalaxsim_amd import ics
 
         rrg.lib()
-        profile_pass(rrg, ics, args.steps, graphs=not args.eager)
+        if args.workload == "direct":
+            profile_pass_direct(rrg, ics, args.steps, graphs=not args.eager)
+        else:
+            profile_pass(rrg, ics, args.steps, graphs=not args.eager)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the hot path has no "
@@ -580,14 +708,19 @@ def main():
     rrg.lib()
     pm = pm_bench(rrg, ics, args.steps, args.warmup)
     done("pm")
-    # the stencil's algorithmic rate: 16 B per cell-iteration over the wave
-    # stage's time (all launches of the step); per launch of the dominant
-    # kernel that is 16 B x cells x its iterations
+    # Dominant kernel: the stencil launch that runs `per_launch` iterations.
+    # Its compulsory HBM bytes per launch: phi^n, phi^(n-1) and the source
+    # read once (12 B per cell in FP32) and its last two levels written
+    # (8 B per cell) for a fused launch; 16 B per cell for a single
+    # iteration.  Launch time: the wave stage's HIP-event time on the
+    # library's stream over its launches.
     cells = PM["nx"] ** 3
-    wave_alg_bytes = 16.0 * cells * PM["iterations"]
-    achieved = wave_alg_bytes / (pm["wave_ms_per_step"] * 1e-3) / 1e9
     per_launch = pm["wave_iterations_per_launch"]
-    own_min_gbs = pm["wave_fused_launch_bytes"] / (pm["wave_avg_launch_us"] * 1e-6) / 1e9
+    launch_s = pm["wave_avg_launch_us"] * 1e-6
+    compulsory = (20 if per_launch > 1 else 16) * cells
+    effective = 16 * cells * per_launch
+    traffic, traffic_src = profiled_traffic(per_launch) if PM["nx"] == 128 else (None, None)
+    step_s = pm["ms_per_step"] * 1e-3
     line = {
         "metric": METRIC,
         "value": pm["value"],
@@ -606,20 +739,23 @@ def main():
                    "n": PM["n"], "nx": PM["nx"], "wave_iterations": PM["iterations"]},
         "roofline": {"bound": "hbm",
                      "kernel": f"k_wave_fused<float, {per_launch}>" if per_launch > 1 else "k_wave<float>",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": WAVE_TRAFFIC_BY_LEVELS_128.get(per_launch) if PM["nx"] == 128 else None,
-                     "algorithmic_bytes": 16 * cells * per_launch,
-                     "per_unit": "16 B per cell-iteration (phi^n, phi^(n-1), source read; phi^(n+1) written), "
-                                 f"{per_launch} iterations x {cells} cells per launch; the fused launch keeps the "
-                                 "intermediate levels on chip, so its HBM traffic (traffic) is below these bytes",
-                     "own_minimum_bytes": pm["wave_fused_launch_bytes"] if per_launch > 1 else 16 * cells,
-                     "own_minimum_frac": (own_min_gbs if per_launch > 1 else achieved) / HBM_PEAK_GBS},
-        "step_roofline": {"algorithmic_bytes": pm["step_algorithmic_bytes"],
-                          "achieved_gbs": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9,
-                          "frac": pm["step_algorithmic_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                          "own_minimum_bytes": pm["step_own_minimum_bytes"],
-                          "own_minimum_frac": pm["step_own_minimum_bytes"] / (pm["ms_per_step"] * 1e-3) / 1e9 /
-                          HBM_PEAK_GBS},
+                     "achieved": compulsory / launch_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": compulsory / launch_s / 1e9 / HBM_PEAK_GBS,
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes": compulsory, "launch_us": launch_s * 1e6,
+                     "per_unit": (f"{compulsory // cells} B per cell per launch ({cells} cells, {per_launch} "
+                                  "iterations): phi^n, phi^(n-1) and the source read once, the last two levels "
+                                  "written" if per_launch > 1 else f"16 B per cell ({cells} cells): phi^n, "
+                                  "phi^(n-1), source read, phi^(n+1) written"),
+                     "effective_gbs": effective / launch_s / 1e9,
+                     "effective_note": "16 B per cell-iteration x iterations per launch over the launch time: the "
+                                       "rate a one-iteration-per-launch stencil would need; not an HBM rate, since "
+                                       "the fused launch keeps its intermediate levels on chip"},
+        "step_roofline": {"compulsory_bytes": pm["step_compulsory_bytes"],
+                          "achieved_gbs": pm["step_compulsory_bytes"] / step_s / 1e9,
+                          "frac": pm["step_compulsory_bytes"] / step_s / 1e9 / HBM_PEAK_GBS,
+                          "effective_bytes": pm["step_effective_bytes"],
+                          "effective_gbs": pm["step_effective_bytes"] / step_s / 1e9},
         "ms_per_step_with_stage_events": pm["ms_per_step_with_stage_events"],
         "stage_ms_per_step": pm["stage_ms_per_step"],
         "wave_avg_launch_us": pm["wave_avg_launch_us"],
@@ -644,10 +780,25 @@ def main():
             d = line["direct_sum"]
             for k in ("mixed", "fp32"):
                 d[k]["fp32_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP32_PEAK_TFLOPS
+                # the FP32 kernels issue one FP32 operation per lane per instruction (packed FP32 is off),
+                # whose ceiling is half the published (packed) figure
+                d[k]["fp32_unpacked_frac"] = d[k]["tflops_20_per_interaction"] / (FP32_PEAK_TFLOPS / 2)
             for k in ("fp64", "fp64_exact"):
                 d[k]["fp64_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP64_PEAK_TFLOPS
+    if "wave_scaling" in line:
+        ws = line["wave_scaling"]
+        line["roofline"]["hbm_bound_grids"] = {
+            nx: {"fused_frac": ws[nx]["fused"]["frac"], "single_frac": ws[nx]["single"]["frac"],
+                 "fused_us_per_iteration": ws[nx]["fused"]["us_per_iteration"]} for nx in ("256", "512") if nx in ws}
     if not args.no_cpu and "cpu" not in skip:
         line["cpu_baseline"] = cpu_baseline(ics)
+        per_leg = [("direct_sum", cpu_baseline_direct), ("direct_sum_c1", cpu_baseline_c1),
+                   ("ensemble", cpu_baseline_ensemble), ("adjoint", cpu_baseline_adjoint),
+                   ("adjoint_pm", cpu_baseline_adjoint_pm),
+                   ("pm_fp64", lambda i: cpu_baseline(i, target_s=5.0, real="fp64"))]
+        for key, fn in per_leg:
+            if key in line:
+                line[key]["cpu_baseline"] = fn(ics)
         done("cpu")
     print(json.dumps(line), flush=True)
 

@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#include <omp.h>
 
 namespace {
 
@@ -270,17 +271,54 @@ template <class real> inline int64_t to_fixed(real v, int e) {
     return (int64_t)std::llrint(std::ldexp((double)v, e));
 }
 
+// Integer sums are exact, so the particles are split over the host threads:
+// each thread deposits into a private grid and the grids are added cell by
+// cell (integer atomics instead when the private grids would pass 1 GiB);
+// the result is the same as a sequential deposit.
 template <class real>
 void cic_deposit_fixed(int64_t n, const real* pos, const real* mass, int nx, real dx, real bc_frac,
                        int e, int64_t* grid) {
     const Grid<real> g = make_grid<real>(nx, dx, bc_frac);
     const V3<real>* p = reinterpret_cast<const V3<real>*>(pos);
-    std::fill(grid, grid + size_t(nx) * nx * nx, int64_t(0));
-    for (int64_t q = 0; q < n; ++q) {
-        const CicStencil<real> s = cic_stencil(p[q], g);
-        size_t c[8];
-        cic_cells(s, nx, c);
-        for (int k = 0; k < 8; ++k) grid[c[k]] += to_fixed<real>(mass[q] * s.w[k], e);
+    const size_t ncell = size_t(nx) * nx * nx;
+    const int nt = omp_get_max_threads();
+    const bool priv = nt > 1 && size_t(nt) * ncell * sizeof(int64_t) <= (size_t(1) << 30) &&
+                      n >= int64_t(ncell) / 8;
+    if (!priv) {
+#pragma omp parallel for schedule(static)
+        for (int64_t c = 0; c < int64_t(ncell); ++c) grid[c] = 0;
+#pragma omp parallel for schedule(static)
+        for (int64_t q = 0; q < n; ++q) {
+            const CicStencil<real> s = cic_stencil(p[q], g);
+            size_t c[8];
+            cic_cells(s, nx, c);
+            for (int k = 0; k < 8; ++k) {
+                const int64_t v = to_fixed<real>(mass[q] * s.w[k], e);
+#pragma omp atomic
+                grid[c[k]] += v;
+            }
+        }
+        return;
+    }
+    std::vector<int64_t> part(size_t(nt) * ncell);
+#pragma omp parallel num_threads(nt)
+    {
+        const int t = omp_get_thread_num();
+        int64_t* mine = part.data() + size_t(t) * ncell;
+        std::fill(mine, mine + ncell, int64_t(0));
+#pragma omp for schedule(static)
+        for (int64_t q = 0; q < n; ++q) {
+            const CicStencil<real> s = cic_stencil(p[q], g);
+            size_t c[8];
+            cic_cells(s, nx, c);
+            for (int k = 0; k < 8; ++k) mine[c[k]] += to_fixed<real>(mass[q] * s.w[k], e);
+        }
+#pragma omp for schedule(static)
+        for (int64_t c = 0; c < int64_t(ncell); ++c) {
+            int64_t v = 0;
+            for (int u = 0; u < nt; ++u) v += part[size_t(u) * ncell + size_t(c)];
+            grid[c] = v;
+        }
     }
 }
 
