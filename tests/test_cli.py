@@ -5,7 +5,12 @@ rrgsim_integration.cu:19-145, io_particles_bin.cu:9-37, io_handler.cpp:71-105).
 CPU tests: usage and error exits.  GPU tests: a PM run and a direct-sum run
 whose dump files are compared with the oracle's run over the same number of
 steps (bitwise for the PM model, 1e-12 for the FP64 direct sum), and whose
-data tables and log follow the reference's layout.
+data tables and log follow the reference's layout; at every save of the PM
+run data/sample_particle.csv and the grid axis cuts (data/a_grid/,
+grav_grid/, mass_grid/; rrgsim_conservation.cu:334-424) hold, as text, the
+oracle's particle 0 and grids at that save, and check_bounds
+(rrgsim_conservation.cu:280-332) logs the particles' extent and the axes
+that leave the domain.
 """
 import glob
 import json
@@ -124,6 +129,7 @@ def test_cli_pm_run_dumps_match_oracle(rrg, orc, tmp_path, real, tol):
     pos0, vel0, m, _ = _ini(rrg)
     e = 61 - math.frexp(float(np.sum(np.abs(m))))[1]
     g, w = p["grid_params"], p["wave_params"]
+    sample_rows = []
     for it, t in saves:
         n, tt, pos, vel = _read_dump(work / "dump" / f"S_{it:5d}.bin", dtype)
         assert n == 256 and tt == pytest.approx(t, rel=1e-6)
@@ -132,6 +138,70 @@ def test_cli_pm_run_dumps_match_oracle(rrg, orc, tmp_path, real, tol):
                        iterations=w["iterations"], tolerance=w["tolerance"], check_step=w["steady_check_step"],
                        hold=w["steady_hold_steps"], fixed_e=e, dtype=dtype)
         assert np.array_equal(pos, o["pos"]) and np.array_equal(vel, o["vel"]), (it, real)
+        mass = orc.fixed_to_real(orc.cic_fixed(o["pos"], m.astype(dtype), g["nx"], g["dx"], e, bc_frac=g["bc_frac"],
+                                               dtype=dtype), e, dtype=dtype)
+        _check_cuts(work, g, dtype, t, o["phi"], o["grid_acc"], mass)
+        sample_rows.append(_row([0.0, *o["pos"][0], *o["vel"][0], o["grav"][0], *o["acc"][0]]))
+    got = open(work / "data" / "sample_particle.csv").read().splitlines()
+    assert got[0] == "i, x, y, z, vx, vy, vz, grav, ax, ay, az" and got[1:] == sample_rows
+
+
+def _row(vals):
+    return ", ".join("%.8f" % float(v) for v in vals)
+
+
+def _check_cuts(work, g, dtype, t, phi, gacc, mass):
+    """The three axis cuts at save time t: coordinate domain_min + dx * i in
+    the run's precision, values at AT(i, h, h) (and the y and z cuts)."""
+    r = np.dtype(dtype).type
+    nx, h = g["nx"], g["nx"] // 2
+    dx = r(g["dx"])
+    dmin = r(-0.5 * float(dx * r(nx)))
+    coord = [float(r(dmin + dx * r(i))) for i in range(nx)]
+    at = lambda x, y, z: x + nx * (y + nx * z)  # noqa: E731
+    want = {
+        ("a_grid", "a_grid_center_%.8f" % t, "x, ax, ay, az"):
+            [_row([coord[i], *gacc[at(i, h, h)]]) for i in range(nx)],
+        ("grav_grid", "grav_axis_center_%.10f" % t, "r, gx, gy, gz"):
+            [_row([coord[i], phi[at(i, h, h)], phi[at(h, i, h)], phi[at(h, h, i)]]) for i in range(nx)],
+        ("mass_grid", "mass_axis_center_%.10f" % t, "r, mx, my, mz"):
+            [_row([coord[i], mass[at(i, h, h)], mass[at(h, i, h)], mass[at(h, h, i)]]) for i in range(nx)],
+    }
+    for (sub, name, head), rows in want.items():
+        got = open(work / "data" / sub / f"{name}.csv").read().splitlines()
+        assert got[0] == head and got[1:] == rows, (sub, name)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("real", ["fp64", "fp32"])
+def test_cli_check_bounds_messages(rrg, tmp_path, real):
+    """A grid narrower than the disk (domain [-0.04, 0.04]): at each save
+    the log gives the particles' extent and one error line per axis and
+    side that leaves the domain, compared in the run's precision."""
+    p = _params(real=real)
+    p["grid_params"] = {"nx": 4, "dx": 0.02, "bc_frac": 0.0}
+    work = _work_dir(tmp_path, p)
+    r = _run(work)
+    assert r.returncode == 0, r.stdout + r.stderr
+    log = open(glob.glob(str(work / "logs" / "rrgsim_*.log"))[0]).read()
+    dtype = np.float32 if real == "fp32" else np.float64
+    rep = (lambda v: str(np.float32(v))) if real == "fp32" else (lambda v: repr(float(v)))
+    dmax = np.dtype(dtype).type(0.04)
+    saves, _ = _save_iterations(p, dtype)
+    want = []
+    for it, _ in saves:
+        _, _, pos, _ = _read_dump(work / "dump" / f"S_{it:5d}.bin", dtype)
+        lo, hi = pos.min(axis=0), pos.max(axis=0)
+        for c, ax in enumerate("XYZ"):
+            assert f"\t {ax}: [{rep(lo[c])} .. {rep(hi[c])}]" in log, (it, ax)
+        want += [f"{ax} min bounds violation! Particle {ax}: {rep(lo[c])}; domain_min: -0.04"
+                 for c, ax in enumerate("XYZ") if lo[c] < -dmax]
+        want += [f"{ax} max bounds violation! Particle {ax}: {rep(hi[c])}; domain_max: 0.04"
+                 for c, ax in enumerate("XYZ") if hi[c] > dmax]
+    got = [ln.split("] ", 2)[-1] for ln in log.splitlines() if "bounds violation" in ln]
+    assert got == want and any(w.startswith("X min") for w in want)
+    assert all("[error]" in ln for ln in log.splitlines() if "bounds violation" in ln)
+    assert log.count("Check bounds") == len(saves)
 
 
 @pytest.mark.gpu
