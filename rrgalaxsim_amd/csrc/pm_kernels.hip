@@ -418,7 +418,9 @@ inline int wave_zb(int nx) { return nx <= 96 ? 2 : 4; }
 template <class real, bool PERIODIC, bool CHECK, int ZB>
 __global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real* __restrict__ p,
                                               const real* __restrict__ po, const real* __restrict__ src, int NX,
-                                              WaveK<real> k, typename MaxBits<real>::T* __restrict__ maxbits) {
+                                              WaveK<real> k, typename MaxBits<real>::T* __restrict__ maxbits,
+                                              const int* __restrict__ halt) {
+    if (halt != nullptr && *halt != 0) return;  // a converged solve's later launches
     int bx, by, bz;
     xcd_block(gridDim.x, gridDim.y, gridDim.z, bx, by, bz);
     const int ix = bx * 64 + threadIdx.x;
@@ -470,6 +472,69 @@ __global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real*
     }
 }
 
+// ------------------------------------------- steady check on the device
+// solve_wave_equation's SteadyStateChecker (wave.cu:253-287,
+// wave_steady_checker.cu:95-143) with its state in a device control block,
+// so a solve with checks is a fixed launch sequence that a graph can hold:
+// each checking launch reduces max|phi_new - phi| and max|phi_new| into the
+// maxbits pair, a one-thread launch then applies the hold rule; once the
+// hold is reached `done` is set and every later stencil launch of the solve
+// returns at once, and a last launch moves the two live levels into the
+// buffers the full schedule would have left them in.
+template <class real>
+__global__ void k_solve_init(SolveCtl* __restrict__ c, typename MaxBits<real>::T* __restrict__ maxbits) {
+    c->done = 0;
+    c->hold = 0;
+    c->iters = 0;
+    c->ip = -1;
+    c->ic = -1;
+    maxbits[0] = 0;
+    maxbits[1] = 0;
+}
+
+// After the checking launch that completes `iters` iterations and leaves
+// phi^(n-1), phi^n in buffers (ip, ic): err = diff / (absu + 1e-30) in real
+// (wave_steady_checker.cu:119-126), then SteadyStateChecker::check.
+template <class real>
+__global__ void k_solve_decide(SolveCtl* __restrict__ c, typename MaxBits<real>::T* __restrict__ maxbits, real tol,
+                               int hold_need, int iters, int ip, int ic) {
+    if (c->done) return;
+    using MB = typename MaxBits<real>::T;
+    const MB bd = maxbits[0], ba = maxbits[1];
+    real diff, absu;
+    static_assert(sizeof(MB) == sizeof(real), "bit pattern width");
+    __builtin_memcpy(&diff, &bd, sizeof(real));
+    __builtin_memcpy(&absu, &ba, sizeof(real));
+    maxbits[0] = 0;
+    maxbits[1] = 0;
+    const real EPS = (real)1.E-30;
+    const real err = diff / (absu + EPS);
+    if ((double)err < (double)tol) ++c->hold;
+    else c->hold = 0;
+    if (c->hold >= hold_need) {
+        c->done = 1;
+        c->iters = iters;
+        c->ip = ip;
+        c->ic = ic;
+    }
+}
+
+// Moves phi^(n-1), phi^n of a converged solve from buffers (c->ip, c->ic) to
+// (ip, ic), where the unconverged schedule ends; each thread reads both
+// values of its cells before writing, so overlapping buffer pairs are safe.
+template <class real>
+__global__ __launch_bounds__(256) void k_solve_fixup(const SolveCtl* __restrict__ c, PhiBufs<real> b, int ip, int ic,
+                                                     int64_t ncell) {
+    if (!c->done) return;
+    const int sp = c->ip, sc = c->ic;
+    if (sp == ip && sc == ic) return;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < ncell; i += int64_t(gridDim.x) * 256) {
+        const real vp = b.p[sp][i], vc = b.p[sc][i];
+        b.p[ip][i] = vp;
+        b.p[ic][i] = vc;
+    }
+}
+
 template <class real>
 __global__ __launch_bounds__(256) void k_max_diff(int64_t ncell, const real* __restrict__ a, const real* __restrict__ b,
                                                   typename MaxBits<real>::T* __restrict__ maxbits) {
@@ -492,7 +557,9 @@ __global__ __launch_bounds__(256) void k_max_diff(int64_t ncell, const real* __r
 template <class real, int BT, bool CHECK, int ZB>
 __global__ __launch_bounds__(256) void k_wave_bc(real* __restrict__ pn, const real* __restrict__ p,
                                                  const real* __restrict__ po, const real* __restrict__ src,
-                                                 WaveModel<real> w, typename MaxBits<real>::T* __restrict__ maxbits) {
+                                                 WaveModel<real> w, typename MaxBits<real>::T* __restrict__ maxbits,
+                                                 const int* __restrict__ halt) {
+    if (halt != nullptr && *halt != 0) return;  // a converged solve's later launches
     int bx, by, bz;
     xcd_block(gridDim.x, gridDim.y, gridDim.z, bx, by, bz);
     const int NX = w.nx;
@@ -893,7 +960,7 @@ void mass_to_source(hipStream_t s, int nx, const real* mass, real dx, real rho_m
 
 template <class real>
 void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, real* pn, const real* p,
-                    const real* po, const real* src, void* maxbits) {
+                    const real* po, const real* src, void* maxbits, const int* halt) {
     using MB = typename MaxBits<real>::T;
     MB* mb = reinterpret_cast<MB*>(maxbits);
     const int nx = m.nx;
@@ -901,11 +968,11 @@ void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& 
     const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned((nx + zb - 1) / zb));
     const dim3 block(64, 4, 1);
     if (m.bc == RRG_BC_PERIODIC) {
-        auto go = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb); };
+        auto go = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, src, nx, k, mb, halt); };
         if (zb == 2) mb ? go(k_wave<real, true, true, 2>) : go(k_wave<real, true, false, 2>);
         else mb ? go(k_wave<real, true, true, 4>) : go(k_wave<real, true, false, 4>);
     } else {
-        auto go = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, src, m, mb); };
+        auto go = [&](auto kern) { kern<<<grid, block, 0, s>>>(pn, p, po, src, m, mb, halt); };
         // face-heavy small grids use the same z depth rule
         if (m.bc == RRG_BC_MONOPOLE_DIRICHLET) {
             if (zb == 2) mb ? go(k_wave_bc<real, RRG_BC_MONOPOLE_DIRICHLET, true, 2>) : go(k_wave_bc<real, RRG_BC_MONOPOLE_DIRICHLET, false, 2>);
@@ -927,10 +994,10 @@ template <class real> int wave_fused_levels(const WaveModel<real>& m, int remain
 
 template <class real>
 void wave_fused(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, int levels, real* qa, real* qb,
-                const real* p, const real* po, const real* src) {
+                const real* p, const real* po, const real* src, const int* halt) {
     if (levels < 2 || fused_levels<real>(m.nx, m.bc, levels) != levels)
         fail(RRG_ERR_INVALID, "wave_fused: no launch shape for this grid and iteration count");
-    launch_fused<real, false, false>(s, m.nx, levels, qa, qb, p, po, src, nullptr, k);
+    launch_fused<real, false, false>(s, m.nx, levels, qa, qb, p, po, src, nullptr, k, halt);
     RRG_LAUNCH_CHECK();
 }
 
@@ -974,6 +1041,25 @@ template <class real> void max_diff(hipStream_t s, int64_t ncell, const real* a,
     using MB = typename MaxBits<real>::T;
     const unsigned nb = unsigned(std::min<int64_t>(4096, (ncell + 255) / 256));
     k_max_diff<real><<<nb, 256, 0, s>>>(ncell, a, b, reinterpret_cast<MB*>(maxbits));
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real> void solve_init(hipStream_t s, SolveCtl* c, void* maxbits) {
+    k_solve_init<real><<<1, 1, 0, s>>>(c, reinterpret_cast<typename MaxBits<real>::T*>(maxbits));
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real>
+void solve_decide(hipStream_t s, SolveCtl* c, void* maxbits, real tol, int hold_need, int iters, int ip, int ic) {
+    k_solve_decide<real><<<1, 1, 0, s>>>(c, reinterpret_cast<typename MaxBits<real>::T*>(maxbits), tol, hold_need,
+                                         iters, ip, ic);
+    RRG_LAUNCH_CHECK();
+}
+
+template <class real>
+void solve_fixup(hipStream_t s, const SolveCtl* c, const PhiBufs<real>& b, int ip, int ic, int64_t ncell) {
+    const unsigned nb = unsigned(std::min<int64_t>(2048, (ncell + 255) / 256));
+    k_solve_fixup<real><<<nb, 256, 0, s>>>(c, b, ip, ic, ncell);
     RRG_LAUNCH_CHECK();
 }
 
@@ -1042,10 +1128,13 @@ size_t ngp_temp_bytes(int64_t n, int nx) {
     template void mass_to_source<real>(hipStream_t, int, const real*, real, real, bool, real, real*,            \
                                        const WaveModel<real>*);                                                 \
     template void wave_iteration<real>(hipStream_t, const WaveModel<real>&, const WaveK<real>&, real*,          \
-                                       const real*, const real*, const real*, void*);                           \
+                                       const real*, const real*, const real*, void*, const int*);               \
     template int wave_fused_levels<real>(const WaveModel<real>&, int);                                          \
     template void wave_fused<real>(hipStream_t, const WaveModel<real>&, const WaveK<real>&, int, real*, real*,  \
-                                   const real*, const real*, const real*);                                      \
+                                   const real*, const real*, const real*, const int*);                          \
+    template void solve_init<real>(hipStream_t, SolveCtl*, void*);                                              \
+    template void solve_decide<real>(hipStream_t, SolveCtl*, void*, real, int, int, int, int);                  \
+    template void solve_fixup<real>(hipStream_t, const SolveCtl*, const PhiBufs<real>&, int, int, int64_t);     \
     template void gradient<real>(hipStream_t, int, real, const real*, real*);                                   \
     template void gather<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*, real, bool);    \
     template void ngp_phi<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*);               \

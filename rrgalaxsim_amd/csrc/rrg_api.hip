@@ -10,6 +10,7 @@
 //   calc_conservation                 solver/lib/rrgsim_nbody/src/nbody.cu:246-291
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -969,6 +970,8 @@ public:
         for (int i = 0; i < nbuf_; ++i) phi_[i].alloc(ncell_);
         gacc_.alloc(4 * ncell_);
         maxbits_.alloc(2);
+        ctl_.alloc(1);
+        zero_async(s_, ctl_.p, ctl_.bytes());
         zero_async(s_, mfix_.p, mfix_.bytes());
         zero_async(s_, mass_.p, mass_.bytes());
         zero_async(s_, src_.p, src_.bytes());
@@ -1000,10 +1003,11 @@ public:
     void step(int nsteps) override {  // rrgsim_integration_rect.cuh:99-112
         if (!precalc_done_) fail(RRG_ERR_STATE, "rrg_step before rrg_precalc");
         const real dt = (real)prm_.sim.dt_dynamics;
-        // Without a steady-state check the step makes no host decision, so
-        // its segments are replayed from captured graphs (one set per buffer
-        // arrangement).
-        const bool graph = prm_.use_graphs && !(tol_ > (real)0);
+        // Steps are replayed from captured graphs (one set per buffer
+        // arrangement); a steady-state check is then decided on the device.
+        // A run recorded for the adjoint with checks keeps the host's
+        // decisions, which give it each step's iteration count.
+        const bool graph = prm_.use_graphs && !(tol_ > (real)0 && recording_);
         for (int k = 0; k < nsteps; ++k) {
             // a recorded run keeps one particle order so its tapes line up
             if (++since_sort_ >= kSortInterval && !recording_) sort_particles();
@@ -1191,7 +1195,7 @@ public:
         out->n = P_.n;
         out->steps = steps_;
         out->time = time_;
-        out->last_wave_iterations = last_iters_;
+        out->last_wave_iterations = solved_iterations();
         out->fixed_exponent = e_;
         out->systems = 1;
         out->graph_launches = graphs_.launches();
@@ -1217,21 +1221,27 @@ private:
     // segment launches.
     void enqueue_step(real dt, bool graph, int fixed_iters = -1) {
         const uint64_t b0 = graph_base();
+        const bool dev_check = graph && tol_ > (real)0;  // steady check decided on the device
+        const int iters = prm_.wave.iterations;
+        // the solve's launches from arrangement (ip, ic); `launch` false: the rotation only
+        auto solve_launches = [&](int& ip, int& ic, bool launch) {
+            return dev_check ? wave_solve_device(iters, ip, ic, launch) : wave_launches(iters, ip, ic, nullptr, launch);
+        };
         if (graph && !timer.on()) {
             // without stage timing the whole step is one graph launch
             const int ip0 = ip_, ic0 = ic_;
-            const int iters = prm_.wave.iterations;
-            rotate(iters);
+            solve_launches(ip_, ic_, false);
             graphs_.run(s_, b0 + 5, [&] {
                 pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p);
                 pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f,
                                           mass_.p, src_.p, layered());
                 int ip = ip0, ic = ic0;  // from the arrangement before the rotation
-                wave_launches(iters, ip, ic);
+                solve_launches(ip, ic, true);
                 pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
                 pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true);
             });
             last_iters_ = iters;
+            iters_on_device_ = dev_check;
             return;
         }
         segment(RRG_STAGE_DEPOSIT, 1, graph, b0 + 0, [&] {
@@ -1241,15 +1251,16 @@ private:
             pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, mass_.p,
                                       src_.p, layered());
         });
-        const int iters = prm_.wave.iterations;
+        iters_on_device_ = false;
         if (graph) {
             const int ip0 = ip_, ic0 = ic_;
-            const int launches = rotate(iters);
+            const int launches = solve_launches(ip_, ic_, false);
             segment(RRG_STAGE_WAVE, launches, true, b0 + 2, [&] {
                 int ip = ip0, ic = ic0;  // captured from the arrangement before the rotation
-                wave_launches(iters, ip, ic);
+                solve_launches(ip, ic, true);
             });
             last_iters_ = iters;
+            iters_on_device_ = dev_check;
         } else if (fixed_iters >= 0) {  // a recorded count, no checks (recomputation)
             const int t = timer.start(s_, RRG_STAGE_WAVE);
             const int launches = wave_launches(fixed_iters, ip_, ic_);
@@ -1270,29 +1281,79 @@ private:
     int levels(int remaining) const { return fused_ ? pm::wave_fused_levels<real>(model_, remaining) : 1; }
 
     // `iters` stencil iterations without a check; returns the launches.
-    int wave_launches(int iters, int& ip, int& ic) {
+    // `halt`: the device stop flag of a solve checked on the device;
+    // `launch` false: only the buffer rotation the launches would leave.
+    int wave_launches(int iters, int& ip, int& ic, const int* halt = nullptr, bool launch = true) {
         int launches = 0;
         pm::wave_schedule(
             iters, [&](int r) { return levels(r); }, ip, ic,
             [&](int out, int cur, int prev) {
-                pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p, nullptr);
+                if (launch)
+                    pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p, nullptr,
+                                             halt);
                 ++launches;
             },
             [&](int lv, int o1, int o2, int cur, int prev) {
-                pm::wave_fused<real>(s_, model_, k_, lv, phi_[o1].p, phi_[o2].p, phi_[cur].p, phi_[prev].p, src_.p);
+                if (launch)
+                    pm::wave_fused<real>(s_, model_, k_, lv, phi_[o1].p, phi_[o2].p, phi_[cur].p, phi_[prev].p,
+                                         src_.p, halt);
                 ++launches;
             });
         return launches;
     }
 
-    // The buffer rotation of `iters` iterations (wave.cu:271-272) as
-    // wave_launches leaves it, without launching.
-    int rotate(int iters) {
-        int launches = 0;
-        pm::wave_schedule(
-            iters, [&](int r) { return levels(r); }, ip_, ic_, [&](int, int, int) { ++launches; },
-            [&](int, int, int, int, int) { ++launches; });
-        return launches;
+    // solve_wave_equation (wave.cu:230-297) as a fixed launch sequence with
+    // the steady check decided on the device: the stretches between checks
+    // as in wave_launches, each checking iteration (iter > 0,
+    // iter % steady_check_step == 0, after the rotation) one k_wave launch
+    // with the max-reduction followed by the hold rule; after convergence
+    // every launch returns at once and a last launch leaves phi^(n-1), phi^n
+    // where the full schedule ends (ip, ic).  Returns the launches; `launch`
+    // false: only the rotation.
+    int wave_solve_device(int iters, int& ip, int& ic, bool launch) {
+        const int cs = prm_.wave.steady_check_step;
+        const int* halt = reinterpret_cast<const int*>(reinterpret_cast<const char*>(ctl_.p) +
+                                                       offsetof(pm::SolveCtl, done));
+        int launches = 0, iter = 0;
+        if (launch) pm::solve_init<real>(s_, ctl_.p, maxbits_.p);
+        ++launches;
+        while (iter < iters) {
+            int c = ((iter + cs - 1) / cs) * cs;
+            if (c == 0) c = cs;
+            c = std::min(c, iters);
+            if (c > iter) {
+                launches += wave_launches(c - iter, ip, ic, halt, launch);
+                iter = c;
+                continue;
+            }
+            pm::wave_schedule(
+                1, [](int) { return 1; }, ip, ic,
+                [&](int out, int cur, int prev) {
+                    if (launch)
+                        pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p,
+                                                 maxbits_.p, halt);
+                },
+                [](int, int, int, int, int) {});
+            ++iter;
+            if (launch) pm::solve_decide<real>(s_, ctl_.p, maxbits_.p, tol_, prm_.wave.steady_hold_steps, iter, ip, ic);
+            launches += 2;
+        }
+        if (launch) {
+            pm::PhiBufs<real> b{};
+            for (int q = 0; q < nbuf_; ++q) b.p[q] = phi_[q].p;
+            pm::solve_fixup<real>(s_, ctl_.p, b, ip, ic, ncell_);
+        }
+        return launches + 1;
+    }
+
+    // Stencil iterations run by the last solve (a device-checked one is read
+    // back from its control block).
+    int solved_iterations() {
+        if (!iters_on_device_) return last_iters_;
+        pm::SolveCtl c{};
+        RRG_HIP(hipMemcpyAsync(&c, ctl_.p, sizeof(c), hipMemcpyDeviceToHost, s_));
+        RRG_HIP(hipStreamSynchronize(s_));
+        return c.done ? c.iters : last_iters_;
     }
 
     real* cur() { return phi_[ic_].p; }
@@ -1436,6 +1497,7 @@ private:
     // iterations run.
     int solve(int max_iter, int* launches) {
         using MB = std::conditional_t<std::is_same_v<real, float>, unsigned int, unsigned long long>;
+        iters_on_device_ = false;
         const bool checking = tol_ > (real)0;
         const int check_step = prm_.wave.steady_check_step;
         int hold_count = 0;
@@ -1507,6 +1569,8 @@ private:
     bool fused_ = false;  // several iterations per launch
     int nbuf_ = 3;       // potential grids in the rotation
     DBuf<unsigned long long> maxbits_;
+    DBuf<pm::SolveCtl> ctl_;       // steady check decided on the device
+    bool iters_on_device_ = false;  // the last solve's iteration count is in ctl_
     DBuf<double> cpart_;
     PinnedBuf host_;
     int ip_ = 0, ic_ = 1;  // indices of phi^{n-1} (prev) and phi^n (curr)

@@ -250,9 +250,11 @@ void mass_to_source(hipStream_t s, int nx, const real* mass, real dx, real rho_m
 // otherwise.  When `maxbits` is non-null the kernel also accumulates
 // max|new - cur| and max|new| as bit patterns into maxbits[0], maxbits[1]
 // (which must hold zero beforehand).
+// `halt` (optional): a device flag; when it holds nonzero at launch time the
+// kernel returns without touching any grid.
 template <class real>
 void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, real* pn, const real* p,
-                    const real* po, const real* src, void* maxbits);
+                    const real* po, const real* src, void* maxbits, const int* halt = nullptr);
 
 // `levels` (2 or 3) wave_diss_iterations in one launch, (p = phi^n,
 // po = phi^{n-1}) -> (qa = phi^{n+levels-1}, qb = phi^{n+levels}), bitwise
@@ -263,7 +265,27 @@ void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& 
 template <class real> int wave_fused_levels(const WaveModel<real>& m, int remaining);
 template <class real>
 void wave_fused(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, int levels, real* qa, real* qb,
-                const real* p, const real* po, const real* src);
+                const real* p, const real* po, const real* src, const int* halt = nullptr);
+
+// Steady-state check of a solve decided on the device (see pm_kernels.hip):
+// `done` stops the solve's later stencil launches (pass &ctl->done as their
+// halt flag); at convergence the block records the iteration count and the
+// buffers holding phi^(n-1), phi^n.
+struct SolveCtl {
+    int done, hold, iters, ip, ic, pad[3];
+};
+template <class real> struct PhiBufs {
+    real* p[4];
+};
+// done = hold = 0, maxbits = 0.
+template <class real> void solve_init(hipStream_t s, SolveCtl* c, void* maxbits);
+// After the checking launch that completes `iters` iterations with
+// phi^(n-1), phi^n in buffers (ip, ic): apply the hold rule and clear maxbits.
+template <class real>
+void solve_decide(hipStream_t s, SolveCtl* c, void* maxbits, real tol, int hold_need, int iters, int ip, int ic);
+// Converged: move the two live levels into buffers (ip, ic).
+template <class real>
+void solve_fixup(hipStream_t s, const SolveCtl* c, const PhiBufs<real>& b, int ip, int ic, int64_t ncell);
 
 // The buffer schedule of `iters` unchecked iterations of solve_wave_equation
 // (wave.cu:253-272), (ip, ic) = (phi^{n-1}, phi^n): levels(r) iterations

@@ -109,7 +109,11 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
                                                                            const real* __restrict__ p,
                                                                            const real* __restrict__ po,
                                                                            const real* __restrict__ src,
-                                                                           real* __restrict__ sbar, WaveK<real> k) {
+                                                                           real* __restrict__ sbar, WaveK<real> k,
+                                                                           const int* __restrict__ halt) {
+    // a converged solve's remaining launches return at once (uniform, before
+    // any barrier)
+    if (halt != nullptr && *halt != 0) return;
     constexpr int NX = 64 * XJ * RS;
     constexpr int NY = NX / TY, NB = NY * (NX / ZB);
     constexpr int ROWS = TY + 2 * K - 2;
@@ -318,10 +322,10 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
 
 template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST>
 void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* po, const real* src, real* sbar,
-                  const WaveK<real>& k) {
+                  const WaveK<real>& k, const int* halt) {
     constexpr int NX = 64 * XJ * RS;
     k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST>
-        <<<(NX / TY) * (NX / ZB), 64 * RS * (TY + 2 * K - 2), 0, s>>>(qa, qb, p, po, src, sbar, k);
+        <<<(NX / TY) * (NX / ZB), 64 * RS * (TY + 2 * K - 2), 0, s>>>(qa, qb, p, po, src, sbar, k, halt);
 }
 
 // Launch shapes (K, XJ, RS, TY, ZB, PD, YL, XC), from sweeps of
@@ -363,55 +367,55 @@ template <class real> int fused_levels(int nx, int bc, int remaining, int cap = 
 // fused_levels(nx, periodic, levels) == levels.
 template <class real, bool ADJ, bool FIRST>
 void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const real* p, const real* po,
-                  const real* src, real* sbar, const WaveK<real>& k) {
+                  const real* src, real* sbar, const WaveK<real>& k, const int* halt = nullptr) {
     constexpr bool f32 = std::is_same_v<real, float>;
     if constexpr (!ADJ) {
         if (levels == 4) {
             if (nx == 128) {
                 if constexpr (f32)
-                    launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                    launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
                 else
-                    launch_shape<real, 4, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                    launch_shape<real, 4, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
             } else if constexpr (f32) {
-                launch_shape<real, 4, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 4, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
             }
             return;
         }
     }
     if (levels == 3) {
         if (nx == 64) {
-            launch_shape<real, 3, 1, 1, 4, 4, 1, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 1, 1, 4, 4, 1, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
         } else if (nx == 128) {
-            launch_shape<real, 3, 2, 1, 8, 8, f32 ? 2 : 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 2, 1, 8, 8, f32 ? 2 : 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
         } else if (nx == 256) {
             if constexpr (f32)
-                launch_shape<real, 3, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 3, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
             else
-                launch_shape<real, 3, 2, 2, 4, 64, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 3, 2, 2, 4, 64, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
         } else if constexpr (f32) {
-            launch_shape<real, 3, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+            launch_shape<real, 3, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
         }
     } else {
         if (nx == 64) {
             if constexpr (f32)
-                launch_shape<real, 2, 1, 1, 4, 4, 2, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 2, 1, 1, 4, 4, 2, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
             else
-                launch_shape<real, 2, 1, 1, 4, 4, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 2, 1, 1, 4, 4, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
         } else if (nx == 128) {
             if constexpr (f32)
-                launch_shape<real, 2, 1, 2, 4, 8, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 2, 1, 2, 4, 8, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
             else
-                launch_shape<real, 2, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 2, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
         } else if (nx == 256) {
             if constexpr (f32)
-                launch_shape<real, 2, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 2, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
             else
-                launch_shape<real, 2, 2, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 2, 2, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
         } else {
             if constexpr (f32)
-                launch_shape<real, 2, 8, 1, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 2, 8, 1, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
             else
-                launch_shape<real, 2, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k);
+                launch_shape<real, 2, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
         }
     }
 }

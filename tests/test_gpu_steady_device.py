@@ -1,0 +1,92 @@
+"""The steady-state early exit (solve_wave_equation with SteadyStateChecker,
+wave.cu:253-287 and wave_steady_checker.cu:95-143) decided on the device, so
+PM steps with tolerance > 0 stay graph-replayed: every step's iteration count
+and the final fields against the oracle's run (oracle.pm_run restates the
+checker and is pinned by tests/test_oracle_pins.py), with graphs (the
+device-side decision: one graph per step, or one per stage with stage
+timing) and without (the host reads each check back), on the fused
+four- and three-iteration stencils, the single-iteration kernel and the
+non-periodic kernels.  The cases mix steps that stop early with steps that
+run to the iteration limit."""
+import numpy as np
+import pytest
+
+from rrgalaxsim_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+C = 4574.337022617616
+STEPS = 6
+
+CASES = [
+    # nx, dx, tolerance, real, boundary
+    (128, 0.125, 3e-5, "fp32", "periodic"),
+    (128, 0.125, 3e-5, "fp64", "periodic"),
+    (64, 0.25, 1e-5, "fp64", "periodic"),
+    (32, 0.5, 1e-5, "fp32", "periodic"),
+    (32, 0.5, 1e-5, "fp64", "zero_dirichlet"),
+    (24, 0.75, 1e-4, "fp64", "monopole_dirichlet"),
+]
+
+
+def _run(rrg, nx, dx, tol, real, boundary, graphs, timing=False):
+    pos, vel, m = ics.exp_disk(20_000, seed=1)
+    wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=400, iterations=60, tolerance=tol,
+                steady_check_step=5, steady_hold_steps=2)
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=dx, bc_frac=0.1 if boundary != "periodic" else 0.0),
+                        wave=wave, real=real, boundary=boundary, graphs=graphs)
+    iters = []
+    with rrg.Simulation(p, pos, vel, m) as sim:
+        sim.precalc()
+        iters.append(sim.stats()["last_wave_iterations"])
+        sim.sync_particles_and_grid()
+        sim.set_timing(timing)
+        for _ in range(STEPS):
+            sim.calc_step(1)
+            iters.append(sim.stats()["last_wave_iterations"])
+        sim.sync_particles_and_grid()
+        st = sim.state()
+        phi = sim.grid()["phi"]
+        stats = sim.stats()
+    return (pos, vel, m), iters, st, phi, stats
+
+
+@pytest.mark.parametrize("nx,dx,tol,real,boundary", CASES)
+def test_device_checked_steps_match_oracle(rrg, orc, nx, dx, tol, real, boundary):
+    print(f"steady check on the device: nx={nx} tol={tol} {real} {boundary}", flush=True)
+    dtype = np.float32 if real == "fp32" else np.float64
+    (pos, vel, m), it_g, st_g, phi_g, stats = _run(rrg, nx, dx, tol, real, boundary, graphs=True)
+    assert stats["graph_launches"] == STEPS
+    _, it_h, st_h, phi_h, stats_h = _run(rrg, nx, dx, tol, real, boundary, graphs=False)
+    assert stats_h["graph_launches"] == 0
+    o = orc.pm_run(pos, vel, m, STEPS, nx=nx, dx=dx, bc_frac=0.1 if boundary != "periodic" else 0.0, dt=5e-3,
+                   diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=400, iterations=60,
+                   tolerance=tol, check_step=5, hold=2, fixed_e=stats["fixed_exponent"], dtype=dtype,
+                   boundary=boundary)
+    assert it_g == it_h == [int(k) for k in o["iterations"]], (it_g, it_h, list(o["iterations"]))
+    for st, phi in ((st_g, phi_g), (st_h, phi_h)):
+        for k in ("pos", "vel", "acc", "grav"):
+            assert np.array_equal(st[k].astype(dtype), o[k]), k
+        assert np.array_equal(phi.astype(dtype), o["phi"])
+
+
+def test_device_check_mixes_early_and_full_solves(rrg, orc):
+    """The headline grid case above has both kinds of steps (the oracle's
+    counts, so the test states what it covers)."""
+    pos, vel, m = ics.exp_disk(20_000, seed=1)
+    o = orc.pm_run(pos, vel, m, STEPS, nx=128, dx=0.125, dt=5e-3, diss_base=0.785, wave_speed=C,
+                   setup_iterations=400, iterations=60, tolerance=3e-5, check_step=5, hold=2, fixed_e=60,
+                   dtype=np.float32)
+    steps = [int(k) for k in o["iterations"][1:]]
+    assert 60 in steps and any(k < 60 for k in steps), steps
+
+
+def test_device_check_with_stage_graphs(rrg):
+    """Stage timing on: the wave stage is its own graph; same counts and
+    fields as the whole-step graphs."""
+    a = _run(rrg, 128, 0.125, 3e-5, "fp32", "periodic", graphs=True, timing=True)
+    b = _run(rrg, 128, 0.125, 3e-5, "fp32", "periodic", graphs=True, timing=False)
+    assert a[4]["graph_launches"] == 5 * STEPS and a[1] == b[1]
+    for k in ("pos", "vel"):
+        assert np.array_equal(a[2][k], b[2][k]), k
+    assert np.array_equal(a[3], b[3])
