@@ -393,6 +393,52 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2, real="fp32"):
     return out
 
 
+def sample_bench(rrg, ics, steps=3):
+    """The reference's sample run (sample/params.json): 512^3 cells of
+    dx = 0.025 (box [-6.4, 6.4]^3), FP64 (the reference's default real),
+    diss_base 0.1, up to 30000 setup iterations and up to 3000 per step with
+    the steady check (tolerance 1e-6 every 30 iterations, hold 3),
+    dt = 0.002.  Particles: N = 262,144 in a synthetic disk shaped like
+    sample/ini/S.txt (exponential, scale 0.25, truncated at R = 2; S.txt
+    itself is not shipped).  Run twice: the check decided on the device with
+    graph-replayed steps, and decided on the host after each check (a
+    blocking read-back per check, as the reference does).  After one warm-up
+    step, `steps` steps are timed one at a time, each with its iteration
+    count."""
+    pos, vel, m = ics.exp_disk(262_144, seed=6, r_max=8.0)
+    pos, vel = pos * 0.25, vel * 2.0
+    wave = dict(diss_base=0.1, diss_extra=0.01, wave_speed=C, setup_iterations=30000, iterations=3000,
+                tolerance=1e-6, steady_check_step=30, steady_hold_steps=3)
+    out = {}
+    for mode, graphs in (("device_check", True), ("host_check", False)):
+        p = rrg.make_params(dt=0.002, grid=dict(nx=512, dx=0.025, bc_frac=0.2), wave=wave, real="fp64",
+                            graphs=graphs)
+        with rrg.Simulation(p, pos, vel, m) as sim:
+            t0 = time.perf_counter()
+            sim.precalc()
+            sim.synchronize()
+            t1 = time.perf_counter()
+            setup_iters = sim.stats()["last_wave_iterations"]
+            sim.sync_particles_and_grid()
+            sim.calc_step(1)
+            sim.synchronize()
+            ms, iters = [], []
+            for _ in range(steps):
+                a = time.perf_counter()
+                sim.calc_step(1)
+                sim.synchronize()
+                ms.append((time.perf_counter() - a) * 1e3)
+                iters.append(sim.stats()["last_wave_iterations"])
+            graph_launches = sim.stats()["graph_launches"]
+        out[mode] = {"ms_per_step": float(np.mean(ms)), "ms_each": ms, "iterations_each": iters,
+                     "us_per_iteration": float(np.sum(ms) * 1e3 / max(1, np.sum(iters))),
+                     "setup_s": t1 - t0, "setup_iterations": setup_iters, "graph_launches": graph_launches}
+    out["config"] = {"workload": "sample/params.json wave settings: 512^3, dx=0.025, fp64, diss_base=0.1, tolerance "
+                                 "1e-6 every 30 iterations (hold 3), <= 3000 iterations per step, dt=0.002; synthetic "
+                                 "disk N=262144 shaped like sample/ini/S.txt", "timed_steps": steps}
+    return out
+
+
 def c1_bench(rrg, ics, n=4096, steps=100):
     """Plain FP64 direct sum: Plummer N=4096 (seed 0), eps=0.01, dt=1e-3, 100
     steps; interactions/s over the steps and the relative energy drift."""
@@ -681,7 +727,7 @@ def main():
                     help="with --profile-pass: the headline PM run (pm) or the direct-sum kinds on the disk + bulge "
                          "workload (direct)")
     ap.add_argument("--skip", default="", help="comma-separated legs to leave out: direct, c1, scaling, ensemble, "
-                                              "adjoint, adjoint_pm, pm_fp64, cpu")
+                                              "adjoint, adjoint_pm, pm_fp64, sample, cpu")
     args = ap.parse_args()
     skip = set(x for x in args.skip.split(",") if x)
     t_start = time.perf_counter()
@@ -770,7 +816,8 @@ def main():
                 ("ensemble", "ensemble", lambda: ensemble_bench(rrg, ics)),
                 ("adjoint", "adjoint", lambda: adjoint_bench(rrg, ics)),
                 ("adjoint_pm", "adjoint_pm", lambda: pm_adjoint_bench(rrg, ics)),
-                ("pm_fp64", "pm_fp64", lambda: pm_fp64_bench(rrg, ics))]
+                ("pm_fp64", "pm_fp64", lambda: pm_fp64_bench(rrg, ics)),
+                ("sample", "sample_workload", lambda: sample_bench(rrg, ics))]
         for leg, key, fn in legs:
             if leg in skip:
                 continue

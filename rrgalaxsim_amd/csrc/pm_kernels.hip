@@ -396,6 +396,35 @@ template <class real> __device__ __forceinline__ real wave_max(real v) {
     return v;
 }
 
+// Max-reduction of a checking launch: the workgroup's maxima (waves, then
+// LDS) go to one of kCheckSlots slot pairs chosen by workgroup index, so the
+// atomics of a large grid spread over many addresses instead of queueing on
+// one; readers take the max over the slots.  Every thread of the workgroup
+// must call it.
+template <class real>
+__device__ __forceinline__ void check_accumulate(real dmax, real amax, typename MaxBits<real>::T* maxbits) {
+    __shared__ real sd[16], sa[16];
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
+    dmax = wave_max(dmax);
+    amax = wave_max(amax);
+    if ((t & 63) == 0) {
+        sd[t >> 6] = dmax;
+        sa[t >> 6] = amax;
+    }
+    __syncthreads();
+    if (t == 0) {
+        real d = sd[0], a = sa[0];
+        for (int w = 1; w < nw; ++w) {
+            d = fmax(d, sd[w]);
+            a = fmax(a, sa[w]);
+        }
+        const unsigned slot = (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % unsigned(kCheckSlots);
+        atomicMax(maxbits + slot, to_bits<real>(d));
+        atomicMax(maxbits + kCheckSlots + slot, to_bits<real>(a));
+    }
+}
+
 __device__ __forceinline__ void xcd_block(int nbx, int nby, int nbz, int& bx, int& by, int& bz) {
     const int nb = nbx * nby * nbz;
     const int b = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
@@ -462,14 +491,7 @@ __global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real*
             cen = above;
         }
     }
-    if (CHECK) {
-        dmax = wave_max(dmax);
-        amax = wave_max(amax);
-        if (((threadIdx.y * 64 + threadIdx.x) & 63) == 0) {
-            atomicMax(maxbits + 0, to_bits<real>(dmax));
-            atomicMax(maxbits + 1, to_bits<real>(amax));
-        }
-    }
+    if (CHECK) check_accumulate<real>(dmax, amax, maxbits);
 }
 
 // ------------------------------------------- steady check on the device
@@ -482,31 +504,55 @@ __global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real*
 // returns at once, and a last launch moves the two live levels into the
 // buffers the full schedule would have left them in.
 template <class real>
-__global__ void k_solve_init(SolveCtl* __restrict__ c, typename MaxBits<real>::T* __restrict__ maxbits) {
-    c->done = 0;
-    c->hold = 0;
-    c->iters = 0;
-    c->ip = -1;
-    c->ic = -1;
-    maxbits[0] = 0;
-    maxbits[1] = 0;
+__global__ __launch_bounds__(kCheckSlots) void k_solve_init(SolveCtl* __restrict__ c,
+                                                          typename MaxBits<real>::T* __restrict__ maxbits) {
+    const int t = threadIdx.x;
+    if (t == 0) {
+        c->done = 0;
+        c->hold = 0;
+        c->iters = 0;
+        c->ip = -1;
+        c->ic = -1;
+    }
+    maxbits[t] = 0;
+    maxbits[kCheckSlots + t] = 0;
 }
 
 // After the checking launch that completes `iters` iterations and leaves
-// phi^(n-1), phi^n in buffers (ip, ic): err = diff / (absu + 1e-30) in real
-// (wave_steady_checker.cu:119-126), then SteadyStateChecker::check.
+// phi^(n-1), phi^n in buffers (ip, ic): the max over the slots, then
+// err = diff / (absu + 1e-30) in real (wave_steady_checker.cu:119-126) and
+// SteadyStateChecker::check; the slots are cleared for the next check.
 template <class real>
-__global__ void k_solve_decide(SolveCtl* __restrict__ c, typename MaxBits<real>::T* __restrict__ maxbits, real tol,
-                               int hold_need, int iters, int ip, int ic) {
-    if (c->done) return;
+__global__ __launch_bounds__(kCheckSlots) void k_solve_decide(SolveCtl* __restrict__ c,
+                                                            typename MaxBits<real>::T* __restrict__ maxbits, real tol,
+                                                            int hold_need, int iters, int ip, int ic) {
     using MB = typename MaxBits<real>::T;
-    const MB bd = maxbits[0], ba = maxbits[1];
-    real diff, absu;
     static_assert(sizeof(MB) == sizeof(real), "bit pattern width");
+    __shared__ MB sd[kCheckSlots / 64], sa[kCheckSlots / 64];
+    if (c->done) return;  // uniform
+    const int t = threadIdx.x;
+    MB bd = maxbits[t], ba = maxbits[kCheckSlots + t];
+    maxbits[t] = 0;
+    maxbits[kCheckSlots + t] = 0;
+    // non-negative values: the bit patterns order as the values do
+    for (int off = 32; off > 0; off >>= 1) {
+        const MB od = __shfl_xor(bd, off, 64), oa = __shfl_xor(ba, off, 64);
+        bd = od > bd ? od : bd;
+        ba = oa > ba ? oa : ba;
+    }
+    if ((t & 63) == 0) {
+        sd[t >> 6] = bd;
+        sa[t >> 6] = ba;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    for (int w = 1; w < kCheckSlots / 64; ++w) {
+        bd = sd[w] > bd ? sd[w] : bd;
+        ba = sa[w] > ba ? sa[w] : ba;
+    }
+    real diff, absu;
     __builtin_memcpy(&diff, &bd, sizeof(real));
     __builtin_memcpy(&absu, &ba, sizeof(real));
-    maxbits[0] = 0;
-    maxbits[1] = 0;
     const real EPS = (real)1.E-30;
     const real err = diff / (absu + EPS);
     if ((double)err < (double)tol) ++c->hold;
@@ -543,12 +589,7 @@ __global__ __launch_bounds__(256) void k_max_diff(int64_t ncell, const real* __r
         dmax = fmax(dmax, fabs(a[c] - b[c]));
         amax = fmax(amax, fabs(a[c]));
     }
-    dmax = wave_max(dmax);
-    amax = wave_max(amax);
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(maxbits + 0, to_bits<real>(dmax));
-        atomicMax(maxbits + 1, to_bits<real>(amax));
-    }
+    check_accumulate<real>(dmax, amax, maxbits);
 }
 
 // wave_diss_iteration on a non-periodic grid: per-cell coefficients, face
@@ -592,14 +633,7 @@ __global__ __launch_bounds__(256) void k_wave_bc(real* __restrict__ pn, const re
             }
         }
     }
-    if (CHECK) {
-        dmax = wave_max(dmax);
-        amax = wave_max(amax);
-        if (((threadIdx.y * 64 + threadIdx.x) & 63) == 0) {
-            atomicMax(maxbits + 0, to_bits<real>(dmax));
-            atomicMax(maxbits + 1, to_bits<real>(amax));
-        }
-    }
+    if (CHECK) check_accumulate<real>(dmax, amax, maxbits);
 }
 
 // Per-cell coefficient grids of a non-periodic grid (transposed stencil).
@@ -1045,14 +1079,14 @@ template <class real> void max_diff(hipStream_t s, int64_t ncell, const real* a,
 }
 
 template <class real> void solve_init(hipStream_t s, SolveCtl* c, void* maxbits) {
-    k_solve_init<real><<<1, 1, 0, s>>>(c, reinterpret_cast<typename MaxBits<real>::T*>(maxbits));
+    k_solve_init<real><<<1, kCheckSlots, 0, s>>>(c, reinterpret_cast<typename MaxBits<real>::T*>(maxbits));
     RRG_LAUNCH_CHECK();
 }
 
 template <class real>
 void solve_decide(hipStream_t s, SolveCtl* c, void* maxbits, real tol, int hold_need, int iters, int ip, int ic) {
-    k_solve_decide<real><<<1, 1, 0, s>>>(c, reinterpret_cast<typename MaxBits<real>::T*>(maxbits), tol, hold_need,
-                                         iters, ip, ic);
+    k_solve_decide<real><<<1, kCheckSlots, 0, s>>>(c, reinterpret_cast<typename MaxBits<real>::T*>(maxbits), tol,
+                                                   hold_need, iters, ip, ic);
     RRG_LAUNCH_CHECK();
 }
 

@@ -144,6 +144,17 @@ void zero_async(hipStream_t s, void* p, size_t bytes) {
     if (bytes) RRG_HIP(hipMemsetAsync(p, 0, bytes, s));
 }
 
+// The maxima of a checking launch's slots (non-negative values, so the bit
+// patterns order as the values do): out[0] = max|new - cur|, out[1] = max|new|.
+template <class MB> void max_over_slots(const void* slots, MB* out) {
+    const MB* v = static_cast<const MB*>(slots);
+    out[0] = out[1] = 0;
+    for (int q = 0; q < pm::kCheckSlots; ++q) {
+        out[0] = std::max(out[0], v[q]);
+        out[1] = std::max(out[1], v[pm::kCheckSlots + q]);
+    }
+}
+
 // Force arithmetic for a direct-sum evaluation; detects uniform softening
 // (then h_i + h_j is folded into one constant, formed in the kernel's
 // precision exactly as the per-pair sum would be).
@@ -969,7 +980,7 @@ public:
         src_.alloc(ncell_);
         for (int i = 0; i < nbuf_; ++i) phi_[i].alloc(ncell_);
         gacc_.alloc(4 * ncell_);
-        maxbits_.alloc(2);
+        maxbits_.alloc(2 * pm::kCheckSlots);
         ctl_.alloc(1);
         zero_async(s_, ctl_.p, ctl_.bytes());
         zero_async(s_, mfix_.p, mfix_.bytes());
@@ -1516,7 +1527,7 @@ private:
                 iter = c;
                 continue;
             }
-            pm::fill_u64(s_, reinterpret_cast<unsigned long long*>(maxbits_.p), 2, 0ull);
+            pm::fill_u64(s_, maxbits_.p, 2 * pm::kCheckSlots, 0ull);
             pm::wave_schedule(
                 1, [](int) { return 1; }, ip_, ic_,
                 [&](int out, int cur, int prev) {
@@ -1526,11 +1537,11 @@ private:
                 [](int, int, int, int, int) {});
             ++*launches;
             ++iter;
-            host_.ensure(sizeof(MB) * 2);
-            RRG_HIP(hipMemcpyAsync(host_.p, maxbits_.p, sizeof(MB) * 2, hipMemcpyDeviceToHost, s_));
+            host_.ensure(sizeof(MB) * 2 * pm::kCheckSlots);
+            RRG_HIP(hipMemcpyAsync(host_.p, maxbits_.p, sizeof(MB) * 2 * pm::kCheckSlots, hipMemcpyDeviceToHost, s_));
             RRG_HIP(hipStreamSynchronize(s_));
             MB bits[2];
-            std::memcpy(bits, host_.p, sizeof(bits));
+            max_over_slots<MB>(host_.p, bits);
             real diff, absu;
             std::memcpy(&diff, &bits[0], sizeof(real));
             std::memcpy(&absu, &bits[1], sizeof(real));
@@ -2099,13 +2110,15 @@ template <class real> void steady_op(int64_t ncell, const void* un, const void* 
     using MB = std::conditional_t<std::is_same_v<real, float>, unsigned int, unsigned long long>;
     OpStream os;
     DBuf<real> a(ncell), b(ncell);
-    DBuf<unsigned long long> mb(2);
+    DBuf<unsigned long long> mb(2 * pm::kCheckSlots);
     h2d(os.s, a.p, un, sizeof(real) * ncell);
     h2d(os.s, b.p, uo, sizeof(real) * ncell);
-    pm::fill_u64(os.s, mb.p, 2, 0ull);
+    pm::fill_u64(os.s, mb.p, 2 * pm::kCheckSlots, 0ull);
     pm::max_diff<real>(os.s, ncell, a.p, b.p, mb.p);
+    std::vector<MB> slots(2 * pm::kCheckSlots);
+    d2h(os.s, slots.data(), reinterpret_cast<MB*>(mb.p), sizeof(MB) * slots.size());
     MB bits[2];
-    d2h(os.s, bits, reinterpret_cast<MB*>(mb.p), sizeof(bits));
+    max_over_slots<MB>(slots.data(), bits);
     real diff, absu;
     std::memcpy(&diff, &bits[0], sizeof(real));
     std::memcpy(&absu, &bits[1], sizeof(real));

@@ -245,11 +245,16 @@ template <class real>
 void mass_to_source(hipStream_t s, int nx, const real* mass, real dx, real rho_mean, bool periodic, real k_f,
                     real* src, const WaveModel<real>* model = nullptr);
 
+// Slots of a checking launch's max-reduction: maxbits holds 2 kCheckSlots
+// bit patterns of `real` (max|new - cur| in [0, kCheckSlots), max|new| in
+// [kCheckSlots, 2 kCheckSlots)), zero before the launch; the maxima are the
+// maxima over the slots.
+constexpr int kCheckSlots = 256;
+
 // One wave_diss_iteration over the grid: uniform coefficients `k` on a
 // periodic grid, per-cell coefficients and boundary values from `m`
 // otherwise.  When `maxbits` is non-null the kernel also accumulates
-// max|new - cur| and max|new| as bit patterns into maxbits[0], maxbits[1]
-// (which must hold zero beforehand).
+// max|new - cur| and max|new| as bit patterns into its slots (above).
 // `halt` (optional): a device flag; when it holds nonzero at launch time the
 // kernel returns without touching any grid.
 template <class real>
@@ -333,7 +338,7 @@ void gather(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real
 // NGP phi projection onto p.g (rm_p2mesh.cu:34-53, rm_mesh2p.cu:29-38).
 template <class real> void ngp_phi(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real* phi);
 
-// Standalone max|a-b|, max|a| reduction (bit patterns, see wave_iteration).
+// Standalone max|a-b|, max|a| reduction into the slots of `maxbits` (see kCheckSlots).
 template <class real> void max_diff(hipStream_t s, int64_t ncell, const real* a, const real* b, void* maxbits);
 
 // NGP counting sort of the particles by cell (the binning of
