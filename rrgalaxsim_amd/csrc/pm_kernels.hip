@@ -39,11 +39,23 @@ template <class real> __device__ __forceinline__ real clampr(real v, real lo, re
 }
 
 // ------------------------------------------------------------------ deposit
-// CIC stencil of deposit_cell_mass_CIC_ at position r: the 8 cells and the 8
-// fixed-point contributions rint(m*w * 2^e).  Returns the base cell index.
-template <class real>
+// rint(x) as an integer for |x| < 2^51: x + 1.5 * 2^52 lands in [2^52, 2^53),
+// where doubles are the integers, so the addition rounds x to the nearest
+// integer (ties to even, as rint in the default rounding mode) and the bit
+// patterns of the sum and of 1.5 * 2^52 differ by that integer.
+__device__ __forceinline__ long long rint_small(double x) {
+    const double magic = 6755399441055744.0;  // 1.5 * 2^52
+    return __double_as_longlong(x + magic) - __double_as_longlong(magic);
+}
+
+// CIC stencil of deposit_cell_mass_CIC_ at position r: the 8 cells, the 8
+// fixed-point contributions rint(m*w * 2^e) and the base cell (i0, j0, k0)
+// with the upper corner's steps (0 or 1 per axis; 0 where clamped onto the
+// last cell).  Returns the base cell index.  FASTQ: |m| * 2^e < 2^51 for
+// every particle, so the rounding can take rint_small.
+template <class real, bool FASTQ>
 __device__ __forceinline__ unsigned cic_fixed(const real r[3], real m, const Geo<real>& g, double scale, long long q[8],
-                                              unsigned c[8]) {
+                                              unsigned c[8], int base[3], int step[3]) {
     const int NX = g.nx;
     real u = (r[0] - g.domain_min) * g.inv_dx - (real)0.5f;
     real v = (r[1] - g.domain_min) * g.inv_dx - (real)0.5f;
@@ -57,6 +69,8 @@ __device__ __forceinline__ unsigned cic_fixed(const real r[3], real m, const Geo
     const int j0 = max(0, min((int)floor(v), NX - 1));
     const int k0 = max(0, min((int)floor(w), NX - 1));
     const int i1 = min(i0 + 1, NX - 1), j1 = min(j0 + 1, NX - 1), k1 = min(k0 + 1, NX - 1);
+    base[0] = i0; base[1] = j0; base[2] = k0;
+    step[0] = i1 - i0; step[1] = j1 - j0; step[2] = k1 - k0;
     const real fx = u - (real)i0, fy = v - (real)j0, fz = w - (real)k0;
     const real wx0 = (real)1.f - fx, wx1 = fx;
     const real wy0 = (real)1.f - fy, wy1 = fy;
@@ -73,21 +87,29 @@ __device__ __forceinline__ unsigned cic_fixed(const real r[3], real m, const Geo
     const real wt[8] = {wx0 * wy0 * wz0, wx1 * wy0 * wz0, wx0 * wy1 * wz0, wx1 * wy1 * wz0,
                         wx0 * wy0 * wz1, wx1 * wy0 * wz1, wx0 * wy1 * wz1, wx1 * wy1 * wz1};
 #pragma unroll
-    for (int k = 0; k < 8; ++k) q[k] = (long long)rint((double)(m * wt[k]) * scale);
+    for (int k = 0; k < 8; ++k) {
+        const double x = (double)(m * wt[k]) * scale;
+        q[k] = FASTQ ? rint_small(x) : (long long)rint(x);
+    }
     return c[0];
 }
 
-// Predictor fused with the deposit.  Lanes of a wave whose particles share a
-// base cell (runs of equal keys; the particle arrays are kept sorted by cell,
-// so dense regions form long runs) are merged by a segmented inclusive scan
-// over the wave; the last lane of each run then adds its 8 corner sums into a
-// workgroup table in LDS when its base cell is interior and at most
-// kDepositWindow-1 cells above the workgroup's smallest key (corner rows
-// (dy, dz) x offsets 0 .. kDepositWindow), and the workgroup issues one
-// global atomic per nonzero entry; other runs issue their 8 atomics directly.
-// Integer addition is exact, so the grid does not depend on the merging.
+// Predictor fused with the deposit.  Adjacent lanes of a wave whose
+// particles share a base cell (runs of equal keys; the particle arrays are
+// kept sorted by cell, so dense regions form long runs) are merged by a
+// segmented inclusive scan over the wave; the last lane of each run then adds
+// its 8 corner sums into a workgroup grid in LDS spanning the bounding box of
+// the workgroup's CIC stencils, and the workgroup adds each nonzero box cell
+// to the global grid with one atomic.  Particles move less than a cell
+// between the periodic re-sorts, so the 256 particles of a workgroup stay in
+// a small box (median 60 cells, 99% within 1,711, on the 10^6-particle disk
+// at 128^3, 16 steps after a sort) and the LDS atomics absorb the runs that
+// the drift splits (cheaper than re-sorting each wave: 35 against 39 us per
+// step on that disk); a workgroup whose box exceeds kDepositBox cells
+// issues each run's 8 atomics directly.  Integer addition is exact, so the
+// grid does not depend on the merging.
 constexpr unsigned kNoCell = 0xffffffffu;
-constexpr int kDepositWindow = 256;
+constexpr int kDepositBox = 2048;
 
 // Lane values moved by a DPP pattern (row_shr:n = 0x110+n within rows of 16
 // lanes, row_bcast:15 = 0x142, row_bcast:31 = 0x143); lanes without a source
@@ -113,19 +135,35 @@ template <int CTRL, int ROWS> __device__ __forceinline__ void seg_step(long long
     f |= fs;
 }
 
-template <class real>
+// Minimum / maximum over the wave by DPP row shifts and row broadcasts (the
+// pattern of the scan above), read from lane 63; `ident` fills lanes
+// without a source.
+template <bool MAX> __device__ __forceinline__ int wave_ext(int v, int ident) {
+    auto step = [&](int t) { v = MAX ? max(v, t) : min(v, t); };
+    step(__builtin_amdgcn_update_dpp(ident, v, 0x111, 0xf, 0xf, false));
+    step(__builtin_amdgcn_update_dpp(ident, v, 0x112, 0xf, 0xf, false));
+    step(__builtin_amdgcn_update_dpp(ident, v, 0x114, 0xf, 0xf, false));
+    step(__builtin_amdgcn_update_dpp(ident, v, 0x118, 0xf, 0xf, false));
+    step(__builtin_amdgcn_update_dpp(ident, v, 0x142, 0xa, 0xf, false));
+    step(__builtin_amdgcn_update_dpp(ident, v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+template <class real, bool FASTQ>
 __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt, int do_predict, Geo<real> g,
                                                          double scale, unsigned long long* __restrict__ mfix) {
-    __shared__ unsigned long long tab[4][kDepositWindow + 1];
-    __shared__ unsigned kmin_s;
-    for (int e = threadIdx.x; e < 4 * (kDepositWindow + 1); e += blockDim.x) (&tab[0][0])[e] = 0ull;
-    if (threadIdx.x == 0) kmin_s = kNoCell;
+    __shared__ unsigned long long box[kDepositBox];
+    __shared__ int bnd[6];  // the workgroup's min i0, j0, k0 and max i1, j1, k1
+    for (int e = threadIdx.x; e < kDepositBox; e += blockDim.x) box[e] = 0ull;
+    if (threadIdx.x < 6) bnd[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : -1;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t n = p.n;
+    const int NX = g.nx;
     unsigned key = kNoCell;  // lanes past n keep it: every thread reaches the barriers below
     long long q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     unsigned c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int base[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, step[3] = {0, 0, 0};
     if (i < n) {
         real r[3];
 #pragma unroll
@@ -140,43 +178,24 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
                 p.x[d * n + i] = r[d];
             }
         }
-        key = cic_fixed<real>(r, p.m[i], g, scale, q, c);
+        key = cic_fixed<real, FASTQ>(r, p.m[i], g, scale, q, c, base, step);
     }
-    // Bitonic sort of the wave's (key, lane) pairs with shuffles, so equal
-    // cells become adjacent lanes even where the particle order has drifted
-    // since the last global sort (without it the deposit takes ~3x longer
-    // on the headline disk); each lane then takes the contributions of the
-    // particle it was sorted to.  Ties keep lane order, so a wave whose keys
-    // are already in order (most of them just after a global sort) would be
-    // sorted to itself and skips it.
-    const unsigned kin = __shfl_up(key, 1, 64);
-    if (!__all(lane == 0 || kin <= key)) {
-        unsigned sk = key;
-        int src = lane;
+    // the wave's part of the bounding box of the stencils (lanes past n add
+    // nothing: base INT_MAX, base + step -1 is below every corner)
+    int lo[3], hi[3];
 #pragma unroll
-        for (int size = 2; size <= 64; size <<= 1) {
-#pragma unroll
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                const unsigned ok = __shfl_xor(sk, stride, 64);
-                const int os = __shfl_xor(src, stride, 64);
-                const bool up = (lane & size) == 0;
-                const bool lower = (lane & stride) == 0;
-                const bool other_smaller = ok < sk || (ok == sk && os < src);
-                if (lower == up ? other_smaller : !other_smaller) {
-                    sk = ok;
-                    src = os;
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            q[k] = __shfl(q[k], src, 64);
-            c[k] = __shfl(c[k], src, 64);
-        }
-        key = sk;
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = wave_ext<false>(base[d], 0x7fffffff);
+        hi[d] = wave_ext<true>(i < n ? base[d] + step[d] : -1, -1);
     }
-    __syncthreads();                          // table cleared, kmin_s set
-    if (lane == 0) atomicMin(&kmin_s, key);  // the wave's smallest key, its keys being in order
+    __syncthreads();  // box cleared, bounds initialised
+    if (lane == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&bnd[d], lo[d]);
+            atomicMax(&bnd[3 + d], hi[d]);
+        }
+    }
     const unsigned kprev = __shfl_up(key, 1, 64);
     const unsigned knext = __shfl_down(key, 1, 64);
     int f = (lane == 0 || kprev != key) ? 1 : 0;  // run head
@@ -188,16 +207,17 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
     seg_step<0x118, 0xf>(q, f);
     seg_step<0x142, 0xa>(q, f);
     seg_step<0x143, 0xc>(q, f);
-    __syncthreads();  // kmin_s final
-    const unsigned kmin = kmin_s;
-    const unsigned sy = unsigned(g.nx), sz = unsigned(g.nx) * unsigned(g.nx);
+    __syncthreads();  // bounds final
+    const int bx = bnd[0], by = bnd[1], bz = bnd[2];
+    const int ex = bnd[3] - bx + 1, ey = bnd[4] - by + 1, ez = bnd[5] - bz + 1;
+    const bool in_box = bnd[3] >= 0 && ex * ey * ez <= kDepositBox;  // uniform over the workgroup
     if (tail && key != kNoCell) {
-        const unsigned off = key - kmin;  // key >= kmin
-        const bool interior = c[1] == c[0] + 1u && c[2] == c[0] + sy && c[4] == c[0] + sz;
-        if (interior && off < unsigned(kDepositWindow)) {
+        if (in_box) {
+            const int di = int(c[1] != c[0]), dj = int(c[2] != c[0]) * ex, dk = int(c[4] != c[0]) * ex * ey;
+            const int b0 = (base[0] - bx) + ex * ((base[1] - by) + ey * (base[2] - bz));
 #pragma unroll
-            for (int k = 0; k < 8; ++k)  // corner k: x offset k&1, row (dy, dz) = k>>1
-                if (q[k] != 0) atomicAdd(&tab[k >> 1][off + (k & 1)], (unsigned long long)q[k]);
+            for (int k = 0; k < 8; ++k)  // corner k: steps (k & 1, (k >> 1) & 1, k >> 2); adding 0 is harmless
+                atomicAdd(&box[b0 + (k & 1) * di + ((k >> 1) & 1) * dj + (k >> 2) * dk], (unsigned long long)q[k]);
         } else {
 #pragma unroll
             for (int k = 0; k < 8; ++k)
@@ -205,13 +225,15 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
         }
     }
     __syncthreads();
-    if (kmin != kNoCell) {
-        for (int e = threadIdx.x; e < 4 * (kDepositWindow + 1); e += blockDim.x) {
-            const unsigned long long v = (&tab[0][0])[e];
+    if (in_box) {
+        const int exy = ex * ey, vol = exy * ez;
+        for (int e = threadIdx.x; e < vol; e += blockDim.x) {
+            const unsigned long long v = box[e];
             if (v != 0ull) {
-                const unsigned row = unsigned(e) / unsigned(kDepositWindow + 1);
-                const unsigned t = unsigned(e) - row * unsigned(kDepositWindow + 1);
-                atomicAdd(mfix + (kmin + t + (row & 1u) * sy + (row >> 1) * sz), v);
+                const int lk = e / exy, r = e - lk * exy;
+                const int lj = r / ex, li = r - lj * ex;
+                atomicAdd(mfix + (unsigned(bx + li) + unsigned(NX) * (unsigned(by + lj) + unsigned(NX) * unsigned(bz + lk))),
+                          v);
             }
         }
     }
@@ -897,9 +919,12 @@ template <class real> WaveK<real> wave_coef(real diss, real C, real DT, real DX)
 
 template <class real>
 void predict_deposit(hipStream_t s, const PState<real>& p, real dt, bool predict, const Geo<real>& g, double scale,
-                     unsigned long long* mfix) {
+                     unsigned long long* mfix, double max_q) {
     if (p.n == 0) return;
-    k_predict_deposit<real><<<grid1d(p.n, 256), 256, 0, s>>>(p, dt, predict ? 1 : 0, g, scale, mfix);
+    if (max_q < 2251799813685248.0)  // 2^51
+        k_predict_deposit<real, true><<<grid1d(p.n, 256), 256, 0, s>>>(p, dt, predict ? 1 : 0, g, scale, mfix);
+    else
+        k_predict_deposit<real, false><<<grid1d(p.n, 256), 256, 0, s>>>(p, dt, predict ? 1 : 0, g, scale, mfix);
     RRG_LAUNCH_CHECK();
 }
 
@@ -1150,7 +1175,7 @@ size_t ngp_temp_bytes(int64_t n, int nx) {
     template Geo<real> make_geo<real>(int, double, double);                                                     \
     template WaveK<real> wave_coef<real>(real, real, real, real);                                               \
     template void predict_deposit<real>(hipStream_t, const PState<real>&, real, bool, const Geo<real>&, double, \
-                                        unsigned long long*);                                                   \
+                                        unsigned long long*, double);                                           \
     template void fixed_to_source<real>(hipStream_t, int, unsigned long long*, double, real, real, bool, real,  \
                                         real*, real*, const WaveModel<real>*);                                  \
     template WaveModel<real> make_model<real>(int, double, double, double, double, double, double, int);        \

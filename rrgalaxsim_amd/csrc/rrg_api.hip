@@ -720,6 +720,9 @@ private:
         grid_e_ = e;
         scale_ = std::ldexp(1.0, e);
         inv_scale_ = std::ldexp(1.0, -e);
+        double mmax = 0;
+        for (int64_t i = 0; i < n; ++i) mmax = std::max(mmax, std::fabs(mass[i]));
+        qmax_ = mmax * scale_;
         mfix_.alloc(ncell_);
         gmass_.alloc(ncell_);
         gphi_.alloc(ncell_);
@@ -733,7 +736,7 @@ private:
     // convert_particles_to_grid (CIC mass, rm_p2mesh.cu:398-451) and the NGP
     // projections of potential and acceleration (rm_p2mesh.cu:453-485).
     void project_grid() {
-        pm::predict_deposit<double>(s_, P_.view(), 0.0, false, geo_, scale_, mfix_.p);
+        pm::predict_deposit<double>(s_, P_.view(), 0.0, false, geo_, scale_, mfix_.p, qmax_);
         pm::fixed_to_mass<double>(s_, nx_, mfix_.p, inv_scale_, gmass_.p);
         pm::NgpBins b;
         b.keys = bkeys_.p; b.keys2 = bkeys2_.p; b.idx = bidx_.p; b.order = border_.p;
@@ -894,7 +897,7 @@ private:
     int nx_ = 0;
     int64_t ncell_ = 0;
     pm::Geo<double> geo_{};
-    double scale_ = 0, inv_scale_ = 0;
+    double scale_ = 0, inv_scale_ = 0, qmax_ = 0;  // qmax_: max |m| * scale_
     int grid_e_ = 0;
     DBuf<unsigned long long> mfix_;
     DBuf<double> gmass_, gphi_, gacc_;
@@ -950,6 +953,11 @@ public:
         require(std::ldexp(std::max(msum, 0.0), e_) < std::ldexp(1.0, 62), "rrg_create: fixed_exponent too large for total mass");
         scale_ = std::ldexp(1.0, e_);
         inv_scale_ = std::ldexp(1.0, -e_);
+        {
+            double mmax = 0;  // the state's masses are `real`: bound them as stored
+            for (int64_t i = 0; i < n; ++i) mmax = std::max(mmax, std::fabs((double)(real)mass[i]));
+            qmax_ = mmax * scale_;
+        }
 
         P_.alloc(n);
         upload_vec<real>(s_, n, pos, P_.X.p);
@@ -1243,7 +1251,7 @@ private:
             const int ip0 = ip_, ic0 = ic_;
             solve_launches(ip_, ic_, false);
             graphs_.run(s_, b0 + 5, [&] {
-                pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p);
+                pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p, qmax_);
                 pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f,
                                           mass_.p, src_.p, layered());
                 int ip = ip0, ic = ic0;  // from the arrangement before the rotation
@@ -1256,7 +1264,7 @@ private:
             return;
         }
         segment(RRG_STAGE_DEPOSIT, 1, graph, b0 + 0, [&] {
-            pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p);
+            pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p, qmax_);
         });
         segment(RRG_STAGE_SOURCE, 1, graph, b0 + 1, [&] {
             pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, mass_.p,
@@ -1494,7 +1502,7 @@ private:
 
     void deposit(bool predict) {
         int t = timer.start(s_, RRG_STAGE_DEPOSIT);
-        pm::predict_deposit<real>(s_, P_.view(), (real)prm_.sim.dt_dynamics, predict, geo_, scale_, mfix_.p);
+        pm::predict_deposit<real>(s_, P_.view(), (real)prm_.sim.dt_dynamics, predict, geo_, scale_, mfix_.p, qmax_);
         timer.stop(s_, t, 1);
         t = timer.start(s_, RRG_STAGE_SOURCE);
         pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, mass_.p, src_.p,
@@ -1565,7 +1573,7 @@ private:
     pm::WaveModel<real> model_{};
     DBuf<unsigned long long> com_scratch_;
     int e_ = 61;
-    double scale_ = 0, inv_scale_ = 0;
+    double scale_ = 0, inv_scale_ = 0, qmax_ = 0;  // qmax_: max |m| * scale_
     Particles<real> P_, P2_;
     DBuf<int> id_, id2_;
     DBuf<unsigned> keys_, keys2_, idx_, idx2_;
@@ -2008,7 +2016,9 @@ void cic_op(int64_t n, const void* pos, const void* mass, int32_t nx, double dx,
     DBuf<unsigned long long> g(ncell);
     pm::fill_u64(os.s, g.p, ncell, 0ull);
     const pm::Geo<real> geo = pm::make_geo<real>(nx, dx, bc_frac);
-    pm::predict_deposit<real>(os.s, P.view(), (real)0, false, geo, std::ldexp(1.0, e), g.p);
+    double mmax = 0;
+    for (int64_t i = 0; i < n; ++i) mmax = std::max(mmax, std::fabs((double)static_cast<const real*>(mass)[i]));
+    pm::predict_deposit<real>(os.s, P.view(), (real)0, false, geo, std::ldexp(1.0, e), g.p, mmax * std::ldexp(1.0, e));
     d2h(os.s, grid, g.p, sizeof(int64_t) * ncell);
 }
 

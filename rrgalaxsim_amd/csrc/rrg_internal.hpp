@@ -216,10 +216,11 @@ void center_of_mass(hipStream_t s, const PState<real>& p, int e_mass, unsigned l
 
 // predict_step_ fused with deposit_cell_mass_CIC_ at the new position into an
 // int64 fixed-point grid (scale = 2^e, exact).  When `predict` is false only
-// the deposit runs (precalc).
+// the deposit runs (precalc).  max_q: an upper bound of |m| * scale over the
+// particles (it selects the cheaper exact rounding below 2^51).
 template <class real>
 void predict_deposit(hipStream_t s, const PState<real>& p, real dt, bool predict, const Geo<real>& g,
-                     double scale, unsigned long long* mfix);
+                     double scale, unsigned long long* mfix, double max_q);
 
 // Spatial ordering: keys[i] = CIC base cell of particle i, idx[i] = i.
 template <class real>

@@ -13,14 +13,14 @@
   FP64 direct sum (oracle.accel_subset, nbody.cu:19-65): median relative
   error <= 1e-6 per system; and 5 steps of the four smallest systems against
   independent oracle runs (nbody.cu:122-181).
-* The CIC deposit's merging paths on pre-sorted dense clusters: the
-  workgroup's LDS table (run tails with an interior base cell less than 256
-  cells above the workgroup's smallest key) and the direct atomics (keys
-  further away, and, in FP32 on grids above 64 cells, particles clamped onto
-  the upper face, whose base cell is the last one).  Which path each run tail
-  takes is recomputed on the host from the kernel's rule, so the test states
-  that both are reached; the grid must be bitwise oracle.cic_fixed, in sorted
-  and in shuffled order.
+* The CIC deposit's two paths on pre-sorted dense clusters: a workgroup whose
+  256 particles' CIC stencils fit a box of at most 2048 cells accumulates in
+  LDS and adds each box cell once; larger boxes issue each run's atomics
+  directly.  Which path each workgroup takes is recomputed on the host from
+  the kernel's rule, so the test states that both are reached, and that in
+  FP32 on grids above 64 cells particles clamped onto the upper face (whose
+  upper corner is their base cell) reach the box path; the grid must be
+  bitwise oracle.cic_fixed, in sorted and in shuffled order.
 """
 import numpy as np
 import pytest
@@ -104,9 +104,9 @@ def test_ensemble_64_galaxies(rrg, orc):
         assert np.max(np.abs(st["vel"][a:b] - ovel)) < 1e-6, g
 
 
-def _cic_keys(pos, nx, dx, dtype):
+def _cic_cells(pos, nx, dx, dtype):
     """Base cell of the CIC stencil as the deposit computes it (clamped
-    offsets; FP32 arithmetic for FP32 state), and the interior flag."""
+    offsets; FP32 arithmetic for FP32 state), its key and the upper corner."""
     dt = dtype
     dmin = dt(-0.5 * (dt(dx) * nx))
     inv_dx = dt(np.float32(1.0) / np.float32(dt(dx)))
@@ -114,26 +114,23 @@ def _cic_keys(pos, nx, dx, dtype):
     u = np.clip((pos.astype(dt) - dmin) * inv_dx - dt(0.5), dt(0), upper)
     b = np.clip(np.floor(u).astype(np.int64), 0, nx - 1)
     key = b[:, 0] + nx * (b[:, 1] + nx * b[:, 2])
-    interior = np.all(b < nx - 1, axis=1)
-    return key, interior
+    top = np.minimum(b + 1, nx - 1)
+    return key, b, top
 
 
-def _branch_counts(key, interior, window=256):
-    """(table, direct) run tails, following the kernel: workgroups of 256
-    particles, waves of 64 sorted by key, one tail per run of equal keys."""
-    table = direct = 0
-    for w0 in range(0, len(key), 256):
-        wk = key[w0:w0 + 256]
-        kmin = wk.min()
-        for l0 in range(0, len(wk), 64):
-            k = np.sort(wk[l0:l0 + 64], kind="stable")
-            order = np.argsort(wk[l0:l0 + 64], kind="stable")
-            inter = interior[w0 + l0:w0 + l0 + 64][order]
-            tails = np.nonzero(np.append(k[1:] != k[:-1], True))[0]
-            t_ok = inter[tails] & ((k[tails] - kmin) < window)
-            table += int(np.sum(t_ok))
-            direct += int(np.sum(~t_ok))
-    return table, direct
+def _path_counts(b, top, box=2048):
+    """Workgroups of 256 particles on the LDS-box path (bounding box of their
+    stencils within `box` cells) and on the direct path, and the clamped
+    particles (upper corner = base cell) on the box path."""
+    nbox = ndirect = clamped_in_box = 0
+    for w0 in range(0, len(b), 256):
+        vol = int(np.prod(top[w0:w0 + 256].max(axis=0) - b[w0:w0 + 256].min(axis=0) + 1))
+        if vol <= box:
+            nbox += 1
+            clamped_in_box += int(np.sum(np.any(top[w0:w0 + 256] == b[w0:w0 + 256], axis=1)))
+        else:
+            ndirect += 1
+    return nbox, ndirect, clamped_in_box
 
 
 def _clusters(nx, dx, dtype, seed=4):
@@ -155,15 +152,15 @@ def _clusters(nx, dx, dtype, seed=4):
 def test_cic_merge_paths_bitwise(rrg, orc, dtype):
     nx, dx, e = 128, 0.125, 60
     pos, m = _clusters(nx, dx, dtype)
-    key, interior = _cic_keys(pos, nx, dx, dtype)
+    key, b, top = _cic_cells(pos, nx, dx, dtype)
     order = np.argsort(key, kind="stable")
-    pos, m, key, interior = pos[order], m[order], key[order], interior[order]
-    table, direct = _branch_counts(key, interior)
-    print(f"cic {np.dtype(dtype).name}: run tails via LDS table {table}, direct {direct}, "
-          f"non-interior particles {int(np.sum(~interior))}", flush=True)
-    assert table > 500 and direct > 1000
+    pos, m, b, top = pos[order], m[order], b[order], top[order]
+    nbox, ndirect, clamped = _path_counts(b, top)
+    print(f"cic {np.dtype(dtype).name}: workgroups on the LDS box path {nbox}, direct {ndirect}, clamped particles "
+          f"in boxes {clamped}", flush=True)
+    assert nbox > 100 and ndirect > 20
     if dtype == np.float32:
-        assert np.sum(~interior) > 0  # clamped onto the last cell: the upper face path
+        assert clamped > 0  # clamped onto the last cell: upper corner = base cell
     ref = orc.cic_fixed(pos, m, nx, dx, e, dtype=dtype)
     assert np.array_equal(rrg.cic_deposit(pos, m, nx, dx, e, dtype=dtype), ref)
     perm = np.random.default_rng(1).permutation(len(m))
