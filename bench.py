@@ -62,6 +62,7 @@ def wave_schedule(rrg, iters, nx, real="fp32", fused=True):
 
 PM = dict(n=1_000_000, nx=128, dx=0.125, dt=5e-3, iterations=200, setup_iterations=2000, diss_base=0.785,
           diss_extra=0.01)
+RESORT_EVERY = 48  # the library's default steps between PM particle re-sorts
 
 
 def _device_sync():
@@ -94,15 +95,17 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     sim.synchronize()
     _device_sync()
     t1 = time.perf_counter()
-    # stage breakdown and the stencil's launch time: the same number of steps
-    # again with HIP events between the stages (one graph per stage, captured
-    # by the first of these steps)
+    # stage breakdown and the stencil's launch time: at least one whole
+    # re-sort cycle of steps again with HIP events between the stages (one
+    # graph per stage, captured by the first of these steps), so the
+    # per-step stage times carry the re-sort's share
     sim.set_timing(True)
     sim.calc_step(1)
     sim.synchronize()
     sim.set_timing(True)
+    estep = max(steps, RESORT_EVERY)
     te0 = time.perf_counter()
-    sim.calc_step(steps)
+    sim.calc_step(estep)
     sim.synchronize()
     te1 = time.perf_counter()
     tm = sim.timing()
@@ -113,7 +116,7 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     avg_wave_s = wave_ms / 1e3 / max(wave_launches, 1)
     iters = PM["iterations"]
     sched = wave_schedule(rrg, iters, PM["nx"], real=real)
-    assert sum(sched.values()) == round(wave_launches / steps), (sched, wave_launches, steps)
+    assert sum(sched.values()) == round(wave_launches / estep), (sched, wave_launches, estep)
     fused = sched[4] + sched[3] + sched[2]
     n = PM["n"]
     # Effective bytes, fp32, per unit of work of the PM step (what the step
@@ -147,11 +150,17 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     return out | {
         "value": n * steps / secs / 1e6,
         "ms_per_step": secs / steps * 1e3,
-        "ms_per_step_with_stage_events": (te1 - te0) / steps * 1e3,
-        "stage_ms_per_step": {k: v[0] / steps for k, v in tm.items() if v[1]},
+        "ms_per_step_with_stage_events": (te1 - te0) / estep * 1e3,
+        "stage_ms_per_step": {k: v[0] / estep for k, v in tm.items() if v[1]},
+        "stage_event_steps": estep,
+        "resort_every_steps": RESORT_EVERY,
+        "resorts_in_stage_steps": int(tm.get("sort", (0, 0))[1] // 3),
+        # the timed steps may fall between re-sorts; this adds the re-sort's
+        # per-step share measured over the stage-event steps
+        "ms_per_step_resort_amortized": secs / steps * 1e3 + tm.get("sort", (0.0, 0))[0] / estep,
         "wave_avg_launch_us": avg_wave_s * 1e6,
-        "wave_ms_per_step": wave_ms / steps,
-        "wave_us_per_iteration": wave_ms * 1e3 / (iters * steps),
+        "wave_ms_per_step": wave_ms / estep,
+        "wave_us_per_iteration": wave_ms * 1e3 / (iters * estep),
         "wave_launches_per_step": {"four_iterations": sched[4], "three_iterations": sched[3],
                                    "two_iterations": sched[2], "single": sched[1]},
         "wave_iterations_per_launch": max(k for k, v in sched.items() if v),
@@ -804,6 +813,10 @@ def main():
                           "effective_gbs": pm["step_effective_bytes"] / step_s / 1e9},
         "ms_per_step_with_stage_events": pm["ms_per_step_with_stage_events"],
         "stage_ms_per_step": pm["stage_ms_per_step"],
+        "stage_event_steps": pm["stage_event_steps"],
+        "resort_every_steps": pm["resort_every_steps"],
+        "resorts_in_stage_steps": pm["resorts_in_stage_steps"],
+        "ms_per_step_resort_amortized": pm["ms_per_step_resort_amortized"],
         "wave_avg_launch_us": pm["wave_avg_launch_us"],
         "wave_us_per_iteration": pm["wave_us_per_iteration"],
         "wave_launches_per_step": pm["wave_launches_per_step"],

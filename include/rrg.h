@@ -135,6 +135,8 @@ typedef struct {
                                at 128^3 and FP32 256^3, three elsewhere, two
                                for FP64 512^3), bitwise the same results;
                                0: one */
+    int32_t sort_interval;  /* PM: steps between re-sorts of the particles by
+                               cell (results do not depend on it); 0 = 48 */
 } rrg_params;
 
 typedef struct {

@@ -65,7 +65,7 @@ class Params(ctypes.Structure):
     _fields_ = [("sim", SimParams), ("has_grid", _i32), ("grid", GridParams), ("has_wave", _i32),
                 ("wave", WaveParams), ("use_wave_model", _i32), ("real_type", _i32), ("force_kind", _i32),
                 ("boundary", _i32), ("fixed_exponent", _i32), ("device", _i32), ("use_graphs", _i32),
-                ("autotune", _i32), ("wave_fused", _i32)]
+                ("autotune", _i32), ("wave_fused", _i32), ("sort_interval", _i32)]
 
 
 class Conservation(ctypes.Structure):

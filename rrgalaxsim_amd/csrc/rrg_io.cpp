@@ -210,6 +210,7 @@ void params_default(rrg_params* p) {
     p->use_graphs = 1;
     p->autotune = 0;
     p->wave_fused = 1;
+    p->sort_interval = 0;
 }
 
 // validate_dependent_fields (io_json.cpp:170-186).  The CFL step is formed
@@ -315,6 +316,11 @@ void params_parse_json(const std::string& path, rrg_params* p, rrg_ini_params* i
         if (const JValue* v = m->find("wave_fused")) {
             if (v->kind != JValue::Bool) fail(RRG_ERR_IO, "mi355x.wave_fused must be true or false");
             p->wave_fused = v->b ? 1 : 0;
+        }
+        if (const JValue* v = m->find("sort_interval")) {
+            if (v->kind != JValue::Number || v->num < 0 || v->num > 1e9 || v->num != (double)(int32_t)v->num)
+                fail(RRG_ERR_IO, "mi355x.sort_interval must be a non-negative integer");
+            p->sort_interval = (int32_t)v->num;
         }
     }
     params_finalize(p);

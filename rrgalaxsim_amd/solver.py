@@ -34,7 +34,7 @@ def wave_schedule(nx, iterations, real="fp32", boundary="periodic", fused=True) 
 
 def make_params(*, dt, time_max=None, dt_save=None, grid=None, wave=None, real="fp64", force="fp64",
                 boundary="periodic", fixed_exponent=0, device=0, graphs=True, autotune=False,
-                wave_fused=True) -> L.Params:
+                wave_fused=True, sort_interval=0) -> L.Params:
     """Build an ``rrg_params``.  ``grid`` = dict(nx, dx, bc_frac); ``wave`` =
     dict(diss_base, diss_extra, wave_speed[, setup_iterations, iterations,
     tolerance, steady_check_step, steady_hold_steps])."""
@@ -59,6 +59,7 @@ def make_params(*, dt, time_max=None, dt_save=None, grid=None, wave=None, real="
     p.use_graphs = 1 if graphs else 0
     p.autotune = 1 if autotune else 0
     p.wave_fused = 1 if wave_fused else 0
+    p.sort_interval = int(sort_interval)
     L.check(L.lib().rrg_params_finalize(ctypes.byref(p)))
     return p
 

@@ -70,16 +70,17 @@ def test_parse_fp32_cfl_step(rrg, tmp_path):
 def test_parse_launch_options(rrg, tmp_path):
     d = json.load(open(os.path.join(GOLDEN, "sample_params.json")))
     p0, _ = rrg.parse_params_json(os.path.join(GOLDEN, "sample_params.json"))
-    assert p0.use_graphs == 1 and p0.autotune == 0 and p0.wave_fused == 1
-    d["mi355x"] = {"graphs": False, "autotune": True, "wave_fused": False}
+    assert p0.use_graphs == 1 and p0.autotune == 0 and p0.wave_fused == 1 and p0.sort_interval == 0
+    d["mi355x"] = {"graphs": False, "autotune": True, "wave_fused": False, "sort_interval": 12}
     f = tmp_path / "params.json"
     f.write_text(json.dumps(d))
     p, _ = rrg.parse_params_json(str(f))
-    assert p.use_graphs == 0 and p.autotune == 1 and p.wave_fused == 0
-    d["mi355x"] = {"wave_fused": 1}
-    f.write_text(json.dumps(d))
-    with pytest.raises(rrg.RRGError):
-        rrg.parse_params_json(str(f))
+    assert p.use_graphs == 0 and p.autotune == 1 and p.wave_fused == 0 and p.sort_interval == 12
+    for bad in ({"wave_fused": 1}, {"sort_interval": -1}, {"sort_interval": 2.5}, {"sort_interval": "8"}):
+        d["mi355x"] = bad
+        f.write_text(json.dumps(d))
+        with pytest.raises(rrg.RRGError):
+            rrg.parse_params_json(str(f))
 
 
 def test_nbody_model_when_wave_section_absent(rrg, tmp_path):

@@ -31,7 +31,7 @@ def _nbody(rrg, n=3000, force="fp64"):
 def _pm(rrg, n=5000, real="fp32"):
     pos, vel, m = ics.exp_disk(n, seed=6)
     wave = dict(diss_base=0.785, diss_extra=0.0, wave_speed=C, setup_iterations=20, iterations=5, tolerance=0.0)
-    p = rrg.make_params(dt=5e-3, grid=dict(nx=32, dx=0.5), wave=wave, real=real)
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=32, dx=0.5), wave=wave, real=real, sort_interval=8)
     return rrg.Simulation(p, pos, vel, m), pos, vel
 
 

@@ -144,7 +144,8 @@ def test_pm_graph_replay_bitwise_equals_eager(rrg, nx, timing):
                 iterations=7, tolerance=0.0)
     res = {}
     for graphs in (False, True):
-        p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real="fp32", graphs=graphs)
+        p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=16.0 / nx), wave=wave, real="fp32", graphs=graphs,
+                            sort_interval=8)
         with rrg.Simulation(p, pos, vel, m) as sim:
             sim.precalc()
             sim.set_timing(timing)

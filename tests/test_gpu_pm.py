@@ -93,12 +93,13 @@ def test_steady_err(rrg, orc, dtype):
 
 
 def _pm(rrg, orc, dtype, *, n=20_000, nx=32, dx=0.5, steps=5, setup=40, iters=20, tol=0.0, check=30, hold=3,
-        boundary="periodic"):
+        boundary="periodic", sort_interval=0):
     pos, vel, m = ics.exp_disk(n, seed=1)
     wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=setup, iterations=iters,
                 tolerance=tol, steady_check_step=check, steady_hold_steps=hold)
     real = "fp32" if dtype == np.float32 else "fp64"
-    p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=dx, bc_frac=0.0), wave=wave, real=real, boundary=boundary)
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=dx, bc_frac=0.0), wave=wave, real=real, boundary=boundary,
+                        sort_interval=sort_interval)
     with rrg.Simulation(p, pos, vel, m) as sim:
         sim.precalc()
         sim.sync_particles_and_grid()
@@ -143,7 +144,7 @@ def test_pm_zero_dirichlet_bitwise(rrg, orc):
 def test_pm_resort_and_state_roundtrip(rrg, orc):
     """More steps than the re-sort interval: the library keeps particles in
     cell order internally; results come back in the caller's order."""
-    _, st, gr, stats, o = _pm(rrg, orc, np.float32, steps=19, iters=10)
+    _, st, gr, stats, o = _pm(rrg, orc, np.float32, steps=19, iters=10, sort_interval=8)
     for k in ("pos", "vel", "acc", "grav"):
         ref = o[k] if k != "grav" else o["grav"]
         assert np.array_equal(st[k].astype(np.float32), ref), k
