@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void k_fixed_to_source(int64_t ncell, unsigned
     const long long q = (long long)mfix[c];
     mfix[c] = 0ull;
     const real m = (real)((double)q * inv_scale);
-    mass[c] = m;
+    if (mass != nullptr) mass[c] = m;
     real kf = k_f;
     if (LAYER) {
         const int NX = w.nx;

@@ -1199,8 +1199,15 @@ public:
         conservation(out);
     }
 
+    // The steps keep no cell-mass grid (only its source term): the cell mass
+    // is deposited again from the current positions, which are those of the
+    // last deposit (the kick does not move particles; set_state would).
     void get_grid(double* mass, double* phi, double* acc) override {
-        if (mass) download_scalar<real>(s_, ncell_, mass_.p, mass);
+        if (mass) {
+            pm::predict_deposit<real>(s_, P_.view(), (real)0, false, geo_, scale_, mfix_.p, qmax_);
+            pm::fixed_to_mass<real>(s_, nx_, mfix_.p, inv_scale_, mass_.p);  // leaves mfix_ zero
+            download_scalar<real>(s_, ncell_, mass_.p, mass);
+        }
         if (phi) download_scalar<real>(s_, ncell_, cur(), phi);
         if (acc) {
             std::vector<real> tmp(4 * ncell_);
@@ -1258,7 +1265,7 @@ private:
             graphs_.run(s_, b0 + 5, [&] {
                 pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p, qmax_);
                 pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f,
-                                          mass_.p, src_.p, layered());
+                                          nullptr, src_.p, layered());
                 int ip = ip0, ic = ic0;  // from the arrangement before the rotation
                 solve_launches(ip, ic, true);
                 pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
@@ -1272,7 +1279,7 @@ private:
             pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p, qmax_);
         });
         segment(RRG_STAGE_SOURCE, 1, graph, b0 + 1, [&] {
-            pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, mass_.p,
+            pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, nullptr,
                                       src_.p, layered());
         });
         iters_on_device_ = false;
@@ -1510,7 +1517,7 @@ private:
         pm::predict_deposit<real>(s_, P_.view(), (real)prm_.sim.dt_dynamics, predict, geo_, scale_, mfix_.p, qmax_);
         timer.stop(s_, t, 1);
         t = timer.start(s_, RRG_STAGE_SOURCE);
-        pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, mass_.p, src_.p,
+        pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, nullptr, src_.p,
                                   layered());
         timer.stop(s_, t, 1);
     }

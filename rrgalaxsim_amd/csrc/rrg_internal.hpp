@@ -233,9 +233,10 @@ void permute(hipStream_t s, const PState<real>& src, const PState<real>& dst, co
 void radix_sort_pairs(hipStream_t s, void* temp, size_t& temp_bytes, const unsigned* keys_in, unsigned* keys_out,
                       const unsigned* vals_in, unsigned* vals_out, int64_t n, int end_bit);
 
-// Fixed-point grid -> cell mass (real) and stencil source term
-// K_F * 4 pi G (m/dx^3 - rho_mean); clears the fixed-point grid for the next deposit.
-// On a non-periodic grid (`model` non-null) K_F is the cell's own.
+// Fixed-point grid -> stencil source term K_F * 4 pi G (m/dx^3 - rho_mean)
+// and, when `mass` is non-null, the cell mass (real); clears the fixed-point
+// grid for the next deposit.  On a non-periodic grid (`model` non-null) K_F
+// is the cell's own.
 template <class real>
 void fixed_to_source(hipStream_t s, int nx, unsigned long long* mfix, double inv_scale, real dx,
                      real rho_mean, bool periodic, real k_f, real* mass, real* src,
