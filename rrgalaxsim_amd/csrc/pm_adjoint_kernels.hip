@@ -9,7 +9,10 @@
 //   v' = v + dt/2 (a + a')                         correct_step_ (nbody.cu:171-180)
 // B zeroes the faces on zero-Dirichlet grids.  The transposed operators:
 //   * Interp^T is a deposit of the particle adjoint with the interpolation
-//     weights (FP64 atomics), plus d(weights)/dx' dotted with G;
+//     weights, plus d(weights)/dx' dotted with G; the deposit is int64 fixed
+//     point (resolution 2^-E, n max|abar + h vbar| 2^E < 2^61, E found on the
+//     device), so the grid adjoint does not depend on the order of the
+//     atomics and the gradients repeat bitwise from run to run;
 //   * D^T is evaluated per cell as a gather from the three cells that read it;
 //   * the stencil recurrence reverses into the same stencil with no source,
 //     mu_{k-1} = B(K_m L mu_k + K_a mu_k + K_o mu_{k+1}), while the source
@@ -18,9 +21,7 @@
 //     derivatives times m (zero on a clamped axis).
 // Particle adjoints are FP64 whatever the state precision; grid adjoints are
 // kept in the state's precision except the interpolation adjoint, which is
-// accumulated by FP64 atomics.
-#include <hip/amd_detail/amd_hip_unsafe_atomics.h>
-
+// accumulated in int64 fixed point.
 #include "rrg_internal.hpp"
 #include "wave_fused.hpp"
 
@@ -57,16 +58,52 @@ __device__ __forceinline__ void xcd_block(int nbx, int nby, int nbz, int& bx, in
 // The deposit of at into Gb merges lanes first: the wave's particles are
 // sorted by their 8-cell stencil (a 64-bit key of the clamped corner
 // indices) with a shuffle bitonic sort, a segmented inclusive scan sums each
-// run of equal stencils, and the run's last lane issues the 8 x 3 FP64
-// atomics.  The particle arrays are kept in cell order during a recorded run,
-// so dense regions form long runs.
+// run of equal stencils, and the run's last lane rounds its 8 x 3 sums to
+// multiples of 2^-E and issues them as int64 atomics.  The particle arrays
+// are kept in cell order during a recorded run, so dense regions form long
+// runs; the merge order within a wave follows the particle order, so it is
+// the same in every run.
 constexpr unsigned long long kNoStencil = ~0ull;
+
+__device__ __forceinline__ double wave_max_d(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// max over particles and components of |abar + h vbar|, as a bit pattern
+// (non-negative doubles order as their bits): one atomic per workgroup.
+__global__ __launch_bounds__(256) void k_adj_at_max(int64_t n, double h, const double* __restrict__ Vb,
+                                                    const double* __restrict__ Ab, AdjScale* __restrict__ sc) {
+    __shared__ double wm[4];
+    double mx = 0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        for (int d = 0; d < 3; ++d) mx = fmax(mx, fabs(Ab[d * n + i] + h * Vb[d * n + i]));
+    mx = wave_max_d(mx);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mx = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+        atomicMax(&sc->maxbits, (unsigned long long)__double_as_longlong(mx));
+    }
+}
+
+// E with n * max * 2^E < 2^61 (the bound of any cell's, and any partial,
+// sum); the max is cleared for the next grid build.
+__global__ void k_adj_scale(int64_t n, AdjScale* __restrict__ sc) {
+    const double mx = __longlong_as_double((long long)sc->maxbits);
+    sc->maxbits = 0;
+    const double b = mx * (double)n;
+    int ex = 0;
+    (void)frexp(b > 1e-300 ? b : 1e-300, &ex);  // b in [2^(ex-1), 2^ex)
+    sc->e = (b < 1e300) ? min(61 - ex, 1000) : 0;
+}
 
 template <class real>
 __global__ __launch_bounds__(256) void k_adj_interp(int64_t n, const real* __restrict__ x, Geo<real> g,
                                                     const real* __restrict__ G4, double h, double* __restrict__ Xb,
                                                     const double* __restrict__ Vb, const double* __restrict__ Ab,
-                                                    double* __restrict__ Gb) {
+                                                    unsigned long long* __restrict__ Gq,
+                                                    const AdjScale* __restrict__ sc) {
     const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int nx = g.nx;
@@ -189,9 +226,12 @@ __global__ __launch_bounds__(256) void k_adj_interp(int64_t n, const real* __res
             }
         }
         if (tail) {
+            const double scale = ldexp(1.0, sc->e);
 #pragma unroll
-            for (int c = 0; c < 8; ++c)
-                if (v[c] != 0.0) unsafeAtomicAdd(Gb + d * ncell + cell[c], v[c]);
+            for (int c = 0; c < 8; ++c) {
+                const long long qv = (long long)rint(v[c] * scale);
+                if (qv != 0) atomicAdd(Gq + d * ncell + cell[c], (unsigned long long)qv);
+            }
         }
     }
 }
@@ -199,8 +239,8 @@ __global__ __launch_bounds__(256) void k_adj_interp(int64_t n, const real* __res
 // mu += B(coef * D^T Gb): each cell gathers the terms of the (up to three)
 // cells per axis whose difference read it.  Gb is SoA, three grids.
 template <class real, bool PERIODIC>
-__global__ __launch_bounds__(256) void k_adj_gradient_t(int NX, double coef, const double* __restrict__ Gb,
-                                                        real* __restrict__ mu) {
+__global__ __launch_bounds__(256) void k_adj_gradient_t(int NX, double coef, const long long* __restrict__ Gq,
+                                                        const AdjScale* __restrict__ sc, real* __restrict__ mu) {
     int bx, by, bz;
     xcd_block(gridDim.x, gridDim.y, gridDim.z, bx, by, bz);
     const int ix = bx * 64 + threadIdx.x;
@@ -212,15 +252,17 @@ __global__ __launch_bounds__(256) void k_adj_gradient_t(int NX, double coef, con
     const size_t st[3] = {1, size_t(NX), size_t(NX) * NX};
     const int id[3] = {ix, iy, iz};
     const size_t c = size_t(ix) + iy * st[1] + iz * st[2];
+    const double unit = ldexp(1.0, -sc->e);
     double sum = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double* gd = Gb + d * ncell;
+        const long long* gq = Gq + d * ncell;
+        auto gd = [&](size_t k) { return (double)gq[k] * unit; };
         const int i = id[d];
-        if (i >= 1) sum += (i - 1 == 0 ? 1.0 : 0.5) * gd[c - st[d]];  // this cell is the +1 neighbour
-        if (i == 0) sum -= gd[c];                                      // one-sided faces read the cell itself
-        else if (i == NX - 1) sum += gd[c];
-        if (i + 1 <= NX - 1) sum -= (i + 1 == NX - 1 ? 1.0 : 0.5) * gd[c + st[d]];  // the -1 neighbour
+        if (i >= 1) sum += (i - 1 == 0 ? 1.0 : 0.5) * gd(c - st[d]);  // this cell is the +1 neighbour
+        if (i == 0) sum -= gd(c);                                      // one-sided faces read the cell itself
+        else if (i == NX - 1) sum += gd(c);
+        if (i + 1 <= NX - 1) sum -= (i + 1 == NX - 1 ? 1.0 : 0.5) * gd(c + st[d]);  // the -1 neighbour
     }
     mu[c] += (real)(coef * sum);
 }
@@ -371,17 +413,24 @@ inline unsigned grid1d(int64_t n, int tpb) { return unsigned((n + tpb - 1) / tpb
 
 template <class real>
 void adj_interp(hipStream_t s, int64_t n, const real* x, const Geo<real>& g, const real* G4, double h, double* Xb,
-                const double* Vb, const double* Ab, double* Gb) {
+                const double* Vb, const double* Ab, long long* Gq, AdjScale* sc) {
     if (n == 0) return;
-    k_adj_interp<real><<<grid1d(n, 256), 256, 0, s>>>(n, x, g, G4, h, Xb, Vb, Ab, Gb);
+    const unsigned nb = unsigned(std::min<int64_t>(512, (n + 255) / 256));
+    k_adj_at_max<<<nb, 256, 0, s>>>(n, h, Vb, Ab, sc);
+    RRG_LAUNCH_CHECK();
+    k_adj_scale<<<1, 1, 0, s>>>(n, sc);
+    RRG_LAUNCH_CHECK();
+    k_adj_interp<real><<<grid1d(n, 256), 256, 0, s>>>(n, x, g, G4, h, Xb, Vb, Ab,
+                                                      reinterpret_cast<unsigned long long*>(Gq), sc);
     RRG_LAUNCH_CHECK();
 }
 
-template <class real> void adj_gradient_t(hipStream_t s, int nx, real dx, bool periodic, const double* Gb, real* mu) {
+template <class real>
+void adj_gradient_t(hipStream_t s, int nx, real dx, bool periodic, const long long* Gq, const AdjScale* sc, real* mu) {
     const double coef = (double)(real)(-1. / (double)dx);  // the forward's coefficient
     const dim3 grid(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned(nx));
-    if (periodic) k_adj_gradient_t<real, true><<<grid, dim3(64, 4, 1), 0, s>>>(nx, coef, Gb, mu);
-    else k_adj_gradient_t<real, false><<<grid, dim3(64, 4, 1), 0, s>>>(nx, coef, Gb, mu);
+    if (periodic) k_adj_gradient_t<real, true><<<grid, dim3(64, 4, 1), 0, s>>>(nx, coef, Gq, sc, mu);
+    else k_adj_gradient_t<real, false><<<grid, dim3(64, 4, 1), 0, s>>>(nx, coef, Gq, sc, mu);
     RRG_LAUNCH_CHECK();
 }
 
@@ -437,8 +486,8 @@ void adj_cic(hipStream_t s, int64_t n, const real* x, const real* m, const Geo<r
 
 #define RRG_PMADJ_INST(real)                                                                                     \
     template void adj_interp<real>(hipStream_t, int64_t, const real*, const Geo<real>&, const real*, double,     \
-                                   double*, const double*, const double*, double*);                              \
-    template void adj_gradient_t<real>(hipStream_t, int, real, bool, const double*, real*);                      \
+                                   double*, const double*, const double*, long long*, AdjScale*);                \
+    template void adj_gradient_t<real>(hipStream_t, int, real, bool, const long long*, const AdjScale*, real*);  \
     template void adj_wave<real>(hipStream_t, int, bool, const WaveK<real>&, real*, const real*, const real*,    \
                                  real*, bool, const real*, const real*, const real*);                            \
     template int adj_wave_levels<real>(int, bool, int);                                                          \

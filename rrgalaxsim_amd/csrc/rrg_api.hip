@@ -1426,6 +1426,8 @@ private:
         if (Xb_.n != size_t(3 * n)) {
             Xb_.alloc(3 * n); Vb_.alloc(3 * n); Ab_.alloc(3 * n); gm_.alloc(n);
             Gb_.alloc(3 * ncell_);
+            ascale_.alloc(1);
+            zero_async(s_, ascale_.p, ascale_.bytes());
             for (int i = 0; i < nbuf_; ++i) mu_[i].alloc(ncell_);
             sbar_.alloc(ncell_);
             gtmp_.alloc(4 * ncell_);
@@ -1448,8 +1450,9 @@ private:
         const double dt = prm_.sim.dt_dynamics;
         int t = timer.start(s_, RRG_STAGE_ADJ_FORCE);
         pm::gradient<real>(s_, nx_, geo_.dx, phi, gtmp_.p);
-        pm::adj_interp<real>(s_, n, x, geo_, gtmp_.p, closing ? 0.0 : 0.5 * dt, Xb_.p, Vb_.p, Ab_.p, Gb_.p);
-        pm::adj_gradient_t<real>(s_, nx_, geo_.dx, periodic_, Gb_.p, mu_[aic_].p);
+        pm::adj_interp<real>(s_, n, x, geo_, gtmp_.p, closing ? 0.0 : 0.5 * dt, Xb_.p, Vb_.p, Ab_.p, Gb_.p,
+                             ascale_.p);
+        pm::adj_gradient_t<real>(s_, nx_, geo_.dx, periodic_, Gb_.p, ascale_.p, mu_[aic_].p);
         pm::fill_u64(s_, reinterpret_cast<unsigned long long*>(Gb_.p), 3 * size_t(ncell_), 0ull);
         timer.stop(s_, t, 4);
         if (iters == 0) {
@@ -1618,7 +1621,9 @@ private:
     std::vector<int> rec_iters_;        // stencil iterations per grid build (0 = precalc)
     std::vector<DBuf<real>> ck_;        // (x, v, a, phi_prev, phi_curr) every ck_every_ steps
     DBuf<real> backup_, xs_, phis_;     // live state; x' and phi_I of one segment
-    DBuf<double> Xb_, Vb_, Ab_, gm_, Gb_;
+    DBuf<double> Xb_, Vb_, Ab_, gm_;
+    DBuf<long long> Gb_;             // interpolation adjoint, int64 fixed point
+    DBuf<pm::AdjScale> ascale_;      // its scale
     DBuf<real> mu_[4], sbar_, gtmp_;
     DBuf<real> kc_[4];                  // off-periodic K_MAIN, K_ACTUAL, K_OLD, d source / d mass per cell
     int aip_ = 0, aic_ = 1;             // potential adjoints mu_{k+1}, mu_k

@@ -367,13 +367,21 @@ template <class real> void fixed_to_mass(hipStream_t s, int nx, unsigned long lo
 void fill_u64(hipStream_t s, unsigned long long* p, size_t count, unsigned long long value);
 
 // Reverse sweep of the PM step (pm_adjoint_kernels.hip).  Particle arrays SoA
-// (n per component), particle adjoints FP64; Gb = three FP64 grids (x, y, z).
-// Interpolation adjoint at x': Xb += d(Interp)/dx' . G, Gb += Interp^T (Ab + h Vb).
+// (n per component), particle adjoints FP64; Gq = three int64 fixed-point
+// grids (x, y, z) of resolution 2^-sc->e, zero on entry.
+// Fixed-point scale of the interpolation adjoint, found on the device; zero
+// before the first use (each use leaves `maxbits` zero).
+struct AdjScale {
+    unsigned long long maxbits;
+    int e, pad;
+};
+// Interpolation adjoint at x': Xb += d(Interp)/dx' . G, Gq += Interp^T (Ab + h Vb).
 template <class real>
 void adj_interp(hipStream_t s, int64_t n, const real* x, const Geo<real>& g, const real* G4, double h, double* Xb,
-                const double* Vb, const double* Ab, double* Gb);
-// mu += B(-1/dx D^T Gb) (B masks the faces of zero-Dirichlet grids).
-template <class real> void adj_gradient_t(hipStream_t s, int nx, real dx, bool periodic, const double* Gb, real* mu);
+                const double* Vb, const double* Ab, long long* Gq, AdjScale* sc);
+// mu += B(-1/dx D^T Gq) (B masks the faces of zero-Dirichlet grids).
+template <class real>
+void adj_gradient_t(hipStream_t s, int nx, real dx, bool periodic, const long long* Gq, const AdjScale* sc, real* mu);
 // One transposed stencil iteration pn = B(K_m L p + K_a p + K_o po) with
 // sbar = p (first) or sbar += p.  With per-cell coefficient grids (km
 // non-null, non-periodic grids) pn = B(L(km p) + ka p + ko po).
