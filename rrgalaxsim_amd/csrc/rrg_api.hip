@@ -12,6 +12,9 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -350,13 +353,53 @@ private:
     int64_t launches_ = 0;
 };
 
-// Process-wide cache of autotuned direct-sum launch shapes.
+// Cache of autotuned direct-sum launch shapes, keyed by GPU architecture,
+// force kind, softening form and size buckets; kept for the process and in a
+// text file (one "key ib len" line per shape) so later processes reuse it:
+// $RRG_TUNE_CACHE if set (empty: no file), else
+// ${XDG_CACHE_HOME:-$HOME/.cache}/rrgalaxsim_amd/autotune.txt.
 struct TunedShape {
     int ib = 0;
     int64_t len = 0;  // grid form: target workgroups; queue form: sources per item
 };
 std::mutex g_tune_mutex;
 std::map<std::string, TunedShape> g_tune_cache;
+bool g_tune_file_read = false;
+
+std::string tune_cache_path() {
+    if (const char* e = std::getenv("RRG_TUNE_CACHE")) return e;
+    std::string base;
+    if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x) base = x;
+    else if (const char* h = std::getenv("HOME"); h && *h) base = std::string(h) + "/.cache";
+    else return "";
+    return base + "/rrgalaxsim_amd/autotune.txt";
+}
+
+// With g_tune_mutex held.
+void tune_file_load() {
+    if (g_tune_file_read) return;
+    g_tune_file_read = true;
+    const std::string path = tune_cache_path();
+    if (path.empty()) return;
+    std::ifstream f(path);
+    std::string key;
+    TunedShape t;
+    while (f >> key >> t.ib >> t.len)
+        if (t.ib > 0 && t.len > 0) g_tune_cache[key] = t;
+}
+
+// With g_tune_mutex held; a cache that cannot be written is skipped.
+void tune_file_append(const std::string& key, const TunedShape& t) {
+    const std::string path = tune_cache_path();
+    if (path.empty()) return;
+    std::error_code ec;
+    std::filesystem::create_directories(std::filesystem::path(path).parent_path(), ec);
+    std::ofstream f(path, std::ios::app);
+    if (f) f << key << ' ' << t.ib << ' ' << t.len << '\n';
+}
+
+// floor(2 log2 v): half-octave size bucket.
+int half_octave(double v) { return v > 0 ? int(std::floor(2.0 * std::log2(v))) : -1; }
 
 // Direct-sum strategy (NBodyIntegrationSolver), FP64 particle state, for one
 // system (grid-form launches) or an ensemble of independent systems
@@ -647,27 +690,45 @@ private:
         RRG_HIP(hipStreamSynchronize(s_));
     }
 
-    // Autotuner: time every candidate shape on the live data (one warm and
-    // one timed force launch each) and keep the fastest; results are cached
-    // per (kind, softening, system sizes) for the process.
+    // Autotuner: time every candidate shape on the live data (one warm-up and
+    // three timed force launches each, median) and keep the fastest; results
+    // are cached per (architecture, kind, softening, size buckets): for one
+    // system its N, for an ensemble the number of systems, sum N^2 and the
+    // largest N, each in half-octaves.
+    std::string tune_key() const {
+        hipDeviceProp_t prop{};
+        RRG_HIP(hipGetDeviceProperties(&prop, prm_.device));
+        std::string arch = prop.gcnArchName;
+        arch = arch.substr(0, arch.find(':'));
+        std::string key = arch + "|k" + std::to_string(kind_) + (force_.uniform ? "u" : "v");
+        if (!ensemble_) return key + "|n" + std::to_string(half_octave(double(P_.n)));
+        double s2 = 0, nmax = 0;
+        for (int g = 0; g < k_; ++g) {
+            const double ng = double(offsets_[g + 1] - offsets_[g]);
+            s2 += ng * ng;
+            nmax = std::max(nmax, ng);
+        }
+        return key + "|e" + std::to_string(k_) + "s" + std::to_string(half_octave(s2)) + "m" +
+               std::to_string(half_octave(nmax));
+    }
+
     void tune() {
-        std::string key = std::to_string(kind_) + (force_.uniform ? "u" : "v");
-        for (int64_t o : offsets_) key += ":" + std::to_string(o);
+        const std::string key = tune_key();
         {
             std::lock_guard<std::mutex> lk(g_tune_mutex);
+            tune_file_load();
             auto it = g_tune_cache.find(key);
-            if (it != g_tune_cache.end()) {
+            if (it != g_tune_cache.end() && nbody::valid_ib(kind_, it->second.ib)) {
                 configure(it->second.ib, it->second.len);
                 tuned_ = true;
+                tune_candidates_ = 0;
                 return;
             }
         }
         const std::vector<int> ibs = nbody::is_fp32_kind(kind_) ? std::vector<int>{1, 2, 4} : std::vector<int>{1, 2};
-        const std::vector<int64_t> lens = ensemble_ ? std::vector<int64_t>{1024, 2048, 4096, 8192}
-                                                    : std::vector<int64_t>{1024, 2048, 4096, 8192};
-        hipEvent_t e0, e1;
-        RRG_HIP(hipEventCreate(&e0));
-        RRG_HIP(hipEventCreate(&e1));
+        const std::vector<int64_t> lens = {1024, 2048, 4096, 8192};
+        hipEvent_t ev[4];
+        for (auto& e : ev) RRG_HIP(hipEventCreate(&e));
         TunedShape best;
         float best_ms = 1e30f;
         tune_candidates_ = 0;
@@ -676,30 +737,32 @@ private:
                 for (int64_t len : lens) {
                     configure(ib, len);
                     accel(false);
-                    RRG_HIP(hipEventRecord(e0, s_));
-                    accel(false);
-                    RRG_HIP(hipEventRecord(e1, s_));
-                    RRG_HIP(hipEventSynchronize(e1));
-                    float ms = 0.f;
-                    RRG_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    RRG_HIP(hipEventRecord(ev[0], s_));
+                    for (int r = 0; r < 3; ++r) {
+                        accel(false);
+                        RRG_HIP(hipEventRecord(ev[r + 1], s_));
+                    }
+                    RRG_HIP(hipEventSynchronize(ev[3]));
+                    float t[3];
+                    for (int r = 0; r < 3; ++r) RRG_HIP(hipEventElapsedTime(&t[r], ev[r], ev[r + 1]));
+                    std::sort(t, t + 3);
                     ++tune_candidates_;
-                    if (ms < best_ms) {
-                        best_ms = ms;
+                    if (t[1] < best_ms) {
+                        best_ms = t[1];
                         best.ib = ib;
                         best.len = len;
                     }
                 }
         } catch (...) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+            for (auto& e : ev) (void)hipEventDestroy(e);
             throw;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        for (auto& e : ev) (void)hipEventDestroy(e);
         configure(best.ib, best.len);
         tuned_ = true;
         std::lock_guard<std::mutex> lk(g_tune_mutex);
         g_tune_cache[key] = best;
+        tune_file_append(key, best);
     }
 
     void grid_init(int64_t n, const double* mass) {

@@ -6,6 +6,9 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
+# the autotuner's cache file stays out of the tests (test_gpu_autotune.py
+# gives its child processes their own)
+os.environ["RRG_TUNE_CACHE"] = ""
 
 
 def pytest_configure(config):
