@@ -111,6 +111,16 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     tm = sim.timing()
     stats = sim.stats()
     sim.close()
+    # the stencil launch time with every kernel launched on its own (no
+    # graph), the form a kernel tracer times (bench.py --profile-pass --eager)
+    pe = rrg.make_params(dt=PM["dt"], grid=dict(nx=PM["nx"], dx=PM["dx"], bc_frac=0.0),
+                         wave=dict(wave, setup_iterations=200), real=real, graphs=False)
+    with rrg.Simulation(pe, pos, vel, m) as se:
+        se.precalc()
+        se.calc_step(1)
+        se.set_timing(True)
+        se.calc_step(3)
+        wave_eager_ms, wave_eager_launches = se.timing()["wave"]
     ncell = PM["nx"] ** 3
     wave_ms, wave_launches = tm["wave"]
     avg_wave_s = wave_ms / 1e3 / max(wave_launches, 1)
@@ -159,6 +169,7 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
         # per-step share measured over the stage-event steps
         "ms_per_step_resort_amortized": secs / steps * 1e3 + tm.get("sort", (0.0, 0))[0] / estep,
         "wave_avg_launch_us": avg_wave_s * 1e6,
+        "wave_avg_launch_us_eager": wave_eager_ms * 1e3 / max(wave_eager_launches, 1),
         "wave_ms_per_step": wave_ms / estep,
         "wave_us_per_iteration": wave_ms * 1e3 / (iters * estep),
         "wave_launches_per_step": {"four_iterations": sched[4], "three_iterations": sched[3],
@@ -798,6 +809,10 @@ def main():
                      "frac": compulsory / launch_s / 1e9 / HBM_PEAK_GBS,
                      "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes": compulsory, "launch_us": launch_s * 1e6,
+                     "launch_us_eager": pm["wave_avg_launch_us_eager"],
+                     "launch_note": "launch_us: HIP events on the library's stream around the stencil launches of "
+                                    "graph-replayed steps (the timed form); launch_us_eager: the same with every "
+                                    "kernel launched on its own, the form the committed kernel trace times",
                      "per_unit": (f"{compulsory // cells} B per cell per launch ({cells} cells, {per_launch} "
                                   "iterations): phi^n, phi^(n-1) and the source read once, the last two levels "
                                   "written" if per_launch > 1 else f"16 B per cell ({cells} cells): phi^n, "
