@@ -141,12 +141,11 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     # source and writes its last two levels (20 B per cell), a single launch
     # 16 B per cell; predict + deposit x, v, a, m in and x, v* out (64 B per
     # particle) and the int64 grid read and written (16 B per cell); fixed
-    # point to source 8 B in, cell mass and source out, the int64 grid
-    # cleared (24 B per cell); gradient 4 B in, 16 B out (padded) per cell;
-    # gather + kick x, v, v* in, a, v out (60 B per particle) and the padded
-    # gradient grid read (16 B per cell).  The re-sort every 16 steps is not
-    # counted.
-    own_bytes = (fused * 20 * ncell + sched[1] * 16 * ncell + (16 + 24 + 20 + 16) * ncell + (64 + 60) * n)
+    # point to source 8 B in, the source out, the int64 grid cleared (20 B
+    # per cell); gradient 4 B in, 16 B out (padded) per cell; gather + kick
+    # x, v, v* in, a, v out (60 B per particle) and the padded gradient grid
+    # read (16 B per cell).  The re-sort (every 48 steps) is not counted.
+    own_bytes = (fused * 20 * ncell + sched[1] * 16 * ncell + (16 + 20 + 20 + 16) * ncell + (64 + 60) * n)
     secs = t1 - t0
     out = {}
     if force_err:

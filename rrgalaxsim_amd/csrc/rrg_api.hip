@@ -320,7 +320,13 @@ template <class real> void conservation_common(hipStream_t s, const PState<real>
 // only: the captured work records no memset.
 class StepGraphs {
 public:
-    ~StepGraphs() { clear(); }
+    ~StepGraphs() { clear(); }  // owners synchronise their stream before destroying
+    // Waits for `s` (the stream the graphs are launched on) first, so no
+    // executable graph is destroyed while a launch of it may be in flight.
+    void clear(hipStream_t s) {
+        if (!execs_.empty()) RRG_HIP(hipStreamSynchronize(s));
+        clear();
+    }
     void clear() {
         for (auto& kv : execs_) (void)hipGraphExecDestroy(kv.second);
         execs_.clear();
@@ -640,7 +646,7 @@ private:
     // Queue form: ib and sources per item (0 = default).
     void configure(int ib, int64_t len) {
         const int64_t n = P_.n;
-        graphs_.clear();
+        graphs_.clear(s_);
         if (!ensemble_) {
             plan_ = nbody::plan_direct(n, kind_, int(len), ib);
             if (plan_.slices > 1 && part_.n < size_t(plan_.slices) * 3 * n) part_.alloc(size_t(plan_.slices) * 3 * n);
@@ -1065,7 +1071,7 @@ public:
     }
 
     void precalc() override {  // rrgsim_integration_rect.cuh:53-83
-        graphs_.clear();  // captured segments hold the boundary model by value (the centre of mass may change)
+        graphs_.clear(s_);  // captured segments hold the boundary model by value (the centre of mass may change)
         sort_particles();
         deposit(false);
         if (model_.bc == RRG_BC_MONOPOLE_DIRICHLET || model_.bc == RRG_BC_MONOPOLE_RADIATIVE) {
