@@ -279,22 +279,47 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
                     if (L < K) lq[L - 1][s & 1][w][x[j]] = nw[L][j];
                 }
                 if (L == K && out_row) {
-                    const size_t o = row(Z0 + s - 2 * (K - 1), y);
+                    // outputs written through L2 (read by the next launch) from
+                    // 128^3 on
+                    const size_t o = row(Z0 + s - 2 * (K - 1), y);  // wave-uniform
+                    const auto ra = through_rsrc(qa + o, NX * sizeof(real));
+                    const auto rb = through_rsrc(qb + o, NX * sizeof(real));
+                    if constexpr (NX < 128) {
 #pragma unroll
-                    for (int j = 0; j < XJ; ++j) {
-                        qa[o + x[j]] = h1[K - 1][j];
-                        qb[o + x[j]] = nw[K][j];
+                        for (int j = 0; j < XJ; ++j) {
+                            qa[o + x[j]] = h1[K - 1][j];
+                            qb[o + x[j]] = nw[K][j];
+                        }
+                    } else if constexpr (XC) {
+                        store_through_run<real, XJ>(ra, x[0] * unsigned(sizeof(real)), h1[K - 1]);
+                        store_through_run<real, XJ>(rb, x[0] * unsigned(sizeof(real)), nw[K]);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < XJ; ++j) {
+                            store_through<real>(ra, x[j] * unsigned(sizeof(real)), h1[K - 1][j]);
+                            store_through<real>(rb, x[j] * unsigned(sizeof(real)), nw[K][j]);
+                        }
                     }
                     if constexpr (ADJ) {
                         // the inputs of the K iterations at this cell: the
                         // input plane K below the newest, level L (1 .. K-1)
                         // from K-L steps back
+                        real acc[XJ];
 #pragma unroll
                         for (int j = 0; j < XJ; ++j) {
-                            real acc = FIRST ? Pv[K][j] : in.sb[j] + Pv[K][j];
-                            if constexpr (K == 3) acc = acc + h2[1][j];
-                            acc = acc + h1[K - 1][j];
-                            sbar[o + x[j]] = acc;
+                            acc[j] = FIRST ? Pv[K][j] : in.sb[j] + Pv[K][j];
+                            if constexpr (K == 3) acc[j] = acc[j] + h2[1][j];
+                            acc[j] = acc[j] + h1[K - 1][j];
+                        }
+                        const auto rs = through_rsrc(sbar + o, NX * sizeof(real));
+                        if constexpr (NX < 128) {
+#pragma unroll
+                            for (int j = 0; j < XJ; ++j) sbar[o + x[j]] = acc[j];
+                        } else if constexpr (XC) {
+                            store_through_run<real, XJ>(rs, x[0] * unsigned(sizeof(real)), acc);
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < XJ; ++j) store_through<real>(rs, x[j] * unsigned(sizeof(real)), acc[j]);
                         }
                     }
                 }
