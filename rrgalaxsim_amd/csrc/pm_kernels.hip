@@ -370,11 +370,18 @@ template <class real, bool LAYER>
 __global__ __launch_bounds__(256) void k_fixed_to_source(int64_t ncell, unsigned long long* __restrict__ mfix,
                                                          double inv_scale, real dx3, real rho_mean, int periodic,
                                                          real k_f, real* __restrict__ mass, real* __restrict__ src,
-                                                         WaveModel<real> w) {
+                                                         WaveModel<real> w, int* __restrict__ occ,
+                                                         real* __restrict__ src_empty) {
     const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const long long q = (long long)mfix[c];
     mfix[c] = 0ull;
+    if (!LAYER && occ != nullptr) {
+        // one wave = one 64-cell segment (c of lane 0 is a multiple of 64)
+        const unsigned long long any = __ballot(q != 0);
+        if ((threadIdx.x & 63) == 0) occ[c >> 6] = any != 0ull ? 1 : 0;
+        if (c == 0) *src_empty = source_term<real>((real)((double)0ll * inv_scale), dx3, rho_mean, periodic, k_f);
+    }
     const real m = (real)((double)q * inv_scale);
     if (mass != nullptr) mass[c] = m;
     real kf = k_f;
@@ -993,14 +1000,19 @@ void center_of_mass(hipStream_t s, const PState<real>& p, int e_mass, unsigned l
 
 template <class real>
 void fixed_to_source(hipStream_t s, int nx, unsigned long long* mfix, double inv_scale, real dx, real rho_mean,
-                     bool periodic, real k_f, real* mass, real* src, const WaveModel<real>* model) {
+                     bool periodic, real k_f, real* mass, real* src, const WaveModel<real>* model, int* occ,
+                     real* src_empty) {
     const int64_t ncell = int64_t(nx) * nx * nx;
     const real dx3 = dx * dx * dx;
     const WaveModel<real> w = model ? *model : WaveModel<real>{};
+    if (occ != nullptr && (model != nullptr || ncell % 64 != 0 || src_empty == nullptr))
+        fail(RRG_ERR_INVALID, "fixed_to_source: segment flags need a uniform K_F, nx^3 a multiple of 64 and src_empty");
     if (model) k_fixed_to_source<real, true><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mfix, inv_scale, dx3, rho_mean,
-                                                                               periodic ? 1 : 0, k_f, mass, src, w);
+                                                                               periodic ? 1 : 0, k_f, mass, src, w,
+                                                                               nullptr, nullptr);
     else k_fixed_to_source<real, false><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mfix, inv_scale, dx3, rho_mean,
-                                                                         periodic ? 1 : 0, k_f, mass, src, w);
+                                                                         periodic ? 1 : 0, k_f, mass, src, w, occ,
+                                                                         src_empty);
     RRG_LAUNCH_CHECK();
 }
 
@@ -1053,10 +1065,12 @@ template <class real> int wave_fused_levels(const WaveModel<real>& m, int remain
 
 template <class real>
 void wave_fused(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, int levels, real* qa, real* qb,
-                const real* p, const real* po, const real* src, const int* halt) {
+                const real* p, const real* po, const real* src, const int* halt, const int* occ,
+                const real* src_empty) {
     if (levels < 2 || fused_levels<real>(m.nx, m.bc, levels) != levels)
         fail(RRG_ERR_INVALID, "wave_fused: no launch shape for this grid and iteration count");
-    launch_fused<real, false, false>(s, m.nx, levels, qa, qb, p, po, src, nullptr, k, halt);
+    if ((occ == nullptr) != (src_empty == nullptr)) fail(RRG_ERR_INVALID, "wave_fused: occ and src_empty go together");
+    launch_fused<real, false, false>(s, m.nx, levels, qa, qb, p, po, src, nullptr, k, halt, occ, src_empty);
     RRG_LAUNCH_CHECK();
 }
 
@@ -1177,7 +1191,7 @@ size_t ngp_temp_bytes(int64_t n, int nx) {
     template void predict_deposit<real>(hipStream_t, const PState<real>&, real, bool, const Geo<real>&, double, \
                                         unsigned long long*, double);                                           \
     template void fixed_to_source<real>(hipStream_t, int, unsigned long long*, double, real, real, bool, real,  \
-                                        real*, real*, const WaveModel<real>*);                                  \
+                                        real*, real*, const WaveModel<real>*, int*, real*);                     \
     template WaveModel<real> make_model<real>(int, double, double, double, double, double, double, int);        \
     template void center_of_mass<real>(hipStream_t, const PState<real>&, int, unsigned long long*, real*);       \
     template void cell_coefficients<real>(hipStream_t, const WaveModel<real>&, real*, real*, real*, real*);      \
@@ -1190,7 +1204,7 @@ size_t ngp_temp_bytes(int64_t n, int nx) {
                                        const real*, const real*, const real*, void*, const int*);               \
     template int wave_fused_levels<real>(const WaveModel<real>&, int);                                          \
     template void wave_fused<real>(hipStream_t, const WaveModel<real>&, const WaveK<real>&, int, real*, real*,  \
-                                   const real*, const real*, const real*, const int*);                          \
+                                   const real*, const real*, const real*, const int*, const int*, const real*); \
     template void solve_init<real>(hipStream_t, SolveCtl*, void*);                                              \
     template void solve_decide<real>(hipStream_t, SolveCtl*, void*, real, int, int, int, int);                  \
     template void solve_fixup<real>(hipStream_t, const SolveCtl*, const PhiBufs<real>&, int, int, int64_t);     \

@@ -1057,6 +1057,15 @@ public:
         mfix_.alloc(ncell_);
         mass_.alloc(ncell_);
         src_.alloc(ncell_);
+        // Source segment flags for the fused stencil (periodic grids): measured
+        // faster where the stencil is bound by its memory traffic (FP64 from
+        // 128^3, FP32 from 256^3: 4-14% per iteration on the disk, which
+        // leaves ~90% of the rows without mass), slower on 64^3 and on FP32
+        // 128^3, where it is bound by its instruction issue.
+        if (fused_ && (nx_ >= 256 || (sizeof(real) == 8 && nx_ >= 128))) {
+            occ_.alloc(size_t(ncell_ / 64));
+            src_empty_.alloc(1);
+        }
         for (int i = 0; i < nbuf_; ++i) phi_[i].alloc(ncell_);
         gacc_.alloc(4 * ncell_);
         maxbits_.alloc(2 * pm::kCheckSlots);
@@ -1334,7 +1343,7 @@ private:
             graphs_.run(s_, b0 + 5, [&] {
                 pm::predict_deposit<real>(s_, P_.view(), dt, true, geo_, scale_, mfix_.p, qmax_);
                 pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f,
-                                          nullptr, src_.p, layered());
+                                          nullptr, src_.p, layered(), occ_.p, src_empty_.p);
                 int ip = ip0, ic = ic0;  // from the arrangement before the rotation
                 solve_launches(ip, ic, true);
                 pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
@@ -1349,7 +1358,7 @@ private:
         });
         segment(RRG_STAGE_SOURCE, 1, graph, b0 + 1, [&] {
             pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, nullptr,
-                                      src_.p, layered());
+                                      src_.p, layered(), occ_.p, src_empty_.p);
         });
         iters_on_device_ = false;
         if (graph) {
@@ -1396,7 +1405,7 @@ private:
             [&](int lv, int o1, int o2, int cur, int prev) {
                 if (launch)
                     pm::wave_fused<real>(s_, model_, k_, lv, phi_[o1].p, phi_[o2].p, phi_[cur].p, phi_[prev].p,
-                                         src_.p, halt);
+                                         src_.p, halt, occ_.p, src_empty_.p);
                 ++launches;
             });
         return launches;
@@ -1590,7 +1599,7 @@ private:
         timer.stop(s_, t, 1);
         t = timer.start(s_, RRG_STAGE_SOURCE);
         pm::fixed_to_source<real>(s_, nx_, mfix_.p, inv_scale_, geo_.dx, rho_mean_, periodic_, k_.k_f, nullptr, src_.p,
-                                  layered());
+                                  layered(), occ_.p, src_empty_.p);
         timer.stop(s_, t, 1);
     }
 
@@ -1670,6 +1679,10 @@ private:
     StepGraphs graphs_;
     DBuf<unsigned long long> mfix_;
     DBuf<real> mass_, src_, phi_[4], gacc_;
+    // (fused stencil) 64-cell segments of the source that hold mass, and the
+    // source value of an empty cell; the stencil skips reading the others
+    DBuf<int> occ_;
+    DBuf<real> src_empty_;
     bool fused_ = false;  // several iterations per launch
     int nbuf_ = 3;       // potential grids in the rotation
     DBuf<unsigned long long> maxbits_;

@@ -286,11 +286,15 @@ void radix_sort_pairs(hipStream_t s, void* temp, size_t& temp_bytes, const unsig
 // Fixed-point grid -> stencil source term K_F * 4 pi G (m/dx^3 - rho_mean)
 // and, when `mass` is non-null, the cell mass (real); clears the fixed-point
 // grid for the next deposit.  On a non-periodic grid (`model` non-null) K_F
-// is the cell's own.
+// is the cell's own.  With a uniform K_F (`model` null) and `occ` non-null
+// (nx^3 a multiple of 64) it also writes, per 64-cell segment of the grid,
+// occ[c / 64] = 1 if a cell of the segment holds mass and 0 if none does,
+// and *src_empty = the source term of an empty cell, which every cell of an
+// unmarked segment holds.
 template <class real>
 void fixed_to_source(hipStream_t s, int nx, unsigned long long* mfix, double inv_scale, real dx,
                      real rho_mean, bool periodic, real k_f, real* mass, real* src,
-                     const WaveModel<real>* model = nullptr);
+                     const WaveModel<real>* model = nullptr, int* occ = nullptr, real* src_empty = nullptr);
 
 // Cell mass (real) -> stencil source term, for callers that hold a mass grid.
 template <class real>
@@ -320,9 +324,13 @@ void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& 
 // should run when r remain: 3 or 2 where a launch shape exists (periodic
 // grids of the sizes it lists), else 1 (one wave_iteration).
 template <class real> int wave_fused_levels(const WaveModel<real>& m, int remaining);
+// With `occ` and `src_empty` from fixed_to_source (optional), the source
+// rows of unmarked segments are not read from src: their cells hold
+// *src_empty.
 template <class real>
 void wave_fused(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, int levels, real* qa, real* qb,
-                const real* p, const real* po, const real* src, const int* halt = nullptr);
+                const real* p, const real* po, const real* src, const int* halt = nullptr, const int* occ = nullptr,
+                const real* src_empty = nullptr);
 
 // Steady-state check of a solve decided on the device (see pm_kernels.hip):
 // `done` stops the solve's later stencil launches (pass &ctl->done as their

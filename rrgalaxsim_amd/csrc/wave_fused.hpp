@@ -103,14 +103,17 @@ __device__ __forceinline__ void row_neighbours(const real (&r)[XJ], real (&m)[XJ
     }
 }
 
-template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST>
+template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST,
+          bool OCC = false>
 __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real* __restrict__ qa,
                                                                            real* __restrict__ qb,
                                                                            const real* __restrict__ p,
                                                                            const real* __restrict__ po,
                                                                            const real* __restrict__ src,
                                                                            real* __restrict__ sbar, WaveK<real> k,
-                                                                           const int* __restrict__ halt) {
+                                                                           const int* __restrict__ halt,
+                                                                           const int* __restrict__ occ,
+                                                                           const real* __restrict__ src_empty) {
     // a converged solve's remaining launches return at once (uniform, before
     // any barrier)
     if (halt != nullptr && *halt != 0) return;
@@ -150,8 +153,30 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     // offset of row yy of plane z (z wraps)
     auto row = [](int z, int yy) { return size_t(unsigned((z + NX) & (NX - 1)) * unsigned(NX) + unsigned(yy)) * NX; };
     const int zs = Z0 - (K - 1);  // level-1 plane of step 0
-    // the inputs of step t: the level-1 plane zs+t, and (adjoint) sbar on the
-    // output plane of step t
+    // (OCC: forward, with occ) bit t: the source row of step t holds mass.  For
+    // the rows without, only lane 0 issues the row's load (of its own
+    // phi^(n-1) cell: a cache hit; every path issues the same load
+    // instruction, so the waits on the prefetched planes stay exact, and a
+    // one-lane load costs the memory pipeline almost nothing) and the step
+    // uses se, the empty-cell value, instead.  Lane t fetches the flags of step t (the
+    // 64-cell segments seg*XJ .. seg*XJ+XJ-1 of its row) with one vector
+    // load issued before the planes' loads.  The first PD planes read their
+    // rows regardless.  (Shapes marching more than 64 steps read every row.)
+    static_assert(!OCC || (!ADJ && S <= 64), "source flags: forward shapes of at most 64 steps");
+    unsigned long long occm = ~0ull;
+    real se = 0;
+    if constexpr (OCC) {
+        {
+            se = *src_empty;
+            int f = 0;
+            if (l >= PD && l < S) {
+                const int* __restrict__ fp = occ + (row(zs + l, y) >> 6) + size_t(seg) * XJ;
+#pragma unroll
+                for (int i = 0; i < XJ; ++i) f |= fp[i];
+            }
+            occm = __ballot(f != 0) | ((1ull << PD) - 1);
+        }
+    }
     auto load = [&](PlaneIn<real, XJ>& in, int t) {
         const int zq = zs + t;
         const real* __restrict__ r0 = p + row(zq, y);
@@ -178,7 +203,16 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
             in.po[j] = ro[x[j]];
             in.pn[j] = r2[x[j]];
         }
-        if constexpr (!ADJ) {
+        if constexpr (OCC) {
+            // a row without mass: lane 0 alone loads (its phi^(n-1) cell), so
+            // every path issues the same load instruction
+            const bool full = (occm >> t) & 1ull;
+            const real* __restrict__ rs = full ? src + row(zq, y) : ro;
+            if (full || l == 0) {
+#pragma unroll
+                for (int j = 0; j < XJ; ++j) in.src[j] = rs[x[j]];
+            }
+        } else if constexpr (!ADJ) {
             const real* __restrict__ rs = src + row(zq, y);
 #pragma unroll
             for (int j = 0; j < XJ; ++j) in.src[j] = rs[x[j]];
@@ -243,6 +277,10 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
 #pragma unroll
         for (int j = 0; j < XJ; ++j) {
             if constexpr (!ADJ) {
+                // a row without mass: the empty-cell value (the load read phi^(n-1))
+                if constexpr (OCC) {
+                    if (!((occm >> s) & 1ull)) in.src[j] = se;
+                }
 #pragma unroll
                 for (int i = K - 1; i > 0; --i) sh[i][j] = sh[i - 1][j];
                 sh[0][j] = in.src[j];
@@ -347,10 +385,20 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
 
 template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST>
 void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* po, const real* src, real* sbar,
-                  const WaveK<real>& k, const int* halt) {
+                  const WaveK<real>& k, const int* halt, const int* occ, const real* src_empty) {
     constexpr int NX = 64 * XJ * RS;
-    k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST>
-        <<<(NX / TY) * (NX / ZB), 64 * RS * (TY + 2 * K - 2), 0, s>>>(qa, qb, p, po, src, sbar, k, halt);
+    constexpr dim3 grid((NX / TY) * (NX / ZB)), block(64 * RS * (TY + 2 * K - 2));
+    // source flags: the shapes that use them (FP64 from 128^3, FP32 from
+    // 256^3, at most 64 planes marched); the others read every source row
+    constexpr bool kOcc = !ADJ && ZB + 2 * K - 2 <= 64 && (NX >= 256 || (sizeof(real) == 8 && NX >= 128));
+    if (kOcc && occ != nullptr) {
+        if constexpr (kOcc)
+            k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST, true>
+                <<<grid, block, 0, s>>>(qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+    } else {
+        k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST, false>
+            <<<grid, block, 0, s>>>(qa, qb, p, po, src, sbar, k, halt, nullptr, nullptr);
+    }
 }
 
 // Launch shapes (K, XJ, RS, TY, ZB, PD, YL, XC), from sweeps of
@@ -392,55 +440,56 @@ template <class real> int fused_levels(int nx, int bc, int remaining, int cap = 
 // fused_levels(nx, periodic, levels) == levels.
 template <class real, bool ADJ, bool FIRST>
 void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const real* p, const real* po,
-                  const real* src, real* sbar, const WaveK<real>& k, const int* halt = nullptr) {
+                  const real* src, real* sbar, const WaveK<real>& k, const int* halt = nullptr,
+                  const int* occ = nullptr, const real* src_empty = nullptr) {
     constexpr bool f32 = std::is_same_v<real, float>;
     if constexpr (!ADJ) {
         if (levels == 4) {
             if (nx == 128) {
                 if constexpr (f32)
-                    launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                    launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
                 else
-                    launch_shape<real, 4, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                    launch_shape<real, 4, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
             } else if constexpr (f32) {
-                launch_shape<real, 4, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 4, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
             }
             return;
         }
     }
     if (levels == 3) {
         if (nx == 64) {
-            launch_shape<real, 3, 1, 1, 4, 4, 1, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+            launch_shape<real, 3, 1, 1, 4, 4, 1, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
         } else if (nx == 128) {
-            launch_shape<real, 3, 2, 1, 8, 8, f32 ? 2 : 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+            launch_shape<real, 3, 2, 1, 8, 8, f32 ? 2 : 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
         } else if (nx == 256) {
             if constexpr (f32)
-                launch_shape<real, 3, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 3, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
             else
-                launch_shape<real, 3, 2, 2, 4, 64, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 3, 2, 2, 4, 64, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
         } else if constexpr (f32) {
-            launch_shape<real, 3, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+            launch_shape<real, 3, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
         }
     } else {
         if (nx == 64) {
             if constexpr (f32)
-                launch_shape<real, 2, 1, 1, 4, 4, 2, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 2, 1, 1, 4, 4, 2, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
             else
-                launch_shape<real, 2, 1, 1, 4, 4, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 2, 1, 1, 4, 4, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
         } else if (nx == 128) {
             if constexpr (f32)
-                launch_shape<real, 2, 1, 2, 4, 8, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 2, 1, 2, 4, 8, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
             else
-                launch_shape<real, 2, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 2, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
         } else if (nx == 256) {
             if constexpr (f32)
-                launch_shape<real, 2, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 2, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
             else
-                launch_shape<real, 2, 2, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 2, 2, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
         } else {
             if constexpr (f32)
-                launch_shape<real, 2, 8, 1, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 2, 8, 1, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
             else
-                launch_shape<real, 2, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt);
+                launch_shape<real, 2, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
         }
     }
 }
