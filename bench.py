@@ -112,15 +112,18 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     stats = sim.stats()
     sim.close()
     # the stencil launch time with every kernel launched on its own (no
-    # graph), the form a kernel tracer times (bench.py --profile-pass --eager)
+    # graph), the form a kernel tracer times (bench.py --profile-pass --eager),
+    # and each stage's eager time over steps 16-19 after the re-sort of
+    # precalc (the deposit's time grows as particles leave their sorted runs)
     pe = rrg.make_params(dt=PM["dt"], grid=dict(nx=PM["nx"], dx=PM["dx"], bc_frac=0.0),
                          wave=dict(wave, setup_iterations=200), real=real, graphs=False)
     with rrg.Simulation(pe, pos, vel, m) as se:
         se.precalc()
-        se.calc_step(1)
+        se.calc_step(16)
         se.set_timing(True)
         se.calc_step(3)
-        wave_eager_ms, wave_eager_launches = se.timing()["wave"]
+        eager_tm = se.timing()
+        wave_eager_ms, wave_eager_launches = eager_tm["wave"]
     ncell = PM["nx"] ** 3
     wave_ms, wave_launches = tm["wave"]
     avg_wave_s = wave_ms / 1e3 / max(wave_launches, 1)
@@ -169,6 +172,8 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
         "ms_per_step_resort_amortized": secs / steps * 1e3 + tm.get("sort", (0.0, 0))[0] / estep,
         "wave_avg_launch_us": avg_wave_s * 1e6,
         "wave_avg_launch_us_eager": wave_eager_ms * 1e3 / max(wave_eager_launches, 1),
+        "stage_us_eager_16_after_sort": {k: v[0] * 1e3 / 3 for k, v in eager_tm.items()
+                                         if k in ("deposit", "source", "gradient", "gather") and v[1]},
         "wave_ms_per_step": wave_ms / estep,
         "wave_us_per_iteration": wave_ms * 1e3 / (iters * estep),
         "wave_launches_per_step": {"four_iterations": sched[4], "three_iterations": sched[3],
@@ -827,6 +832,7 @@ def main():
                           "effective_gbs": pm["step_effective_bytes"] / step_s / 1e9},
         "ms_per_step_with_stage_events": pm["ms_per_step_with_stage_events"],
         "stage_ms_per_step": pm["stage_ms_per_step"],
+        "stage_us_eager_16_after_sort": pm["stage_us_eager_16_after_sort"],
         "stage_event_steps": pm["stage_event_steps"],
         "resort_every_steps": pm["resort_every_steps"],
         "resorts_in_stage_steps": pm["resorts_in_stage_steps"],
