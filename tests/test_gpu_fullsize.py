@@ -1,6 +1,6 @@
 """The workloads at their stated sizes on the GPU, against the CPU oracle.
 
-* PM headline: exponential disk N = 1,000,000 (seed 1), FP32, 128^3 cells on
+* PM headline: exponential disk N = 1,000,000 (seed 1), FP32 (and FP64), 128^3 cells on
   [-8, 8]^3, 200 unchecked damped-wave iterations per step, dt = 5e-3, 18
   steps with the particle arrays re-sorted by cell every 16 steps (the run
   crosses one re-sort; the default interval is 48, the same code).  Positions, velocities, accelerations and the
@@ -32,11 +32,11 @@ pytestmark = pytest.mark.gpu
 C = 4574.337022617616
 
 
-def _pm_headline(rrg, timing, steps):
+def _pm_headline(rrg, timing, steps, real="fp32"):
     pos, vel, m = ics.exp_disk(1_000_000, seed=1)
     wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=200, iterations=200,
                 tolerance=0.0)
-    p = rrg.make_params(dt=5e-3, grid=dict(nx=128, dx=0.125, bc_frac=0.0), wave=wave, real="fp32", sort_interval=16)
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=128, dx=0.125, bc_frac=0.0), wave=wave, real=real, sort_interval=16)
     with rrg.Simulation(p, pos, vel, m) as sim:
         sim.precalc()
         sim.sync_particles_and_grid()
@@ -66,6 +66,22 @@ def test_pm_headline_workload_bitwise(rrg, headline_oracle, timing):
     for k in ("pos", "vel", "acc"):
         assert np.array_equal(st[k].astype(np.float32), o[k]), k
     assert np.array_equal(phi.astype(np.float32), o["phi"])
+
+
+def test_pm_headline_workload_fp64_bitwise(rrg, orc):
+    """The headline workload in FP64 (the reference's default precision):
+    the four-iteration FP64 stencil reads only the source rows that hold
+    mass (the disk leaves ~90% of them empty) and must still give the
+    oracle's bits after 18 steps across a re-sort."""
+    print("PM headline N=1e6 128^3 fp64, 18 steps", flush=True)
+    pos, vel, m, st, phi, stats = _pm_headline(rrg, False, 18, real="fp64")
+    o = orc.pm_run(pos, vel, m, 18, nx=128, dx=0.125, dt=5e-3, diss_base=0.785, wave_speed=C,
+                   setup_iterations=200, iterations=200, tolerance=0.0, fixed_e=stats["fixed_exponent"],
+                   dtype=np.float64)
+    assert stats["steps"] == 18 and stats["graph_launches"] == 18
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(st[k], o[k]), k
+    assert np.array_equal(phi, o["phi"])
 
 
 def test_ensemble_64_galaxies(rrg, orc):
