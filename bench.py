@@ -605,6 +605,27 @@ def cpu_baseline(ics, target_s=10.0, real="fp32"):
                       "precalc"}
 
 
+def cpu_baseline_stencil(nx, real, iters, dx=None):
+    """The oracle's wave_diss_iteration (wave.cu:106-193, periodic, OpenMP
+    over cells) on an nx^3 grid: microseconds per iteration over `iters`
+    iterations (the values do not change the work)."""
+    oracle = _oracle()
+    dtype = np.float32 if real == "fp32" else np.float64
+    dx = 16.0 / nx if dx is None else dx
+    ncell = nx ** 3
+    phi, old, mass = np.zeros(ncell, dtype), np.zeros(ncell, dtype), np.zeros(ncell, dtype)
+    dt = 0.5 * dx / (C * np.sqrt(3.0))
+    oracle.wave_iteration(nx, dx, PM["diss_base"], C, dt, True, 1.0 / (nx * dx) ** 3, phi, old, mass, dtype=dtype)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        phi, old = oracle.wave_iteration(nx, dx, PM["diss_base"], C, dt, True, 1.0 / (nx * dx) ** 3, phi, old, mass,
+                                         dtype=dtype), phi
+    t = time.perf_counter() - t0
+    return {"value": t / iters * 1e6, "unit": f"us per stencil iteration ({nx}^3 {real}, lower is better)",
+            "cores": _cores(), "kind": "port",
+            "sample": f"oracle wave_iteration, {nx}^3 {real}, {iters} iterations after one untimed, {t:.2f} s"}
+
+
 def cpu_baseline_c1(ics):
     """The oracle (FP64, IEEE 1/sqrt as nbody.cu:50-53) on the whole C1 run:
     precalc + 100 steps of Plummer N=4096, N^2 interactions per evaluation."""
@@ -875,7 +896,10 @@ def main():
         per_leg = [("direct_sum", cpu_baseline_direct), ("direct_sum_c1", cpu_baseline_c1),
                    ("ensemble", cpu_baseline_ensemble), ("adjoint", cpu_baseline_adjoint),
                    ("adjoint_pm", cpu_baseline_adjoint_pm),
-                   ("pm_fp64", lambda i: cpu_baseline(i, target_s=5.0, real="fp64"))]
+                   ("pm_fp64", lambda i: cpu_baseline(i, target_s=5.0, real="fp64")),
+                   ("wave_scaling", lambda i: {"128": cpu_baseline_stencil(128, "fp32", 20)}),
+                   ("wave_scaling_fp64", lambda i: {"128": cpu_baseline_stencil(128, "fp64", 20)}),
+                   ("sample_workload", lambda i: cpu_baseline_stencil(512, "fp64", 3, dx=0.025))]
         for key, fn in per_leg:
             if key in line:
                 line[key]["cpu_baseline"] = fn(ics)
