@@ -94,51 +94,57 @@ __device__ __forceinline__ unsigned cic_fixed(const real r[3], real m, const Geo
     return c[0];
 }
 
-// Predictor fused with the deposit.  Adjacent lanes of a wave whose
-// particles share a base cell (runs of equal keys; the particle arrays are
-// kept sorted by cell, so dense regions form long runs) are merged by a
-// segmented inclusive scan over the wave; the last lane of each run then adds
-// its 8 corner sums into a workgroup grid in LDS spanning the bounding box of
-// the workgroup's CIC stencils, and the workgroup adds each nonzero box cell
-// to the global grid with one atomic.  Particles move less than a cell
-// between the periodic re-sorts, so the 256 particles of a workgroup stay in
-// a small box (median 60 cells, 99% within 1,711, on the 10^6-particle disk
-// at 128^3, 16 steps after a sort) and the LDS atomics absorb the runs that
-// the drift splits (cheaper than re-sorting each wave: 35 against 39 us per
-// step on that disk); a workgroup whose box exceeds kDepositBox cells
-// issues each run's 8 atomics directly.  Integer addition is exact, so the
-// grid does not depend on the merging.
+// Predictor fused with the deposit.  Adjacent lanes of a row of 16 lanes
+// whose particles share a base cell (runs of equal keys; the particle arrays
+// are kept sorted by cell, so dense regions form long runs) are merged by a
+// segmented inclusive scan within the row; the last lane of each run's piece
+// in a row then adds its 8 corner sums into a workgroup grid in LDS spanning
+// the bounding box of the workgroup's CIC stencils, and the workgroup adds
+// each nonzero box cell to the global grid with one atomic.  A run that
+// crosses rows leaves one tail per row (at most four LDS atomics on one
+// address per instruction); keeping the scan inside rows needs four DPP
+// steps that write every lane (bound_ctrl) and stops as soon as every tail
+// holds its whole piece, instead of six steps with masked cross-row
+// broadcasts.  Particles move less than a cell between the periodic re-sorts,
+// so the 256 particles of a workgroup stay in a small box (median 60 cells,
+// 99% within 1,711, on the 10^6-particle disk at 128^3, 16 steps after a
+// sort) and the LDS atomics absorb the runs that the drift splits; a
+// workgroup whose box exceeds kDepositBox cells issues each tail's 8 atomics
+// directly.  Integer addition is exact, so the grid does not depend on the
+// merging.
 constexpr unsigned kNoCell = 0xffffffffu;
 constexpr int kDepositBox = 2048;
 
-// Lane values moved by a DPP pattern (row_shr:n = 0x110+n within rows of 16
-// lanes, row_bcast:15 = 0x142, row_bcast:31 = 0x143); lanes without a source
-// or outside ROWS read 0.
-template <int CTRL, int ROWS> __device__ __forceinline__ int dpp_i32(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+// v of lane l - n (CTRL row_shr:n = 0x110 + n) or l + n (row_shl:n = 0x100 +
+// n) within its row of 16 lanes; lanes whose source lies outside the row read
+// 0 (bound_ctrl), so every lane is written and no old value is needed.
+template <int CTRL> __device__ __forceinline__ int row_mov(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
 }
-template <int CTRL, int ROWS> __device__ __forceinline__ long long dpp_i64(long long v) {
-    const unsigned long long u = (unsigned long long)v;
-    const unsigned lo = (unsigned)dpp_i32<CTRL, ROWS>((int)(unsigned)u);
-    const unsigned hi = (unsigned)dpp_i32<CTRL, ROWS>((int)(unsigned)(u >> 32));
-    return (long long)(((unsigned long long)hi << 32) | lo);
+template <int CTRL> __device__ __forceinline__ long long row_mov64(long long v) {
+    return __builtin_amdgcn_update_dpp((long long)0, v, CTRL, 0xf, 0xf, true);
 }
 // One step of the segmented scan on (q, f), f = "a run head lies in my
 // window": the window to the left joins unless a head is already in mine.
-template <int CTRL, int ROWS> __device__ __forceinline__ void seg_step(long long (&q)[8], int& f) {
-    const int fs = dpp_i32<CTRL, ROWS>(f);
+// Row starts are heads, so no window reaches across a row.
+template <int CTRL> __device__ __forceinline__ void seg_step(long long (&q)[8], int& f) {
+    const int fs = row_mov<CTRL>(f);
+    long long s[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const long long s = dpp_i64<CTRL, ROWS>(q[k]);
-        if (!f) q[k] += s;
+    for (int k = 0; k < 8; ++k) s[k] = row_mov64<CTRL>(q[k]);
+    if (!f) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] += s[k];
     }
     f |= fs;
 }
 
-// Minimum / maximum over the wave by DPP row shifts and row broadcasts (the
-// pattern of the scan above), read from lane 63; `ident` fills lanes
-// without a source.
-template <bool MAX> __device__ __forceinline__ int wave_ext(int v, int ident) {
+// Minimum / maximum over the wave by DPP row shifts and row broadcasts, read
+// from lane 63.  Lanes without a source read the identity (INT_MAX for min,
+// INT_MIN for max), which lets the compiler fold each step into one DPP min /
+// max.
+template <bool MAX> __device__ __forceinline__ int wave_ext(int v) {
+    constexpr int ident = MAX ? int(0x80000000u) : 0x7fffffff;
     auto step = [&](int t) { v = MAX ? max(v, t) : min(v, t); };
     step(__builtin_amdgcn_update_dpp(ident, v, 0x111, 0xf, 0xf, false));
     step(__builtin_amdgcn_update_dpp(ident, v, 0x112, 0xf, 0xf, false));
@@ -149,12 +155,22 @@ template <bool MAX> __device__ __forceinline__ int wave_ext(int v, int ident) {
     return __builtin_amdgcn_readlane(v, 63);
 }
 
+// e / d for 0 <= e < 2^12 and 1 <= d <= 2^12, given rd = 1.f / d: the float
+// estimate is within 1e-3 of e / d, so one correction either way makes it
+// exact.
+__device__ __forceinline__ int small_div(int e, int d, float rd) {
+    int q = (int)((float)e * rd);
+    const int r = e - __mul24(q, d);
+    if (r < 0) --q;
+    else if (r >= d) ++q;
+    return q;
+}
+
 template <class real, bool FASTQ>
 __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt, int do_predict, Geo<real> g,
                                                          double scale, unsigned long long* __restrict__ mfix) {
     __shared__ unsigned long long box[kDepositBox];
     __shared__ int bnd[6];  // the workgroup's min i0, j0, k0 and max i1, j1, k1
-    for (int e = threadIdx.x; e < kDepositBox; e += blockDim.x) box[e] = 0ull;
     if (threadIdx.x < 6) bnd[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : -1;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -165,30 +181,39 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
     unsigned c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int base[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, step[3] = {0, 0, 0};
     if (i < n) {
-        real r[3];
+        // every load is issued before the first store: the particle arrays
+        // are not declared disjoint, so a load after a store would wait for it
+        real r[3], v[3] = {0, 0, 0}, a[3] = {0, 0, 0};
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            r[d] = p.x[d * n + i];
-            if (do_predict) {  // predict_step_ (nbody.cu:135-151)
-                const real v = p.v[d * n + i];
-                const real a = p.a[d * n + i];
-                const real vp = v + dt * a;
+        for (int d = 0; d < 3; ++d) r[d] = p.x[d * n + i];
+        if (do_predict) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                v[d] = p.v[d * n + i];
+                a[d] = p.a[d * n + i];
+            }
+        }
+        const real m = p.m[i];
+        if (do_predict) {  // predict_step_ (nbody.cu:135-151)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const real vp = v[d] + dt * a[d];
                 p.w[d * n + i] = vp;
-                r[d] = (real)((double)r[d] + ((double)dt * 0.5) * (double)(v + vp));
+                r[d] = (real)((double)r[d] + ((double)dt * 0.5) * (double)(v[d] + vp));
                 p.x[d * n + i] = r[d];
             }
         }
-        key = cic_fixed<real, FASTQ>(r, p.m[i], g, scale, q, c, base, step);
+        key = cic_fixed<real, FASTQ>(r, m, g, scale, q, c, base, step);
     }
     // the wave's part of the bounding box of the stencils (lanes past n add
     // nothing: base INT_MAX, base + step -1 is below every corner)
     int lo[3], hi[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        lo[d] = wave_ext<false>(base[d], 0x7fffffff);
-        hi[d] = wave_ext<true>(i < n ? base[d] + step[d] : -1, -1);
+        lo[d] = wave_ext<false>(base[d]);
+        hi[d] = wave_ext<true>(i < n ? base[d] + step[d] : -1);
     }
-    __syncthreads();  // box cleared, bounds initialised
+    __syncthreads();  // bounds initialised
     if (lane == 0) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -196,24 +221,34 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
             atomicMax(&bnd[3 + d], hi[d]);
         }
     }
-    const unsigned kprev = __shfl_up(key, 1, 64);
-    const unsigned knext = __shfl_down(key, 1, 64);
-    int f = (lane == 0 || kprev != key) ? 1 : 0;  // run head
-    const bool tail = (lane == 63 || knext != key);
-    // inclusive segmented scan: within rows of 16 lanes, then across rows
-    seg_step<0x111, 0xf>(q, f);
-    seg_step<0x112, 0xf>(q, f);
-    seg_step<0x114, 0xf>(q, f);
-    seg_step<0x118, 0xf>(q, f);
-    seg_step<0x142, 0xa>(q, f);
-    seg_step<0x143, 0xc>(q, f);
+    // run heads and tails within rows of 16 lanes
+    const int rl = lane & 15;
+    const unsigned kprev = (unsigned)row_mov<0x111>((int)key);
+    const unsigned knext = (unsigned)row_mov<0x101>((int)key);
+    int f = (rl == 0 || kprev != key) ? 1 : 0;
+    const bool tail = (rl == 15 || knext != key);
+    // inclusive segmented scan within the row, until every tail holds its run
+    // piece (a uniform exit)
+    seg_step<0x111>(q, f);
+    if (__ballot(tail && !f) != 0ull) {
+        seg_step<0x112>(q, f);
+        if (__ballot(tail && !f) != 0ull) {
+            seg_step<0x114>(q, f);
+            if (__ballot(tail && !f) != 0ull) seg_step<0x118>(q, f);
+        }
+    }
     __syncthreads();  // bounds final
     const int bx = bnd[0], by = bnd[1], bz = bnd[2];
     const int ex = bnd[3] - bx + 1, ey = bnd[4] - by + 1, ez = bnd[5] - bz + 1;
     const bool in_box = bnd[3] >= 0 && ex * ey * ez <= kDepositBox;  // uniform over the workgroup
+    const int exy = in_box ? ex * ey : 0, vol = in_box ? exy * ez : 0;
+    if (in_box) {
+        for (int e = threadIdx.x; e < vol; e += blockDim.x) box[e] = 0ull;
+        __syncthreads();  // the box's cells cleared
+    }
     if (tail && key != kNoCell) {
         if (in_box) {
-            const int di = int(c[1] != c[0]), dj = int(c[2] != c[0]) * ex, dk = int(c[4] != c[0]) * ex * ey;
+            const int di = int(c[1] != c[0]), dj = int(c[2] != c[0]) * ex, dk = int(c[4] != c[0]) * exy;
             const int b0 = (base[0] - bx) + ex * ((base[1] - by) + ey * (base[2] - bz));
 #pragma unroll
             for (int k = 0; k < 8; ++k)  // corner k: steps (k & 1, (k >> 1) & 1, k >> 2); adding 0 is harmless
@@ -224,14 +259,14 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
                 if (q[k] != 0) atomicAdd(mfix + c[k], (unsigned long long)q[k]);
         }
     }
-    __syncthreads();
     if (in_box) {
-        const int exy = ex * ey, vol = exy * ez;
+        __syncthreads();
+        const float rxy = 1.f / float(exy), rx = 1.f / float(ex);
         for (int e = threadIdx.x; e < vol; e += blockDim.x) {
             const unsigned long long v = box[e];
             if (v != 0ull) {
-                const int lk = e / exy, r = e - lk * exy;
-                const int lj = r / ex, li = r - lj * ex;
+                const int lk = small_div(e, exy, rxy), r = e - lk * exy;
+                const int lj = small_div(r, ex, rx), li = r - lj * ex;
                 atomicAdd(mfix + (unsigned(bx + li) + unsigned(NX) * (unsigned(by + lj) + unsigned(NX) * unsigned(bz + lk))),
                           v);
             }
@@ -769,6 +804,16 @@ __global__ __launch_bounds__(256) void k_gather(PState<real> p, Geo<real> g, con
     const int nx = g.nx;
     const real dx = g.dx, domain_min = g.domain_min;
     const real px = p.x[q], py = p.x[n + q], pz = p.x[2 * n + q];
+    // the kick's inputs are loaded before the first store (the particle
+    // arrays are not declared disjoint, so a later load would wait for it)
+    real v[3] = {0, 0, 0}, vv[3] = {0, 0, 0};
+    if (KICK) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] = p.v[c * n + q];
+            vv[c] = p.w[c * n + q];
+        }
+    }
     const real fx = (px - domain_min) / dx - (real)0.5f;
     const real fy = (py - domain_min) / dx - (real)0.5f;
     const real fz = (pz - domain_min) / dx - (real)0.5f;
@@ -810,11 +855,7 @@ __global__ __launch_bounds__(256) void k_gather(PState<real> p, Geo<real> g, con
     if (KICK) {  // correct_step_ (nbody.cu:171-180)
         const real a3[3] = {ax, ay, az};
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const real v = p.v[c * n + q];
-            const real vv = p.w[c * n + q];
-            p.v[c * n + q] = (real)(0.5 * (double)(v + vv + dt * a3[c]));
-        }
+        for (int c = 0; c < 3; ++c) p.v[c * n + q] = (real)(0.5 * (double)(v[c] + vv[c] + dt * a3[c]));
     }
 }
 
