@@ -301,23 +301,30 @@ __global__ void k_predict(PState<real> p, real dt, float4* __restrict__ A, float
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     const int64_t n = p.n;
+    // every load is issued before the first store: the particle arrays are
+    // not declared disjoint, so a load after a store would wait for it
+    real v[3], r[3], a[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        v[c] = p.v[c * n + i];
+        r[c] = p.x[c * n + i];
+        a[c] = p.a[c * n + i];
+    }
+    const double m = PACK ? (double)p.m[i] : 0.0, s2 = PACK ? (double)p.s2[i] : 0.0;
     double xn[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const real v = p.v[c * n + i];
-        const real r = p.x[c * n + i];
-        const real a = p.a[c * n + i];
-        const real vp = v + dt * a;
+        const real vp = v[c] + dt * a[c];
         p.w[c * n + i] = vp;
-        const real rn = (real)((double)r + ((double)dt * 0.5) * (double)(v + vp));
+        const real rn = (real)((double)r[c] + ((double)dt * 0.5) * (double)(v[c] + vp));
         p.x[c * n + i] = rn;
         xn[c] = (double)rn;
     }
     if constexpr (PACK) {
         const float xh = float(xn[0]), yh = float(xn[1]), zh = float(xn[2]);
-        A[i] = make_float4(xh, yh, zh, float(p.m[i]));
+        A[i] = make_float4(xh, yh, zh, float(m));
         B[i] = make_float4(float(xn[0] - double(xh)), float(xn[1] - double(yh)), float(xn[2] - double(zh)),
-                           float(0.5 * (double)p.s2[i]));
+                           float(0.5 * s2));
     }
 }
 
@@ -335,20 +342,24 @@ __global__ void k_correct(PState<real> p, real dt, int slices, const unsigned ch
     if (i >= p.n) return;
     const int64_t n = p.n;
     const int si = slices_of(i, slices, ns);
+    // loads first (see k_predict)
+    real a[3], v[3], vv[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        real a;
         if (part) {
             double s = part[c * n + i];
             for (int q = 1; q < si; ++q) s += part[(int64_t(q) * 3 + c) * n + i];
-            a = (real)s;
-            p.a[c * n + i] = a;
+            a[c] = (real)s;
         } else {
-            a = p.a[c * n + i];
+            a[c] = p.a[c * n + i];
         }
-        const real v = p.v[c * n + i];
-        const real vv = p.w[c * n + i];
-        p.v[c * n + i] = (real)(0.5 * (double)(v + vv + dt * a));
+        v[c] = p.v[c * n + i];
+        vv[c] = p.w[c * n + i];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (part) p.a[c * n + i] = a[c];
+        p.v[c * n + i] = (real)(0.5 * (double)(v[c] + vv[c] + dt * a[c]));
     }
 }
 

@@ -473,6 +473,12 @@ public:
             time_ += prm_.sim.dt_dynamics;
             if (recording_) {
                 ++rec_steps_;
+                // positions of the open segment (its steps 1 .. K): the
+                // reverse sweep takes the last segment from here instead of
+                // recomputing it
+                const int64_t slot = rec_steps_ - 1 - ck_every_ * ((rec_steps_ - 1) / ck_every_);
+                RRG_HIP(hipMemcpyAsync(tape_.p + size_t(slot) * 3 * P_.n, P_.X.p, sizeof(double) * 3 * P_.n,
+                                       hipMemcpyDeviceToDevice, s_));
                 if (rec_steps_ % ck_every_ == 0) store_checkpoint();
             }
         }
@@ -486,6 +492,7 @@ public:
         ck_every_ = every;
         rec_steps_ = 0;
         ck_.clear();
+        tape_.alloc(size_t(every) * 3 * P_.n);
         recording_ = true;
         store_checkpoint();
     }
@@ -523,6 +530,12 @@ public:
         const size_t v3 = sizeof(double) * 3 * n;
         for (int c = nseg - 1; c >= 0; --c) {
             const int s0 = c * K, len = std::min(T, s0 + K) - s0;
+            if (c == nseg - 1) {
+                // the last segment's positions were kept by the recorded run;
+                // reverse steps n = s0+q, q = len-1 .. 0, each at x_{n+1}
+                for (int q = len - 1; q >= 0; --q) reverse_step(tape_.p + size_t(q) * 3 * n, dt, false);
+                continue;
+            }
             // recompute the segment forward on the scratch state
             const DBuf<double>& ck = ck_[c];
             RRG_HIP(hipMemcpyAsync(R_.X.p, ck.p, v3, hipMemcpyDeviceToDevice, s_));
@@ -958,6 +971,7 @@ private:
     std::vector<DBuf<double>> ck_;  // (x, v, a) every ck_every_ steps
     Particles<double> R_;           // scratch state for segment recomputation
     DBuf<double> xs_;               // positions of one segment
+    DBuf<double> tape_;             // positions after steps 1 .. K of the recorded run's open segment
     DBuf<double> Xb_, Vb_, Ab_, At_, gm_, vpart_;
     DBuf<float4> Uf_, Ap_, Bp_;
     nbody::Plan vplan_;
