@@ -145,10 +145,10 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     # 16 B per cell; predict + deposit x, v, a, m in and x, v* out (64 B per
     # particle) and the int64 grid read and written (16 B per cell); fixed
     # point to source 8 B in, the source out, the int64 grid cleared (20 B
-    # per cell); gradient 4 B in, 16 B out (padded) per cell; gather + kick
-    # x, v, v* in, a, v out (60 B per particle) and the padded gradient grid
-    # read (16 B per cell).  The re-sort (every 48 steps) is not counted.
-    own_bytes = (fused * 20 * ncell + sched[1] * 16 * ncell + (16 + 20 + 20 + 16) * ncell + (64 + 60) * n)
+    # per cell); gather + kick (with -grad phi computed from phi on the way)
+    # x, v, v* in, a, v out (60 B per particle) and phi read (4 B per cell).
+    # The re-sort (every 48 steps) is not counted.
+    own_bytes = (fused * 20 * ncell + sched[1] * 16 * ncell + (16 + 20 + 4) * ncell + (64 + 60) * n)
     secs = t1 - t0
     out = {}
     if force_err:
@@ -173,7 +173,7 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
         "wave_avg_launch_us": avg_wave_s * 1e6,
         "wave_avg_launch_us_eager": wave_eager_ms * 1e3 / max(wave_eager_launches, 1),
         "stage_us_eager_16_after_sort": {k: v[0] * 1e3 / 3 for k, v in eager_tm.items()
-                                         if k in ("deposit", "source", "gradient", "gather") and v[1]},
+                                         if k in ("deposit", "source", "gather") and v[1]},
         "wave_ms_per_step": wave_ms / estep,
         "wave_us_per_iteration": wave_ms * 1e3 / (iters * estep),
         "wave_launches_per_step": {"four_iterations": sched[4], "three_iterations": sched[3],

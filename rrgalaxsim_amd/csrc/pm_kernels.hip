@@ -859,6 +859,154 @@ __global__ __launch_bounds__(256) void k_gather(PState<real> p, Geo<real> g, con
     }
 }
 
+// -grad phi of cell (ix, iy, iz) as calc_acceleration_field_ computes it
+// (rm_p2mesh.cu:238-290, and k_gradient above): one-sided differences on the
+// faces, half the central difference inside (through double, as the
+// reference's 0.5 literal promotes it), times coef = -1/dx.
+template <class real>
+__device__ __forceinline__ void cell_grad(const real* __restrict__ phi, int NX, int ix, int iy, int iz, real coef,
+                                          real g[3]) {
+    const size_t sy = size_t(NX), sz = size_t(NX) * NX;
+    const size_t c = size_t(ix) + iy * sy + iz * sz;
+    const real pc = phi[c];
+    real d[3];
+    if (ix == 0) d[0] = phi[c + 1] - pc;
+    else if (ix == NX - 1) d[0] = pc - phi[c - 1];
+    else d[0] = (real)(0.5 * (double)(phi[c + 1] - phi[c - 1]));
+    if (iy == 0) d[1] = phi[c + sy] - pc;
+    else if (iy == NX - 1) d[1] = pc - phi[c - sy];
+    else d[1] = (real)(0.5 * (double)(phi[c + sy] - phi[c - sy]));
+    if (iz == 0) d[2] = phi[c + sz] - pc;
+    else if (iz == NX - 1) d[2] = pc - phi[c - sz];
+    else d[2] = (real)(0.5 * (double)(phi[c + sz] - phi[c - sz]));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g[k] = d[k] * coef;
+}
+
+// The gather of k_gather with -grad phi computed from phi on the way, so a
+// step writes no gradient grid.  A workgroup of 256 particles (kept sorted by
+// cell) reduces the bounding box of the cells its particles interpolate from;
+// if the box holds at most kGatherBox cells, its threads compute each box
+// cell's -grad phi once into LDS and every particle reads its eight corners
+// from there; otherwise (sparse regions) each particle computes its corners'
+// -grad phi itself.  Either way each corner value is cell_grad's, the value
+// k_gradient writes, and the interpolation keeps k_gather's order, so the
+// result is bitwise that of k_gradient followed by k_gather.
+template <class real> constexpr int kGatherBox = 16384 / int(sizeof(typename Vec4<real>::T));
+
+template <class real, bool KICK>
+__global__ __launch_bounds__(256) void k_gather_phi(PState<real> p, Geo<real> g, const real* __restrict__ phi,
+                                                    real coef, real dt) {
+    using V4 = typename Vec4<real>::T;
+    __shared__ V4 box[kGatherBox<real>];
+    __shared__ int bnd[6];  // the workgroup's min and max corner index per axis
+    if (threadIdx.x < 6) bnd[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : int(0x80000000u);
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t n = p.n;
+    const int nx = g.nx;
+    const bool live = q < n;  // lanes past n reach every barrier and store nothing
+    const int64_t qq = live ? q : n - 1;
+    const real dx = g.dx, domain_min = g.domain_min;
+    const real px = p.x[qq], py = p.x[n + qq], pz = p.x[2 * n + qq];
+    // the kick's inputs are loaded before the first store (the particle
+    // arrays are not declared disjoint, so a later load would wait for it)
+    real v[3] = {0, 0, 0}, vv[3] = {0, 0, 0};
+    if (KICK) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] = p.v[c * n + qq];
+            vv[c] = p.w[c * n + qq];
+        }
+    }
+    // interpolate_acceleration_to_particles_kernel (rm_mesh2p.cu:60-111)
+    const real fx = (px - domain_min) / dx - (real)0.5f;
+    const real fy = (py - domain_min) / dx - (real)0.5f;
+    const real fz = (pz - domain_min) / dx - (real)0.5f;
+    int i0 = (int)floorf((float)fx);
+    int j0 = (int)floorf((float)fy);
+    int k0 = (int)floorf((float)fz);
+    int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
+    const real wx = fx - (real)i0, wy = fy - (real)j0, wz = fz - (real)k0;
+    i0 = max(0, min(i0, nx - 1));
+    i1 = max(0, min(i1, nx - 1));
+    j0 = max(0, min(j0, nx - 1));
+    j1 = max(0, min(j1, nx - 1));
+    k0 = max(0, min(k0, nx - 1));
+    k1 = max(0, min(k1, nx - 1));
+    {
+        const int lo[3] = {wave_ext<false>(i0), wave_ext<false>(j0), wave_ext<false>(k0)};
+        const int hi[3] = {wave_ext<true>(i1), wave_ext<true>(j1), wave_ext<true>(k1)};
+        __syncthreads();  // bounds initialised
+        if (lane == 0) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(&bnd[d], lo[d]);
+                atomicMax(&bnd[3 + d], hi[d]);
+            }
+        }
+        __syncthreads();  // bounds final
+    }
+    const int bx = bnd[0], by = bnd[1], bz = bnd[2];
+    const int ex = bnd[3] - bx + 1, ey = bnd[4] - by + 1, ez = bnd[5] - bz + 1;
+    const bool in_box = ex * ey * ez <= kGatherBox<real>;  // uniform over the workgroup (every box cell is a grid cell)
+    if (in_box) {
+        const int exy = ex * ey, vol = exy * ez;
+        const float rxy = 1.f / float(exy), rx = 1.f / float(ex);
+        for (int e = threadIdx.x; e < vol; e += blockDim.x) {
+            const int lk = small_div(e, exy, rxy), r = e - lk * exy;
+            const int lj = small_div(r, ex, rx), li = r - lj * ex;
+            real gc[3];
+            cell_grad<real>(phi, nx, bx + li, by + lj, bz + lk, coef, gc);
+            V4 o;
+            o.x = gc[0];
+            o.y = gc[1];
+            o.z = gc[2];
+            o.w = 0;
+            box[e] = o;
+        }
+        __syncthreads();  // box filled
+    }
+    if (!live) return;
+    real ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int k = 0; k <= 1; k++) {
+        const int kk = (k == 0) ? k0 : k1;
+        const real wk = (k == 0) ? ((real)1.0f - wz) : wz;
+#pragma unroll
+        for (int j = 0; j <= 1; j++) {
+            const int jj = (j == 0) ? j0 : j1;
+            const real wj = (j == 0) ? ((real)1.0f - wy) : wy;
+#pragma unroll
+            for (int i = 0; i <= 1; i++) {
+                const int ii = (i == 0) ? i0 : i1;
+                const real wi = (i == 0) ? ((real)1.0f - wx) : wx;
+                const real weight = wi * wj * wk;
+                real f[3];
+                if (in_box) {
+                    const V4 o = box[(ii - bx) + ex * ((jj - by) + ey * (kk - bz))];
+                    f[0] = o.x;
+                    f[1] = o.y;
+                    f[2] = o.z;
+                } else {
+                    cell_grad<real>(phi, nx, ii, jj, kk, coef, f);
+                }
+                ax += weight * f[0];
+                ay += weight * f[1];
+                az += weight * f[2];
+            }
+        }
+    }
+    p.a[q] = ax;
+    p.a[n + q] = ay;
+    p.a[2 * n + q] = az;
+    if (KICK) {  // correct_step_ (nbody.cu:171-180)
+        const real a3[3] = {ax, ay, az};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p.v[c * n + q] = (real)(0.5 * (double)(v[c] + vv[c] + dt * a3[c]));
+    }
+}
+
 template <class real>
 __global__ __launch_bounds__(256) void k_ngp_phi(PState<real> p, Geo<real> g, const real* __restrict__ phi) {
     const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1145,6 +1293,15 @@ void gather(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real
     RRG_LAUNCH_CHECK();
 }
 
+template <class real>
+void gather_phi(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real* phi, real dt, bool kick) {
+    if (p.n == 0) return;
+    const real coef = (real)(-1. / (double)g.dx);  // rm_p2mesh.cu:284
+    if (kick) k_gather_phi<real, true><<<grid1d(p.n, 256), 256, 0, s>>>(p, g, phi, coef, dt);
+    else k_gather_phi<real, false><<<grid1d(p.n, 256), 256, 0, s>>>(p, g, phi, coef, dt);
+    RRG_LAUNCH_CHECK();
+}
+
 template <class real> void ngp_phi(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real* phi) {
     if (p.n == 0) return;
     k_ngp_phi<real><<<grid1d(p.n, 256), 256, 0, s>>>(p, g, phi);
@@ -1251,6 +1408,7 @@ size_t ngp_temp_bytes(int64_t n, int nx) {
     template void solve_fixup<real>(hipStream_t, const SolveCtl*, const PhiBufs<real>&, int, int, int64_t);     \
     template void gradient<real>(hipStream_t, int, real, const real*, real*);                                   \
     template void gather<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*, real, bool);    \
+    template void gather_phi<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*, real, bool);\
     template void ngp_phi<real>(hipStream_t, const PState<real>&, const Geo<real>&, const real*);               \
     template void max_diff<real>(hipStream_t, int64_t, const real*, const real*, void*);                       \
     template void ngp_bin<real>(hipStream_t, const PState<real>&, const Geo<real>&, NgpBins&);                   \

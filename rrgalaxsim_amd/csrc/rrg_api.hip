@@ -1105,7 +1105,6 @@ public:
         }
         int launches = 0;
         last_iters_ = solve(prm_.wave.setup_iterations, &launches);
-        pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
         grav_actual_ = false;
         acc_actual_ = false;
         precalc_done_ = true;
@@ -1237,7 +1236,7 @@ public:
 
     void sync() override {  // rrgsim_integration_rect.cuh:85-97
         if (!grav_actual_) pm::ngp_phi<real>(s_, P_.view(), geo_, cur());
-        if (!acc_actual_) pm::gather<real>(s_, P_.view(), geo_, gacc_.p, (real)0, false);
+        if (!acc_actual_) pm::gather_phi<real>(s_, P_.view(), geo_, cur(), (real)0, false);
         grav_actual_ = acc_actual_ = true;
     }
 
@@ -1302,6 +1301,8 @@ public:
         }
         if (phi) download_scalar<real>(s_, ncell_, cur(), phi);
         if (acc) {
+            // steps keep no gradient grid: -grad phi of the current potential
+            pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
             std::vector<real> tmp(4 * ncell_);
             RRG_HIP(hipMemcpyAsync(tmp.data(), gacc_.p, sizeof(real) * 4 * ncell_, hipMemcpyDeviceToHost, s_));
             RRG_HIP(hipStreamSynchronize(s_));
@@ -1360,8 +1361,7 @@ private:
                                           nullptr, src_.p, layered(), occ_.p, src_empty_.p);
                 int ip = ip0, ic = ic0;  // from the arrangement before the rotation
                 solve_launches(ip, ic, true);
-                pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p);
-                pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true);
+                pm::gather_phi<real>(s_, P_.view(), geo_, cur(), dt, true);
             });
             last_iters_ = iters;
             iters_on_device_ = dev_check;
@@ -1395,9 +1395,9 @@ private:
             last_iters_ = solve(iters, &launches);
             timer.stop(s_, t, launches);
         }
+        // -grad phi is computed inside the gather (no gradient grid is written)
         const uint64_t b1 = graph_base();
-        segment(RRG_STAGE_GRADIENT, 1, graph, b1 + 3, [&] { pm::gradient<real>(s_, nx_, geo_.dx, cur(), gacc_.p); });
-        segment(RRG_STAGE_GATHER, 1, graph, b1 + 4, [&] { pm::gather<real>(s_, P_.view(), geo_, gacc_.p, dt, true); });
+        segment(RRG_STAGE_GATHER, 1, graph, b1 + 4, [&] { pm::gather_phi<real>(s_, P_.view(), geo_, cur(), dt, true); });
     }
 
     // iterations the next launch runs when `remaining` are left

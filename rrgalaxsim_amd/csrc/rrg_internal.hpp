@@ -395,6 +395,12 @@ template <class real> void gradient(hipStream_t s, int nx, real dx, const real* 
 template <class real>
 void gather(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real* acc4, real dt, bool kick);
 
+// gradient + gather in one launch: -grad phi is computed from phi at the
+// cells the particles interpolate from (bitwise the same accelerations), so
+// no gradient grid is written.
+template <class real>
+void gather_phi(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real* phi, real dt, bool kick);
+
 // NGP phi projection onto p.g (rm_p2mesh.cu:34-53, rm_mesh2p.cu:29-38).
 template <class real> void ngp_phi(hipStream_t s, const PState<real>& p, const Geo<real>& g, const real* phi);
 
