@@ -138,7 +138,7 @@ def test_multi_step_graphs_bitwise_equal_eager(rrg, ensemble):
 def test_pm_graph_replay_bitwise_equals_eager(rrg, nx, timing):
     """PM steps from graphs equal eager steps (64^3: fused stencil launches
     and four potential grids); one graph per step without stage timing, one
-    per stage (5) with it."""
+    per stage (4: deposit, source, wave, gather) with it."""
     pos, vel, m = ics.exp_disk(20_000, seed=1)
     wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=4574.337022617616, setup_iterations=30,
                 iterations=7, tolerance=0.0)
@@ -155,7 +155,7 @@ def test_pm_graph_replay_bitwise_equals_eager(rrg, nx, timing):
                 sim.conservation()
                 seq.append((sim.state(), sim.grid()["phi"]))
             res[graphs] = (seq, sim.stats()["graph_launches"])
-    assert res[True][1] == (90 if timing else 18)
+    assert res[True][1] == (72 if timing else 18)
     for (a, pa), (b, pb) in zip(res[False][0], res[True][0]):
         assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
         assert np.array_equal(pa, pb)
