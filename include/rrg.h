@@ -167,7 +167,7 @@ enum {
     RRG_STAGE_DEPOSIT = 0,  /* predict + CIC deposit (PM) */
     RRG_STAGE_SOURCE = 1,   /* fixed point -> mass, source term (PM) */
     RRG_STAGE_WAVE = 2,     /* wave_diss_iteration launches (PM) */
-    RRG_STAGE_GRADIENT = 3, /* -grad phi (PM) */
+    RRG_STAGE_GRADIENT = 3, /* -grad phi (PM; a step forms it inside the gather, so no launches) */
     RRG_STAGE_GATHER = 4,   /* trilinear gather + kick (PM) */
     RRG_STAGE_DIRECT = 5,   /* direct-sum force launches (N-body) */
     RRG_STAGE_PREDICT = 6,  /* predictor (N-body) */
