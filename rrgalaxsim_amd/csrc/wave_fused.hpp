@@ -138,6 +138,19 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     const int y = (Y0 - (K - 1) + w + NX) & (NX - 1);
     const int ym = (y - 1) & (NX - 1), yp = (y + 1) & (NX - 1);
     const bool out_row = w >= K - 1 && w < K - 1 + TY;  // rows Y0 .. Y0+TY-1
+    // Issue priority by the levels a row computes (row w computes levels 1 ..
+    // min(w, ROWS-1-w, K-1) + 1): the rows that compute every level are each
+    // barrier step's critical path, so the SIMD arbiter serves them first and
+    // the halo rows fill the gaps.  Graph chains on MI355X, per iteration:
+    // 128^3 FP32 K=4 2.87 -> 2.77 us, FP64 4.93 -> 4.79 us, 64^3 FP32 and
+    // FP64 K=3 -2% and -3%, 256^3 FP32 unchanged; with several waves per row
+    // (512^3 FP32 192 -> 200 us, 256^3 FP64 +1%) it loses, so those shapes
+    // keep one priority.  Results are unchanged either way.
+    if constexpr (RS == 1) {
+        const int lv = min(min(w, ROWS - 1 - w), K - 1);
+        if (lv == K - 1) __builtin_amdgcn_s_setprio(3);
+        else if (lv >= 1) __builtin_amdgcn_s_setprio(1);
+    }
     const bool edge = !YL || w == 0 || w == ROWS - 1;    // loads the input's y neighbours
     // XC: XJ consecutive cells per lane; otherwise XJ chunks of 64 lanes
     unsigned x[XJ], xm[XJ], xp[XJ];
