@@ -578,7 +578,7 @@ def _oracle():
     return oracle
 
 
-def cpu_baseline(ics, target_s=10.0, real="fp32"):
+def cpu_baseline(ics, target_s=10.0, real="fp32", one_core=False):
     """The CPU oracle on a bounded sample of the headline workload: the full
     1e6-particle / 128^3 problem, K PM steps (each with the 200 wave
     iterations; K sized for about `target_s` of CPU work), timed as
@@ -599,10 +599,19 @@ def cpu_baseline(ics, target_s=10.0, real="fp32"):
 
     k = int(min(100, max(2, target_s / max(per_step(2), 1e-3))))
     t = per_step(k)
-    return {"value": PM["n"] / t / 1e6, "unit": "Mparticle-steps/s", "cores": _cores(), "kind": "port",
-            "sample": f"oracle/rrg_oracle.cpp ({real}, OpenMP over cells and particles, deposit into per-thread "
-                      f"grids) on the full workload, {k} PM steps of 200 wave iterations ({k * t:.1f} s), minus "
-                      "precalc"}
+    out = {"value": PM["n"] / t / 1e6, "unit": "Mparticle-steps/s", "cores": _cores(), "kind": "port",
+           "sample": f"oracle/rrg_oracle.cpp ({real}, OpenMP over cells and particles, deposit into per-thread "
+                     f"grids) on the full workload, {k} PM steps of 200 wave iterations ({k * t:.1f} s), minus "
+                     "precalc"}
+    if one_core:
+        prev = oracle.set_threads(1)
+        try:
+            t1 = per_step(1)
+        finally:
+            oracle.set_threads(prev)
+        out["value_1_core"] = PM["n"] / t1 / 1e6
+        out["sample_1_core"] = f"the same on one thread, 1 PM step ({t1:.1f} s), minus precalc"
+    return out
 
 
 def cpu_baseline_stencil(nx, real, iters, dx=None):
@@ -892,7 +901,7 @@ def main():
             nx: {"fused_frac": ws[nx]["fused"]["frac"], "single_frac": ws[nx]["single"]["frac"],
                  "fused_us_per_iteration": ws[nx]["fused"]["us_per_iteration"]} for nx in ("256", "512") if nx in ws}
     if not args.no_cpu and "cpu" not in skip:
-        line["cpu_baseline"] = cpu_baseline(ics)
+        line["cpu_baseline"] = cpu_baseline(ics, one_core=True)
         per_leg = [("direct_sum", cpu_baseline_direct), ("direct_sum_c1", cpu_baseline_c1),
                    ("ensemble", cpu_baseline_ensemble), ("adjoint", cpu_baseline_adjoint),
                    ("adjoint_pm", cpu_baseline_adjoint_pm),

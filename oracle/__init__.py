@@ -75,6 +75,13 @@ def lib():
     return _lib
 
 
+def set_threads(n: int) -> int:
+    """Sets the OpenMP threads of the parallel routines; returns the previous count."""
+    prev = int(lib().orc_get_threads())
+    lib().orc_set_threads(_int(int(n)))
+    return prev
+
+
 def _suf(dtype) -> str:
     dtype = np.dtype(dtype)
     if dtype == np.float64:

@@ -1154,6 +1154,11 @@ int pm_adjoint(const PmConfig& cf, int64_t n, const double* x0, const double* v0
 // ---------------------------------------------------------------------------
 #define ORC_EXPORT extern "C" __attribute__((visibility("default")))
 
+// OpenMP threads of the parallel routines (the CPU baseline also times one
+// core); the default is the OpenMP runtime's.
+ORC_EXPORT void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+ORC_EXPORT int orc_get_threads() { return omp_get_max_threads(); }
+
 #define ORC_INSTANTIATE(SUF, real)                                                                   \
     ORC_EXPORT void orc_accel_##SUF(int64_t n, const real* pos, const real* m, const real* s2,     \
                                     real* acc) { accel<real>(n, pos, m, s2, acc); }                \
