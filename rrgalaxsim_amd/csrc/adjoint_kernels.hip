@@ -117,18 +117,22 @@ __global__ __launch_bounds__(TPB) void k_vjp_f64(int64_t n, const double* __rest
     }
 }
 
-// FP32 VJP on packed records A = (x_hi, m), B = (x_lo, h), Uf = (u, 0).
+// FP32 VJP on packed records A = (x_hi, m), B = (x_lo, h), Uf = (u, 0), with
+// the split of k_vjp_f64: per pair P += r^-3 u_j, S += m_j r^-3 and
+// Q += r^-5 (d.w) d in FP32 tile sums entering two-sum accumulators, and
+// m_k P - u_k S - 3 Q formed in FP64 at the end.
 template <int IB, bool HILO>
 __global__ __launch_bounds__(TPB) void k_vjp_f32(int64_t n, const float4* __restrict__ A, const float4* __restrict__ B,
                                                  const float4* __restrict__ Uf, int64_t slice_len,
                                                  double* __restrict__ out) {
+    constexpr int NC = 8;  // P (3), S, Q (3), mass gradient
     __shared__ float4 sa[TPB];
     __shared__ float4 sb[TPB];
     __shared__ float4 su[TPB];
     const int t = threadIdx.x;
     const int64_t base = int64_t(blockIdx.x) * TPB * IB;
     float xh[IB], yh[IB], zh[IB], xl[IB], yl[IB], zl[IB], mk[IB], hk[IB], uxk[IB], uyk[IB], uzk[IB];
-    float sg[4][IB], cg[4][IB];
+    float sg[NC][IB], cg[NC][IB];
 #pragma unroll
     for (int k = 0; k < IB; ++k) {
         const int64_t i = min(base + k * TPB + t, n - 1);
@@ -137,7 +141,7 @@ __global__ __launch_bounds__(TPB) void k_vjp_f32(int64_t n, const float4* __rest
         xl[k] = b.x; yl[k] = b.y; zl[k] = b.z; hk[k] = b.w;
         uxk[k] = u.x; uyk[k] = u.y; uzk[k] = u.z;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sg[c][k] = cg[c][k] = 0.f;
+        for (int c = 0; c < NC; ++c) sg[c][k] = cg[c][k] = 0.f;
     }
     const int64_t j0 = int64_t(blockIdx.y) * slice_len;
     const int64_t j1 = min(n, j0 + slice_len);
@@ -155,11 +159,11 @@ __global__ __launch_bounds__(TPB) void k_vjp_f32(int64_t n, const float4* __rest
         sb[t] = qb;
         su[t] = qu;
         __syncthreads();
-        float tg[4][IB];
+        float tg[NC][IB];
 #pragma unroll
         for (int k = 0; k < IB; ++k)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) tg[c][k] = 0.f;
+            for (int c = 0; c < NC; ++c) tg[c][k] = 0.f;
 #pragma unroll 2
         for (int jj = 0; jj < TPB; ++jj) {
             const float4 a = sa[jj], b = sb[jj], u = su[jj];
@@ -180,30 +184,37 @@ __global__ __launch_bounds__(TPB) void k_vjp_f32(int64_t n, const float4* __rest
                 const float inv2 = inv * inv;
                 const float inv3 = inv * inv2;
                 const float inv5 = inv3 * inv2;
-                const float wx = fmaf(mk[k], u.x, -a.w * uxk[k]);
-                const float wy = fmaf(mk[k], u.y, -a.w * uyk[k]);
-                const float wz = fmaf(mk[k], u.z, -a.w * uzk[k]);
-                const float dw = fmaf(dz, wz, fmaf(dy, wy, dx * wx));
-                const float sc = 3.f * inv5 * dw;
-                tg[0][k] = fmaf(inv3, wx, fmaf(-sc, dx, tg[0][k]));
-                tg[1][k] = fmaf(inv3, wy, fmaf(-sc, dy, tg[1][k]));
-                tg[2][k] = fmaf(inv3, wz, fmaf(-sc, dz, tg[2][k]));
-                const float ud = fmaf(dz, u.z, fmaf(dy, u.y, dx * u.x));
-                tg[3][k] = fmaf(-ud, inv3, tg[3][k]);
+                const float du = fmaf(dz, u.z, fmaf(dy, u.y, dx * u.x));            // d . u_j
+                const float dk = fmaf(dz, uzk[k], fmaf(dy, uyk[k], dx * uxk[k]));  // d . u_k
+                const float sc = inv5 * fmaf(mk[k], du, -a.w * dk);                // r^-5 (d . w)
+                tg[0][k] = fmaf(inv3, u.x, tg[0][k]);
+                tg[1][k] = fmaf(inv3, u.y, tg[1][k]);
+                tg[2][k] = fmaf(inv3, u.z, tg[2][k]);
+                tg[3][k] = fmaf(inv3, a.w, tg[3][k]);
+                tg[4][k] = fmaf(sc, dx, tg[4][k]);
+                tg[5][k] = fmaf(sc, dy, tg[5][k]);
+                tg[6][k] = fmaf(sc, dz, tg[6][k]);
+                tg[7][k] = fmaf(-du, inv3, tg[7][k]);
             }
         }
 #pragma unroll
         for (int k = 0; k < IB; ++k)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) two_sum_acc(sg[c][k], cg[c][k], tg[c][k]);
+            for (int c = 0; c < NC; ++c) two_sum_acc(sg[c][k], cg[c][k], tg[c][k]);
     }
     double* o = out + int64_t(blockIdx.y) * 4 * n;
 #pragma unroll
     for (int k = 0; k < IB; ++k) {
         const int64_t i = base + k * TPB + t;
         if (i < n) {
+            double v[NC];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) o[int64_t(c) * n + i] = double(sg[c][k]) + double(cg[c][k]);
+            for (int c = 0; c < NC; ++c) v[c] = double(sg[c][k]) + double(cg[c][k]);
+            const double m = double(mk[k]), ux = double(uxk[k]), uy = double(uyk[k]), uz = double(uzk[k]);
+            o[i] = fma(m, v[0], fma(-ux, v[3], -3.0 * v[4]));
+            o[n + i] = fma(m, v[1], fma(-uy, v[3], -3.0 * v[5]));
+            o[2 * n + i] = fma(m, v[2], fma(-uz, v[3], -3.0 * v[6]));
+            o[3 * n + i] = v[7];
         }
     }
 }
