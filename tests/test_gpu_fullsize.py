@@ -181,3 +181,57 @@ def test_cic_merge_paths_bitwise(rrg, orc, dtype):
     assert np.array_equal(rrg.cic_deposit(pos, m, nx, dx, e, dtype=dtype), ref)
     perm = np.random.default_rng(1).permutation(len(m))
     assert np.array_equal(rrg.cic_deposit(pos[perm], m[perm], nx, dx, e, dtype=dtype), ref)
+
+
+def _gather_boxes(pos, nx, dx, dtype, box):
+    """Workgroups of 256 particles (in cell order) whose gather corners fit a
+    box of at most `box` cells (the gather's LDS path) and the others (each
+    particle forms its corners' -grad phi itself); the corners as the gather
+    computes them ((x - min) / dx - 0.5, floored, each index clamped)."""
+    dt = dtype
+    dmin = dt(-0.5 * (dt(dx) * nx))
+    f = np.floor(((pos.astype(dt) - dmin) / dt(dx) - dt(0.5)).astype(np.float32)).astype(np.int64)
+    lo, hi = np.clip(f, 0, nx - 1), np.clip(f + 1, 0, nx - 1)
+    nbox = nfall = 0
+    for w0 in range(0, len(pos), 256):
+        vol = int(np.prod(hi[w0:w0 + 256].max(axis=0) - lo[w0:w0 + 256].min(axis=0) + 1))
+        if vol <= box:
+            nbox += 1
+        else:
+            nfall += 1
+    return nbox, nfall
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_gather_phi_paths_bitwise(rrg, orc, dtype):
+    """The steps' gather forms -grad phi itself: workgroups whose corner cells
+    fit the LDS box fill it once, the others form each particle's corners
+    directly.  Clusters that reach both paths (and particles beyond the box
+    faces): the accelerations after precalc + sync and the state after two
+    kicked steps are the oracle's (k_gradient followed by k_gather), bitwise."""
+    nx, dx = 128, 0.125
+    pos, m = _clusters(nx, dx, dtype, seed=7)
+    vel = np.zeros_like(pos)
+    key, _, _ = _cic_cells(pos, nx, dx, dtype)
+    order = np.argsort(key, kind="stable")  # the library's particle order after precalc's sort
+    box = 1024 if dtype == np.float32 else 512
+    nbox, nfall = _gather_boxes(pos[order], nx, dx, dtype, box)
+    print(f"gather {np.dtype(dtype).name}: workgroups on the LDS box path {nbox}, per-particle {nfall}", flush=True)
+    assert nbox > 100 and nfall > 20
+    real = "fp32" if dtype == np.float32 else "fp64"
+    wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=20, iterations=8, tolerance=0.0)
+    p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=dx, bc_frac=0.0), wave=wave, real=real)
+    with rrg.Simulation(p, pos.astype(np.float64), vel, m.astype(np.float64)) as sim:
+        sim.precalc()
+        sim.sync_particles_and_grid()
+        acc0 = sim.state()["acc"]
+        e = sim.stats()["fixed_exponent"]
+        sim.calc_step(2)
+        st = sim.state()
+    kw = dict(nx=nx, dx=dx, dt=5e-3, diss_base=0.785, wave_speed=C, setup_iterations=20, iterations=8,
+              tolerance=0.0, fixed_e=e, dtype=dtype)
+    o0 = orc.pm_run(pos, vel, m, 0, **kw)
+    assert np.array_equal(acc0.astype(dtype), o0["acc"])
+    o2 = orc.pm_run(pos, vel, m, 2, **kw)
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(st[k].astype(dtype), o2[k]), k
