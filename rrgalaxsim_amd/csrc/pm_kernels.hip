@@ -166,12 +166,37 @@ __device__ __forceinline__ int small_div(int e, int d, float rd) {
     return q;
 }
 
+// The bounding box of a 256-thread workgroup from each wave's (min, max) per
+// axis: lane 0 of every wave stores its six values in the wave's own LDS slot
+// (no atomics and no initialisation, so one barrier suffices), and after the
+// caller's barrier every thread reduces the four slots.
+__device__ __forceinline__ void box_slot_store(int (*slot)[6], const int lo[3], const int hi[3]) {
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            slot[w][d] = lo[d];
+            slot[w][3 + d] = hi[d];
+        }
+    }
+}
+__device__ __forceinline__ void box_slot_reduce(const int (*slot)[6], int b[6]) {
+#pragma unroll
+    for (int d = 0; d < 6; ++d) b[d] = slot[0][d];
+#pragma unroll
+    for (int w = 1; w < 4; ++w)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            b[d] = min(b[d], slot[w][d]);
+            b[3 + d] = max(b[3 + d], slot[w][3 + d]);
+        }
+}
+
 template <class real, bool FASTQ>
 __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt, int do_predict, Geo<real> g,
                                                          double scale, unsigned long long* __restrict__ mfix) {
     __shared__ unsigned long long box[kDepositBox];
-    __shared__ int bnd[6];  // the workgroup's min i0, j0, k0 and max i1, j1, k1
-    if (threadIdx.x < 6) bnd[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : -1;
+    __shared__ int slot[4][6];  // each wave's min i0, j0, k0 and max i1, j1, k1
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t n = p.n;
@@ -213,14 +238,7 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
         lo[d] = wave_ext<false>(base[d]);
         hi[d] = wave_ext<true>(i < n ? base[d] + step[d] : -1);
     }
-    __syncthreads();  // bounds initialised
-    if (lane == 0) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            atomicMin(&bnd[d], lo[d]);
-            atomicMax(&bnd[3 + d], hi[d]);
-        }
-    }
+    box_slot_store(slot, lo, hi);
     // run heads and tails within rows of 16 lanes
     const int rl = lane & 15;
     const unsigned kprev = (unsigned)row_mov<0x111>((int)key);
@@ -237,7 +255,9 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
             if (__ballot(tail && !f) != 0ull) seg_step<0x118>(q, f);
         }
     }
-    __syncthreads();  // bounds final
+    __syncthreads();  // the waves' bounds stored
+    int bnd[6];
+    box_slot_reduce(slot, bnd);
     const int bx = bnd[0], by = bnd[1], bz = bnd[2];
     const int ex = bnd[3] - bx + 1, ey = bnd[4] - by + 1, ez = bnd[5] - bz + 1;
     const bool in_box = bnd[3] >= 0 && ex * ey * ez <= kDepositBox;  // uniform over the workgroup
@@ -899,10 +919,8 @@ __global__ __launch_bounds__(256) void k_gather_phi(PState<real> p, Geo<real> g,
                                                     real coef, real dt) {
     using V4 = typename Vec4<real>::T;
     __shared__ V4 box[kGatherBox<real>];
-    __shared__ int bnd[6];  // the workgroup's min and max corner index per axis
-    if (threadIdx.x < 6) bnd[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : int(0x80000000u);
+    __shared__ int slot[4][6];  // each wave's min and max corner index per axis
     const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
     const int64_t n = p.n;
     const int nx = g.nx;
     const bool live = q < n;  // lanes past n reach every barrier and store nothing
@@ -934,18 +952,13 @@ __global__ __launch_bounds__(256) void k_gather_phi(PState<real> p, Geo<real> g,
     j1 = max(0, min(j1, nx - 1));
     k0 = max(0, min(k0, nx - 1));
     k1 = max(0, min(k1, nx - 1));
+    int bnd[6];
     {
         const int lo[3] = {wave_ext<false>(i0), wave_ext<false>(j0), wave_ext<false>(k0)};
         const int hi[3] = {wave_ext<true>(i1), wave_ext<true>(j1), wave_ext<true>(k1)};
-        __syncthreads();  // bounds initialised
-        if (lane == 0) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                atomicMin(&bnd[d], lo[d]);
-                atomicMax(&bnd[3 + d], hi[d]);
-            }
-        }
-        __syncthreads();  // bounds final
+        box_slot_store(slot, lo, hi);
+        __syncthreads();  // the waves' bounds stored
+        box_slot_reduce(slot, bnd);
     }
     const int bx = bnd[0], by = bnd[1], bz = bnd[2];
     const int ex = bnd[3] - bx + 1, ey = bnd[4] - by + 1, ez = bnd[5] - bz + 1;
@@ -1083,7 +1096,6 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t count, uns
 }
 
 inline unsigned grid1d(int64_t n, int tpb) { return unsigned((n + tpb - 1) / tpb); }
-inline dim3 grid3d(int nx) { return dim3(unsigned((nx + 63) / 64), unsigned((nx + 3) / 4), unsigned(nx)); }
 
 }  // namespace
 
