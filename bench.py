@@ -125,6 +125,7 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
         eager_tm = se.timing()
         wave_eager_ms, wave_eager_launches = eager_tm["wave"]
     ncell = PM["nx"] ** 3
+    rb = 4 if real == "fp32" else 8
     wave_ms, wave_launches = tm["wave"]
     avg_wave_s = wave_ms / 1e3 / max(wave_launches, 1)
     iters = PM["iterations"]
@@ -155,10 +156,10 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
         # PM force vs an FP64 direct sum over all particles (softening = dx):
         # the method error of the PM model (periodic source, cell-size
         # resolution)
-        idx, ref = _fp64_sample_forces(pos, m, np.full(n, PM["dx"] ** 2), nsample=256)
+        idx, ref = _fp64_sample_forces(pos, m, np.full(n, PM["dx"] ** 2), nsample=4096)
         err = np.linalg.norm(acc0[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
         out["pm_force_err_vs_fp64_direct"] = {"median": float(np.median(err)), "p99": float(np.percentile(err, 99)),
-                                              "sample": "256 particles (seed 5), t=0, direct sum softened by dx"}
+                                              "sample": "4096 particles (seed 5), t=0, direct sum softened by dx"}
     return out | {
         "value": n * steps / secs / 1e6,
         "ms_per_step": secs / steps * 1e3,
@@ -174,6 +175,18 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
         "wave_avg_launch_us_eager": wave_eager_ms * 1e3 / max(wave_eager_launches, 1),
         "stage_us_eager_16_after_sort": {k: v[0] * 1e3 / 3 for k, v in eager_tm.items()
                                          if k in ("deposit", "source", "gather") and v[1]},
+        # the other stages' compulsory bytes over their eager stage times (the
+        # event bracket includes the launch, so these are lower bounds)
+        "stage_rates_eager": {
+            k: {"compulsory_bytes": b, "gbs": b / (eager_tm[k][0] * 1e-3 / 3) / 1e9,
+                "frac": b / (eager_tm[k][0] * 1e-3 / 3) / 1e9 / HBM_PEAK_GBS, "per_unit": u}
+            for k, b, u in (("deposit", 16 * rb * n + 16 * ncell,
+                             f"{16 * rb} B per particle (x, v, a, m in; x, v* out) + 16 B per cell (int64 grid)"),
+                            ("source", (16 + rb) * ncell,
+                             f"{16 + rb} B per cell (int64 grid read and cleared, source written)"),
+                            ("gather", 15 * rb * n + rb * ncell,
+                             f"{15 * rb} B per particle (x, v, v* in; a, v out) + phi read once"))
+            if eager_tm.get(k, (0, 0))[1]},
         "wave_ms_per_step": wave_ms / estep,
         "wave_us_per_iteration": wave_ms * 1e3 / (iters * estep),
         "wave_launches_per_step": {"four_iterations": sched[4], "three_iterations": sched[3],
@@ -863,6 +876,7 @@ def main():
         "ms_per_step_with_stage_events": pm["ms_per_step_with_stage_events"],
         "stage_ms_per_step": pm["stage_ms_per_step"],
         "stage_us_eager_16_after_sort": pm["stage_us_eager_16_after_sort"],
+        "stage_rates_eager": pm["stage_rates_eager"],
         "stage_event_steps": pm["stage_event_steps"],
         "resort_every_steps": pm["resort_every_steps"],
         "resorts_in_stage_steps": pm["resorts_in_stage_steps"],
