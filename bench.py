@@ -294,11 +294,42 @@ def direct_bench(rrg, ics, steps=20):
     return out
 
 
+def _ensemble_force_err(pos, m, s2, off, acc, per_system=32):
+    """Median over each system's seeded sample of targets of |a - a_ref| / |a_ref|,
+    a_ref from a host FP64 direct sum over the system's own sources (the
+    reference kernel's formula); returns the worst system's median and the
+    largest error seen."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(9)
+    jobs = []
+    for g in range(len(off) - 1):
+        a, b = int(off[g]), int(off[g + 1])
+        for t in np.sort(rng.choice(b - a, min(per_system, b - a), replace=False)):
+            jobs.append((g, a, b, a + int(t)))
+
+    def one(job):
+        g, a, b, t = job
+        d = pos[a:b] - pos[t]
+        q = np.einsum("ij,ij->i", d, d) + 0.5 * (s2[t] + s2[a:b])
+        ref = (m[a:b] / (q * np.sqrt(q))) @ d
+        return g, float(np.linalg.norm(acc[t] - ref) / np.linalg.norm(ref))
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        res = list(ex.map(one, jobs))
+    per = {}
+    for g, e in res:
+        per.setdefault(g, []).append(e)
+    return max(float(np.median(v)) for v in per.values()), max(e for _, e in res)
+
+
 def ensemble_bench(rrg, ics, steps=100):
     """64 systems, N_g = round(2^U(12,16)) (seed 2), Plummer for even g and
-    exponential disk for odd g, eps = 0.01, dt = 1e-3, FP32 forces (hi/lo) on
-    FP64 state; one persistent work-queue launch per force evaluation,
-    hipGraph-replayed steps, autotuned launch shape."""
+    exponential disk for odd g, eps = 0.01, dt = 1e-3, FP32 pair forces on
+    FP64 state: the mixed kind (hi/lo positions) at the top level and the fp32
+    kind beside it; one persistent work-queue launch per force evaluation,
+    hipGraph-replayed steps, autotuned launch shape; force errors at t = 0
+    against a host FP64 direct sum per system."""
     sizes = ics.ensemble_sizes(64, seed=2)
     rng = np.random.default_rng(2)
     parts = [ics.plummer(int(nn), rng=rng) if g % 2 == 0 else ics.exp_disk(int(nn), rng=rng)
@@ -307,41 +338,54 @@ def ensemble_bench(rrg, ics, steps=100):
     pos = np.concatenate([q[0] for q in parts]); vel = np.concatenate([q[1] for q in parts])
     m = np.concatenate([q[2] for q in parts])
     s2 = np.full(len(m), 0.01 ** 2)
-    p = rrg.make_params(dt=1e-3, force="mixed", autotune=True)
-    sim = rrg.Simulation(p, pos, vel, m, s2, offsets=off)
-    sim.precalc()
-    sim.calc_step(9)  # captures the eight-step and the one-step graph
-    sim.synchronize()
-    g0 = sim.stats()["graph_launches"]
-    t0 = time.perf_counter()
-    sim.calc_step(steps)
-    sim.synchronize()
-    t1 = time.perf_counter()
-    stats = sim.stats()
-    # kernel time of the force launch, from events on the context stream
-    sim.set_timing(True)
-    sim.calc_step(1)  # captures the per-stage graphs
-    sim.set_timing(True)
-    sim.calc_step(3)
-    tm = sim.timing()
-    sim.close()
     inter = float(np.sum(sizes.astype(np.float64) ** 2))
-    kern_s = tm["direct"][0] / 1e3 / 3
-    return {
-        "ginteractions_per_s": inter * steps / (t1 - t0) / 1e9,
-        "kernel_ginteractions_per_s": inter / kern_s / 1e9,
-        "ms_per_step": (t1 - t0) / steps * 1e3,
-        "force_kernel_ms": kern_s * 1e3,
-        "fp32_vector_frac": 20 * inter / kern_s / 1e12 / FP32_PEAK_TFLOPS,
-        "kernel_launches_per_step": 3,
-        "graph_launches_per_step": (stats["graph_launches"] - g0) / steps,
-        "tuned_shape": {"ib": stats["tile_ib"], "sources_per_item": stats["slice_len"],
-                        "workgroups": stats["workgroups"], "items": stats["work_items"],
-                        "candidates": stats["tune_candidates"]},
-        "config": {"workload": "64 galaxies, N log-uniform in [2^12, 2^16] (seed 2), Plummer/disk alternating, "
-                               "eps=0.01, dt=1e-3, mixed FP32 forces / FP64 state", "systems": 64,
-                   "n_total": int(off[-1]), "sum_n2": inter, "timed_steps": steps},
-    }
+
+    def run(force):
+        p = rrg.make_params(dt=1e-3, force=force, autotune=True)
+        sim = rrg.Simulation(p, pos, vel, m, s2, offsets=off)
+        sim.precalc()
+        acc0 = sim.state()["acc"]
+        sim.calc_step(9)  # captures the eight-step and the one-step graph
+        sim.synchronize()
+        g0 = sim.stats()["graph_launches"]
+        t0 = time.perf_counter()
+        sim.calc_step(steps)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        stats = sim.stats()
+        # kernel time of the force launch, from events on the context stream
+        sim.set_timing(True)
+        sim.calc_step(1)  # captures the per-stage graphs
+        sim.set_timing(True)
+        sim.calc_step(3)
+        tm = sim.timing()
+        sim.close()
+        kern_s = tm["direct"][0] / 1e3 / 3
+        worst_median, worst = _ensemble_force_err(pos, m, s2, off, acc0)
+        return {
+            "ginteractions_per_s": inter * steps / (t1 - t0) / 1e9,
+            "kernel_ginteractions_per_s": inter / kern_s / 1e9,
+            "ms_per_step": (t1 - t0) / steps * 1e3,
+            "force_kernel_ms": kern_s * 1e3,
+            "fp32_vector_frac": 20 * inter / kern_s / 1e12 / FP32_PEAK_TFLOPS,
+            "kernel_launches_per_step": 3,
+            "graph_launches_per_step": (stats["graph_launches"] - g0) / steps,
+            "tuned_shape": {"ib": stats["tile_ib"], "sources_per_item": stats["slice_len"],
+                            "workgroups": stats["workgroups"], "items": stats["work_items"],
+                            "candidates": stats["tune_candidates"]},
+            "force_rel_err_worst_system_median": worst_median,
+            "force_rel_err_max": worst,
+        }
+
+    out = run("mixed")
+    out["fp32"] = run("fp32")
+    out["config"] = {"workload": "64 galaxies, N log-uniform in [2^12, 2^16] (seed 2), Plummer/disk alternating, "
+                                 "eps=0.01, dt=1e-3, FP32 pair forces / FP64 state (top level: the mixed kind; "
+                                 "`fp32`: plain FP32 positions)", "systems": 64,
+                     "n_total": int(off[-1]), "sum_n2": inter, "timed_steps": steps,
+                     "force_error": "|a - a_ref| / |a_ref| at t=0 on 32 sampled targets per system (seed 9); "
+                                    "a_ref = FP64 direct sum over the system's sources on the host (numpy)"}
+    return out
 
 
 def adjoint_bench(rrg, ics, n=65_536, steps=200, every=20):
