@@ -948,9 +948,6 @@ def main():
             d = line["direct_sum"]
             for k in ("mixed", "fp32"):
                 d[k]["fp32_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP32_PEAK_TFLOPS
-                # the FP32 kernels issue one FP32 operation per lane per instruction (packed FP32 is off),
-                # whose ceiling is half the published (packed) figure
-                d[k]["fp32_unpacked_frac"] = d[k]["tflops_20_per_interaction"] / (FP32_PEAK_TFLOPS / 2)
             for k in ("fp64", "fp64_exact"):
                 d[k]["fp64_vector_frac"] = d[k]["tflops_20_per_interaction"] / FP64_PEAK_TFLOPS
     if "wave_scaling" in line:
