@@ -247,12 +247,14 @@ __global__ __launch_bounds__(256) void k_predict_deposit(PState<real> p, real dt
     const bool tail = (rl == 15 || knext != key);
     // inclusive segmented scan within the row, until every tail holds its run
     // piece (a uniform exit)
-    seg_step<0x111>(q, f);
     if (__ballot(tail && !f) != 0ull) {
-        seg_step<0x112>(q, f);
+        seg_step<0x111>(q, f);
         if (__ballot(tail && !f) != 0ull) {
-            seg_step<0x114>(q, f);
-            if (__ballot(tail && !f) != 0ull) seg_step<0x118>(q, f);
+            seg_step<0x112>(q, f);
+            if (__ballot(tail && !f) != 0ull) {
+                seg_step<0x114>(q, f);
+                if (__ballot(tail && !f) != 0ull) seg_step<0x118>(q, f);
+            }
         }
     }
     __syncthreads();  // the waves' bounds stored
