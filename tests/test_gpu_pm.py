@@ -126,6 +126,19 @@ def test_pm_run_bitwise(rrg, orc, dtype):
     assert stats["last_wave_iterations"] == 20
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("nx,dx", [(17, 0.9), (33, 0.45)])
+def test_pm_run_odd_grid_bitwise(rrg, orc, dtype, nx, dx):
+    """Grids that are not a power of two (one stencil iteration per launch,
+    the deposit's and the gather's workgroup boxes over odd strides; the
+    disk reaches beyond the 17^3 box's faces)."""
+    _, st, gr, stats, o = _pm(rrg, orc, dtype, n=3000, nx=nx, dx=dx, steps=3, setup=12, iters=7)
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(st[k].astype(dtype), o[k]), k
+    assert np.array_equal(gr["phi"].astype(dtype), o["phi"])
+    assert np.array_equal(gr["acc"].astype(dtype), o["grid_acc"])
+
+
 def test_pm_steady_check_semantics(rrg, orc):
     """tolerance > 0: the early exit happens at the same iteration as the
     oracle's SteadyStateChecker and the fields stay bitwise equal."""
