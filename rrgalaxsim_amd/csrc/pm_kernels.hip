@@ -1266,14 +1266,21 @@ template <class real> int wave_fused_levels(const WaveModel<real>& m, int remain
     return fused_levels<real>(m.nx, m.bc, remaining);
 }
 
+template <class real> bool wave_fused_pairs(const WaveModel<real>& m, int levels) {
+    return fused_pairs<real>(m.nx, m.bc, levels);
+}
+
 template <class real>
 void wave_fused(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, int levels, real* qa, real* qb,
                 const real* p, const real* po, const real* src, const int* halt, const int* occ,
-                const real* src_empty) {
+                const real* src_empty, bool pin, bool pout) {
     if (levels < 2 || fused_levels<real>(m.nx, m.bc, levels) != levels)
         fail(RRG_ERR_INVALID, "wave_fused: no launch shape for this grid and iteration count");
     if ((occ == nullptr) != (src_empty == nullptr)) fail(RRG_ERR_INVALID, "wave_fused: occ and src_empty go together");
-    launch_fused<real, false, false>(s, m.nx, levels, qa, qb, p, po, src, nullptr, k, halt, occ, src_empty);
+    if ((pin || pout) && (!fused_pairs<real>(m.nx, m.bc, levels) || occ != nullptr))
+        fail(RRG_ERR_INVALID, "wave_fused: no pair-grid launch shape for this grid and iteration count");
+    launch_fused<real, false, false>(s, m.nx, levels, qa, pout ? qa : qb, p, pin ? p : po, src, nullptr, k, halt, occ,
+                                     src_empty, pin, pout);
     RRG_LAUNCH_CHECK();
 }
 
@@ -1415,8 +1422,10 @@ size_t ngp_temp_bytes(int64_t n, int nx) {
     template void wave_iteration<real>(hipStream_t, const WaveModel<real>&, const WaveK<real>&, real*,          \
                                        const real*, const real*, const real*, void*, const int*);               \
     template int wave_fused_levels<real>(const WaveModel<real>&, int);                                          \
+    template bool wave_fused_pairs<real>(const WaveModel<real>&, int);                                          \
     template void wave_fused<real>(hipStream_t, const WaveModel<real>&, const WaveK<real>&, int, real*, real*,  \
-                                   const real*, const real*, const real*, const int*, const int*, const real*); \
+                                   const real*, const real*, const real*, const int*, const int*, const real*,  \
+                                   bool, bool);                                                                 \
     template void solve_init<real>(hipStream_t, SolveCtl*, void*);                                              \
     template void solve_decide<real>(hipStream_t, SolveCtl*, void*, real, int, int, int, int);                  \
     template void solve_fixup<real>(hipStream_t, const SolveCtl*, const PhiBufs<real>&, int, int, int64_t);     \

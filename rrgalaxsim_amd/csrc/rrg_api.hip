@@ -1081,6 +1081,8 @@ public:
             src_empty_.alloc(1);
         }
         for (int i = 0; i < nbuf_; ++i) phi_[i].alloc(ncell_);
+        if (fused_ && pm::wave_fused_pairs<real>(model_, 4))
+            for (auto& q : pair_) q.alloc(2 * size_t(ncell_));
         gacc_.alloc(4 * ncell_);
         maxbits_.alloc(2 * pm::kCheckSlots);
         ctl_.alloc(1);
@@ -1406,8 +1408,17 @@ private:
     // `iters` stencil iterations without a check; returns the launches.
     // `halt`: the device stop flag of a solve checked on the device;
     // `launch` false: only the buffer rotation the launches would leave.
+    // Consecutive fused launches with a pair-grid shape hand their two levels
+    // over in pair_[0] / pair_[1] (alternately); the stretch's first launch
+    // reads and its last writes the split grids of the rotation.
     int wave_launches(int iters, int& ip, int& ic, const int* halt = nullptr, bool launch = true) {
         int launches = 0;
+        std::vector<int> seq;  // iterations of each launch
+        for (int r = iters; r > 0; r -= seq.back()) seq.push_back(levels(r));
+        auto paired = [&](size_t i) {  // launches i and i+1 exchange a pair grid
+            return pair_[0].p != nullptr && i + 1 < seq.size() && seq[i] == seq[i + 1] &&
+                   pm::wave_fused_pairs<real>(model_, seq[i]);
+        };
         pm::wave_schedule(
             iters, [&](int r) { return levels(r); }, ip, ic,
             [&](int out, int cur, int prev) {
@@ -1417,9 +1428,14 @@ private:
                 ++launches;
             },
             [&](int lv, int o1, int o2, int cur, int prev) {
-                if (launch)
-                    pm::wave_fused<real>(s_, model_, k_, lv, phi_[o1].p, phi_[o2].p, phi_[cur].p, phi_[prev].p,
-                                         src_.p, halt, occ_.p, src_empty_.p);
+                if (launch) {
+                    const size_t i = size_t(launches);
+                    const bool pin = i > 0 && paired(i - 1), pout = paired(i);
+                    real* qa = pout ? pair_[i & 1].p : phi_[o1].p;
+                    const real* p = pin ? pair_[(i - 1) & 1].p : phi_[cur].p;
+                    pm::wave_fused<real>(s_, model_, k_, lv, qa, phi_[o2].p, p, phi_[prev].p, src_.p, halt,
+                                         occ_.p, src_empty_.p, pin, pout);
+                }
                 ++launches;
             });
         return launches;
@@ -1697,6 +1713,7 @@ private:
     // source value of an empty cell; the stencil skips reading the others
     DBuf<int> occ_;
     DBuf<real> src_empty_;
+    DBuf<real> pair_[2];  // pair grids between consecutive fused launches (wave_fused_pairs shapes)
     bool fused_ = false;  // several iterations per launch
     int nbuf_ = 3;       // potential grids in the rotation
     DBuf<unsigned long long> maxbits_;

@@ -317,20 +317,25 @@ template <class real>
 void wave_iteration(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, real* pn, const real* p,
                     const real* po, const real* src, void* maxbits, const int* halt = nullptr);
 
-// `levels` (2 or 3) wave_diss_iterations in one launch, (p = phi^n,
+// `levels` (2 to 4) wave_diss_iterations in one launch, (p = phi^n,
 // po = phi^{n-1}) -> (qa = phi^{n+levels-1}, qb = phi^{n+levels}), bitwise
 // equal to that many wave_iteration calls; qa and qb must not alias p, po or
 // src.  wave_fused_levels(m, r) is the number of iterations the next launch
-// should run when r remain: 3 or 2 where a launch shape exists (periodic
+// should run when r remain: 4, 3 or 2 where a launch shape exists (periodic
 // grids of the sizes it lists), else 1 (one wave_iteration).
 template <class real> int wave_fused_levels(const WaveModel<real>& m, int remaining);
+// Whether launches of `levels` iterations can pass their two levels to the
+// next such launch as one interleaved pair grid of 2 nx^3 reals (phi^n and
+// phi^(n-1) of each two-cell block side by side): 128^3 FP32, four levels.
+template <class real> bool wave_fused_pairs(const WaveModel<real>& m, int levels);
 // With `occ` and `src_empty` from fixed_to_source (optional), the source
 // rows of unmarked segments are not read from src: their cells hold
-// *src_empty.
+// *src_empty.  pin: p is a pair grid (po unused); pout: the two output levels
+// go to the pair grid qa (qb unused); either needs wave_fused_pairs.
 template <class real>
 void wave_fused(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, int levels, real* qa, real* qb,
                 const real* p, const real* po, const real* src, const int* halt = nullptr, const int* occ = nullptr,
-                const real* src_empty = nullptr);
+                const real* src_empty = nullptr, bool pin = false, bool pout = false);
 
 // Steady-state check of a solve decided on the device (see pm_kernels.hip):
 // `done` stops the solve's later stencil launches (pass &ctl->done as their

@@ -48,8 +48,17 @@ namespace {
 // so they stay in flight across it.  HBM traffic per cell: 3 reals read and
 // 2 written for K forward iterations (4K for K single launches), 3 read and
 // 3 written for K adjoint iterations (5K), and one launch instead of K.
+//
+// Pair grids (PIN / POUT, forward FP32 with two consecutive cells per lane and
+// one wave per row): consecutive launches of a stretch pass (phi^n, phi^(n-1))
+// as one grid of 2 NX^3 reals in which every pair of cells (2b, 2b+1) of a row
+// is stored as [phi^n(2b), phi^n(2b+1), phi^(n-1)(2b), phi^(n-1)(2b+1)], row r
+// (r = y + NX z) starting at 2 NX r.  A lane then reads both levels of its two
+// cells with one 16-byte load and writes the outputs with one 16-byte store,
+// where the split grids take two of each.  PIN: p is such a grid (po unused);
+// POUT: qa receives one (qb unused).
 template <class real, int XJ> struct PlaneIn {
-    real a[XJ], b[XJ], c[XJ], d[XJ], po[XJ], src[XJ], pn[XJ], sb[XJ];
+    real a[XJ], b[XJ], c[XJ], d[XJ], po[XJ], src[XJ], pn[XJ], sb[XJ], pq[XJ];
 };
 
 // v of lane l-1 (CTRL 0x13C, wave_ror:1) or of lane l+1 (0x134, wave_rol:1),
@@ -104,7 +113,7 @@ __device__ __forceinline__ void row_neighbours(const real (&r)[XJ], real (&m)[XJ
 }
 
 template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST,
-          bool OCC = false>
+          bool OCC = false, bool PIN = false, bool POUT = false>
 __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real* __restrict__ qa,
                                                                            real* __restrict__ qb,
                                                                            const real* __restrict__ p,
@@ -125,6 +134,8 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     static_assert(RS * ROWS <= 16, "a workgroup holds at most 16 waves");
     static_assert(!XC || RS == 1 || YL, "segment ends of several waves per row come from the input plane in LDS");
     static_assert(!ADJ || K <= 3, "the adjoint's s-bar order is written for at most three levels");
+    static_assert(!(PIN || POUT) || (std::is_same_v<real, float> && XJ == 2 && RS == 1 && XC && !ADJ && !OCC),
+                  "pair grids: forward FP32 shapes with two consecutive cells per lane and one wave per row");
     __shared__ real lq[K - 1][2][ROWS][NX];
     __shared__ real l0[YL ? 2 : 1][YL ? ROWS : 1][NX];  // (YL) the input plane, for y neighbours
     int b = blockIdx.x;
@@ -204,17 +215,35 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
                 in.b[j] = r0[xp[j]];
             }
         }
-        if (edge) {
+        if constexpr (PIN) {
+            // phi^n of the y neighbours (the first half of each lane's block);
+            // both levels of plane z+2 (phi^(n-1) of it is used two steps on)
+            if (edge) {
+                const float2 cm = *reinterpret_cast<const float2*>(p + 2 * row(zq, ym) + 4 * l);
+                const float2 cp = *reinterpret_cast<const float2*>(p + 2 * row(zq, yp) + 4 * l);
+                in.c[0] = cm.x;
+                in.c[1] = cm.y;
+                in.d[0] = cp.x;
+                in.d[1] = cp.y;
+            }
+            const float4 q2 = *reinterpret_cast<const float4*>(p + 2 * row(zq + 2, y) + 4 * l);
+            in.pn[0] = q2.x;
+            in.pn[1] = q2.y;
+            in.pq[0] = q2.z;
+            in.pq[1] = q2.w;
+        } else {
+            if (edge) {
+#pragma unroll
+                for (int j = 0; j < XJ; ++j) {
+                    in.c[j] = rm[x[j]];
+                    in.d[j] = rp[x[j]];
+                }
+            }
 #pragma unroll
             for (int j = 0; j < XJ; ++j) {
-                in.c[j] = rm[x[j]];
-                in.d[j] = rp[x[j]];
+                in.po[j] = ro[x[j]];
+                in.pn[j] = r2[x[j]];
             }
-        }
-#pragma unroll
-        for (int j = 0; j < XJ; ++j) {
-            in.po[j] = ro[x[j]];
-            in.pn[j] = r2[x[j]];
         }
         if constexpr (OCC) {
             // a row without mass: lane 0 alone loads (its phi^(n-1) cell), so
@@ -241,16 +270,34 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     // the level-1 plane of step s-L+1)
     real Pv[4][XJ];
     real h1[K + 1][XJ], h2[K + 1][XJ], sh[K][XJ];
+    real pq[2][XJ];  // (PIN) phi^(n-1) of the level-1 planes of steps s and s+1, slot s & 1
     {
+        if constexpr (PIN) {
+            const float2 a = *reinterpret_cast<const float2*>(p + 2 * row(zs - 1, y) + 4 * l);
+            const float4 c = *reinterpret_cast<const float4*>(p + 2 * row(zs, y) + 4 * l);
+            const float4 d = *reinterpret_cast<const float4*>(p + 2 * row(zs + 1, y) + 4 * l);
+            Pv[2][0] = a.x;
+            Pv[2][1] = a.y;
+            Pv[1][0] = c.x;
+            Pv[1][1] = c.y;
+            pq[0][0] = c.z;
+            pq[0][1] = c.w;
+            Pv[0][0] = d.x;
+            Pv[0][1] = d.y;
+            pq[1][0] = d.z;
+            pq[1][1] = d.w;
+        }
         const real* __restrict__ a = p + row(zs - 1, y);
         const real* __restrict__ c = p + row(zs, y);
         const real* __restrict__ d = p + row(zs + 1, y);
 #pragma unroll
         for (int j = 0; j < XJ; ++j) {
             Pv[3][j] = 0;
-            Pv[2][j] = a[x[j]];
-            Pv[1][j] = c[x[j]];
-            Pv[0][j] = d[x[j]];
+            if constexpr (!PIN) {
+                Pv[2][j] = a[x[j]];
+                Pv[1][j] = c[x[j]];
+                Pv[0][j] = d[x[j]];
+            }
 #pragma unroll
             for (int L = 0; L <= K; ++L) h1[L][j] = h2[L][j] = 0;
 #pragma unroll
@@ -269,6 +316,13 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     for (int s = 0; s < S; ++s) {
         PlaneIn<real, XJ> in = ring[s % PD];
         if (s + PD < S) load(ring[s % PD], s + PD);
+        if constexpr (PIN) {
+#pragma unroll
+            for (int j = 0; j < XJ; ++j) {
+                in.po[j] = pq[s & 1][j];
+                pq[s & 1][j] = in.pq[j];
+            }
+        }
         if constexpr (YL) {
             if (!edge) {
 #pragma unroll
@@ -333,9 +387,12 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
                     // outputs written through L2 (read by the next launch) from
                     // 128^3 on
                     const size_t o = row(Z0 + s - 2 * (K - 1), y);  // wave-uniform
-                    const auto ra = through_rsrc(qa + o, NX * sizeof(real));
-                    const auto rb = through_rsrc(qb + o, NX * sizeof(real));
-                    if constexpr (NX < 128) {
+                    const auto ra = through_rsrc(qa + (POUT ? 2 : 1) * o, (POUT ? 2 : 1) * NX * sizeof(real));
+                    const auto rb = through_rsrc(qb + (POUT ? 0 : o), NX * sizeof(real));
+                    if constexpr (POUT) {
+                        const real v4[4] = {nw[K][0], nw[K][1], h1[K - 1][0], h1[K - 1][1]};
+                        store_through_run<real, 4>(ra, 16u * unsigned(l), v4);
+                    } else if constexpr (NX < 128) {
 #pragma unroll
                         for (int j = 0; j < XJ; ++j) {
                             qa[o + x[j]] = h1[K - 1][j];
@@ -396,7 +453,8 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     }
 }
 
-template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST>
+template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST,
+          bool PIN = false, bool POUT = false>
 void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* po, const real* src, real* sbar,
                   const WaveK<real>& k, const int* halt, const int* occ, const real* src_empty) {
     constexpr int NX = 64 * XJ * RS;
@@ -404,7 +462,11 @@ void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* 
     // source flags: the shapes that use them (FP64 from 128^3, FP32 from
     // 256^3, at most 64 planes marched); the others read every source row
     constexpr bool kOcc = !ADJ && ZB + 2 * K - 2 <= 64 && (NX >= 256 || (sizeof(real) == 8 && NX >= 128));
-    if (kOcc && occ != nullptr) {
+    if constexpr (PIN || POUT) {
+        static_assert(!kOcc, "pair grids: shapes without source flags");
+        k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST, false, PIN, POUT>
+            <<<grid, block, 0, s>>>(qa, qb, p, po, src, sbar, k, halt, nullptr, nullptr);
+    } else if (kOcc && occ != nullptr) {
         if constexpr (kOcc)
             k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST, true>
                 <<<grid, block, 0, s>>>(qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
@@ -430,6 +492,11 @@ void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* 
 // K=5 gains nothing over K=4 (128^3 FP32 3.25 us, 256^3 FP32 19.8 us).  The
 // adjoint uses the K<=3 shapes.  Other grids, and non-periodic ones, run one
 // iteration per launch.
+// The shapes with pair-grid variants (PIN / POUT): 128^3 FP32, four levels.
+template <class real> bool fused_pairs(int nx, int bc, int levels) {
+    return std::is_same_v<real, float> && bc == RRG_BC_PERIODIC && nx == 128 && levels == 4;
+}
+
 template <class real> int fused_max(int nx, int bc) {
     constexpr bool f32 = std::is_same_v<real, float>;
     if (bc != RRG_BC_PERIODIC) return 1;
@@ -450,18 +517,26 @@ template <class real> int fused_levels(int nx, int bc, int remaining, int cap = 
 }
 
 // One launch of `levels` iterations with the shape above; the caller checks
-// fused_levels(nx, periodic, levels) == levels.
+// fused_levels(nx, periodic, levels) == levels, and fused_pairs for pin
+// (p is a pair grid) and pout (qa receives one).
 template <class real, bool ADJ, bool FIRST>
 void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const real* p, const real* po,
                   const real* src, real* sbar, const WaveK<real>& k, const int* halt = nullptr,
-                  const int* occ = nullptr, const real* src_empty = nullptr) {
+                  const int* occ = nullptr, const real* src_empty = nullptr, bool pin = false, bool pout = false) {
     constexpr bool f32 = std::is_same_v<real, float>;
     if constexpr (!ADJ) {
         if (levels == 4) {
             if (nx == 128) {
-                if constexpr (f32)
-                    launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
-                else
+                if constexpr (f32) {
+                    if (pin && pout)
+                        launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST, true, true>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                    else if (pin)
+                        launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST, true, false>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                    else if (pout)
+                        launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST, false, true>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                    else
+                        launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                } else
                     launch_shape<real, 4, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
             } else if constexpr (f32) {
                 launch_shape<real, 4, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
