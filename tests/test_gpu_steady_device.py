@@ -29,10 +29,10 @@ CASES = [
 ]
 
 
-def _run(rrg, nx, dx, tol, real, boundary, graphs, timing=False):
+def _run(rrg, nx, dx, tol, real, boundary, graphs, timing=False, check=5):
     pos, vel, m = ics.exp_disk(20_000, seed=1)
     wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=400, iterations=60, tolerance=tol,
-                steady_check_step=5, steady_hold_steps=2)
+                steady_check_step=check, steady_hold_steps=2)
     p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=dx, bc_frac=0.1 if boundary != "periodic" else 0.0),
                         wave=wave, real=real, boundary=boundary, graphs=graphs)
     iters = []
@@ -79,6 +79,27 @@ def test_device_check_mixes_early_and_full_solves(rrg, orc):
                    dtype=np.float32)
     steps = [int(k) for k in o["iterations"][1:]]
     assert 60 in steps and any(k < 60 for k in steps), steps
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_device_check_pair_stretches(rrg, orc, graphs):
+    """Checks every 13 iterations on the 128^3 FP32 grid: each stretch between
+    checks is three four-iteration launches that pass (phi^n, phi^(n-1)) as
+    pair grids, and a converged solve's remaining stretches (pairs
+    included) return at once.  The oracle's counts mix full solves (60) with
+    solves that stop at 53 and 40 iterations; counts, particles and phi
+    bitwise."""
+    tol = 5e-5
+    (pos, vel, m), its, st, phi, stats = _run(rrg, 128, 0.125, tol, "fp32", "periodic", graphs=graphs, check=13)
+    o = orc.pm_run(pos, vel, m, STEPS, nx=128, dx=0.125, dt=5e-3, diss_base=0.785, diss_extra=0.01, wave_speed=C,
+                   setup_iterations=400, iterations=60, tolerance=tol, check_step=13, hold=2,
+                   fixed_e=stats["fixed_exponent"], dtype=np.float32)
+    steps = [int(k) for k in o["iterations"][1:]]
+    assert 60 in steps and 40 in steps, steps
+    assert its == [int(k) for k in o["iterations"]], (its, list(o["iterations"]))
+    for k in ("pos", "vel", "acc", "grav"):
+        assert np.array_equal(st[k].astype(np.float32), o[k]), k
+    assert np.array_equal(phi.astype(np.float32), o["phi"])
 
 
 def test_device_check_with_stage_graphs(rrg):
