@@ -468,49 +468,6 @@ __global__ __launch_bounds__(256) void k_mass_to_source(int64_t ncell, const rea
 // renumbered so that each of the 8 XCDs (which receive workgroups round-robin)
 // sweeps one contiguous slab of z-planes, keeping the +-z neighbour planes in
 // that XCD's L2.
-template <class real> struct MaxBits;
-template <> struct MaxBits<float> { using T = unsigned int; };
-template <> struct MaxBits<double> { using T = unsigned long long; };
-
-template <class real> __device__ __forceinline__ typename MaxBits<real>::T to_bits(real v) {
-    if constexpr (std::is_same_v<real, float>) return __float_as_uint(v);
-    else return (unsigned long long)__double_as_longlong(v);
-}
-
-template <class real> __device__ __forceinline__ real wave_max(real v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-// Max-reduction of a checking launch: the workgroup's maxima (waves, then
-// LDS) go to one of kCheckSlots slot pairs chosen by workgroup index, so the
-// atomics of a large grid spread over many addresses instead of queueing on
-// one; readers take the max over the slots.  Every thread of the workgroup
-// must call it.
-template <class real>
-__device__ __forceinline__ void check_accumulate(real dmax, real amax, typename MaxBits<real>::T* maxbits) {
-    __shared__ real sd[16], sa[16];
-    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-    const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
-    dmax = wave_max(dmax);
-    amax = wave_max(amax);
-    if ((t & 63) == 0) {
-        sd[t >> 6] = dmax;
-        sa[t >> 6] = amax;
-    }
-    __syncthreads();
-    if (t == 0) {
-        real d = sd[0], a = sa[0];
-        for (int w = 1; w < nw; ++w) {
-            d = fmax(d, sd[w]);
-            a = fmax(a, sa[w]);
-        }
-        const unsigned slot = (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % unsigned(kCheckSlots);
-        atomicMax(maxbits + slot, to_bits<real>(d));
-        atomicMax(maxbits + kCheckSlots + slot, to_bits<real>(a));
-    }
-}
-
 __device__ __forceinline__ void xcd_block(int nbx, int nby, int nbz, int& bx, int& by, int& bz) {
     const int nb = nbx * nby * nbz;
     const int b = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
@@ -1273,14 +1230,19 @@ template <class real> bool wave_fused_pairs(const WaveModel<real>& m, int levels
 template <class real>
 void wave_fused(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, int levels, real* qa, real* qb,
                 const real* p, const real* po, const real* src, const int* halt, const int* occ,
-                const real* src_empty, bool pin, bool pout) {
+                const real* src_empty, bool pin, bool pout, void* maxbits) {
     if (levels < 2 || fused_levels<real>(m.nx, m.bc, levels) != levels)
         fail(RRG_ERR_INVALID, "wave_fused: no launch shape for this grid and iteration count");
     if ((occ == nullptr) != (src_empty == nullptr)) fail(RRG_ERR_INVALID, "wave_fused: occ and src_empty go together");
     if ((pin || pout) && (!fused_pairs<real>(m.nx, m.bc, levels) || occ != nullptr))
         fail(RRG_ERR_INVALID, "wave_fused: no pair-grid launch shape for this grid and iteration count");
-    launch_fused<real, false, false>(s, m.nx, levels, qa, pout ? qa : qb, p, pin ? p : po, src, nullptr, k, halt, occ,
-                                     src_empty, pin, pout);
+    if (maxbits != nullptr && (pin || pout)) fail(RRG_ERR_INVALID, "wave_fused: a checking launch uses split grids");
+    if (maxbits != nullptr)
+        launch_fused<real, false, false, true>(s, m.nx, levels, qa, qb, p, po, src, nullptr, k, halt, occ, src_empty,
+                                               false, false, reinterpret_cast<typename MaxBits<real>::T*>(maxbits));
+    else
+        launch_fused<real, false, false>(s, m.nx, levels, qa, pout ? qa : qb, p, pin ? p : po, src, nullptr, k, halt,
+                                         occ, src_empty, pin, pout);
     RRG_LAUNCH_CHECK();
 }
 
@@ -1425,7 +1387,7 @@ size_t ngp_temp_bytes(int64_t n, int nx) {
     template bool wave_fused_pairs<real>(const WaveModel<real>&, int);                                          \
     template void wave_fused<real>(hipStream_t, const WaveModel<real>&, const WaveK<real>&, int, real*, real*,  \
                                    const real*, const real*, const real*, const int*, const int*, const real*,  \
-                                   bool, bool);                                                                 \
+                                   bool, bool, void*);                                                          \
     template void solve_init<real>(hipStream_t, SolveCtl*, void*);                                              \
     template void solve_decide<real>(hipStream_t, SolveCtl*, void*, real, int, int, int, int);                  \
     template void solve_fixup<real>(hipStream_t, const SolveCtl*, const PhiBufs<real>&, int, int, int64_t);     \

@@ -1410,21 +1410,23 @@ private:
     // `launch` false: only the buffer rotation the launches would leave.
     // Consecutive fused launches with a pair-grid shape hand their two levels
     // over in pair_[0] / pair_[1] (alternately); the stretch's first launch
-    // reads and its last writes the split grids of the rotation.
-    int wave_launches(int iters, int& ip, int& ic, const int* halt = nullptr, bool launch = true) {
+    // reads and its last writes the split grids of the rotation.  `check`
+    // (the maxbits pair): the last launch is a checking one, on split grids.
+    int wave_launches(int iters, int& ip, int& ic, const int* halt = nullptr, bool launch = true,
+                      void* check = nullptr) {
         int launches = 0;
         std::vector<int> seq;  // iterations of each launch
         for (int r = iters; r > 0; r -= seq.back()) seq.push_back(levels(r));
         auto paired = [&](size_t i) {  // launches i and i+1 exchange a pair grid
             return pair_[0].p != nullptr && i + 1 < seq.size() && seq[i] == seq[i + 1] &&
-                   pm::wave_fused_pairs<real>(model_, seq[i]);
+                   pm::wave_fused_pairs<real>(model_, seq[i]) && !(check != nullptr && i + 2 == seq.size());
         };
         pm::wave_schedule(
             iters, [&](int r) { return levels(r); }, ip, ic,
             [&](int out, int cur, int prev) {
                 if (launch)
-                    pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p, nullptr,
-                                             halt);
+                    pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p,
+                                             size_t(launches) + 1 == seq.size() ? check : nullptr, halt);
                 ++launches;
             },
             [&](int lv, int o1, int o2, int cur, int prev) {
@@ -1434,7 +1436,7 @@ private:
                     real* qa = pout ? pair_[i & 1].p : phi_[o1].p;
                     const real* p = pin ? pair_[(i - 1) & 1].p : phi_[cur].p;
                     pm::wave_fused<real>(s_, model_, k_, lv, qa, phi_[o2].p, p, phi_[prev].p, src_.p, halt,
-                                         occ_.p, src_empty_.p, pin, pout);
+                                         occ_.p, src_empty_.p, pin, pout, i + 1 == seq.size() ? check : nullptr);
                 }
                 ++launches;
             });
@@ -1442,13 +1444,16 @@ private:
     }
 
     // solve_wave_equation (wave.cu:230-297) as a fixed launch sequence with
-    // the steady check decided on the device: the stretches between checks
-    // as in wave_launches, each checking iteration (iter > 0,
-    // iter % steady_check_step == 0, after the rotation) one k_wave launch
-    // with the max-reduction followed by the hold rule; after convergence
-    // every launch returns at once and a last launch leaves phi^(n-1), phi^n
-    // where the full schedule ends (ip, ic).  Returns the launches; `launch`
-    // false: only the rotation.
+    // the steady check decided on the device: each stretch between checks and
+    // the checking iteration after it (iter > 0, iter % steady_check_step ==
+    // 0, after the rotation) run as in wave_launches, the checking iteration
+    // being the last level of the stretch's last launch, which reduces its
+    // maxima (a fused launch checks its last level against the level before,
+    // a single launch its one level against its input: the same values);
+    // the hold rule follows on the device.  After convergence every launch
+    // returns at once and a last launch leaves phi^(n-1), phi^n where the
+    // full schedule ends (ip, ic).  Returns the launches; `launch` false:
+    // only the rotation.
     int wave_solve_device(int iters, int& ip, int& ic, bool launch) {
         const int cs = prm_.wave.steady_check_step;
         const int* halt = reinterpret_cast<const int*>(reinterpret_cast<const char*>(ctl_.p) +
@@ -1457,25 +1462,16 @@ private:
         if (launch) pm::solve_init<real>(s_, ctl_.p, maxbits_.p);
         ++launches;
         while (iter < iters) {
-            int c = ((iter + cs - 1) / cs) * cs;
+            int c = ((iter + cs - 1) / cs) * cs;  // the next check follows iteration c
             if (c == 0) c = cs;
-            c = std::min(c, iters);
-            if (c > iter) {
-                launches += wave_launches(c - iter, ip, ic, halt, launch);
-                iter = c;
-                continue;
+            if (c >= iters) {  // no check before the limit
+                launches += wave_launches(iters - iter, ip, ic, halt, launch);
+                break;
             }
-            pm::wave_schedule(
-                1, [](int) { return 1; }, ip, ic,
-                [&](int out, int cur, int prev) {
-                    if (launch)
-                        pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p,
-                                                 maxbits_.p, halt);
-                },
-                [](int, int, int, int, int) {});
-            ++iter;
+            launches += wave_launches(c - iter + 1, ip, ic, halt, launch, maxbits_.p);
+            iter = c + 1;
             if (launch) pm::solve_decide<real>(s_, ctl_.p, maxbits_.p, tol_, prm_.wave.steady_hold_steps, iter, ip, ic);
-            launches += 2;
+            ++launches;
         }
         if (launch) {
             pm::PhiBufs<real> b{};

@@ -332,10 +332,13 @@ template <class real> bool wave_fused_pairs(const WaveModel<real>& m, int levels
 // rows of unmarked segments are not read from src: their cells hold
 // *src_empty.  pin: p is a pair grid (po unused); pout: the two output levels
 // go to the pair grid qa (qb unused); either needs wave_fused_pairs.
+// maxbits (split grids only): a checking launch, which reduces
+// max|phi^(n+levels) - phi^(n+levels-1)| and max|phi^(n+levels)| into the
+// slots of maxbits like wave_iteration's check of its one level.
 template <class real>
 void wave_fused(hipStream_t s, const WaveModel<real>& m, const WaveK<real>& k, int levels, real* qa, real* qb,
                 const real* p, const real* po, const real* src, const int* halt = nullptr, const int* occ = nullptr,
-                const real* src_empty = nullptr, bool pin = false, bool pout = false);
+                const real* src_empty = nullptr, bool pin = false, bool pout = false, void* maxbits = nullptr);
 
 // Steady-state check of a solve decided on the device (see pm_kernels.hip):
 // `done` stops the solve's later stencil launches (pass &ctl->done as their

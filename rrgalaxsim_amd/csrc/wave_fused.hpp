@@ -13,6 +13,51 @@ namespace rrg {
 namespace pm {
 namespace {
 
+// Bit patterns of non-negative reals, whose unsigned order is their order
+// (the maxima of a checking launch are reduced with integer atomics).
+template <class real> struct MaxBits;
+template <> struct MaxBits<float> { using T = unsigned int; };
+template <> struct MaxBits<double> { using T = unsigned long long; };
+
+template <class real> __device__ __forceinline__ typename MaxBits<real>::T to_bits(real v) {
+    if constexpr (std::is_same_v<real, float>) return __float_as_uint(v);
+    else return (unsigned long long)__double_as_longlong(v);
+}
+
+template <class real> __device__ __forceinline__ real wave_max(real v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Max-reduction of a checking launch: the workgroup's maxima (waves, then
+// LDS) go to one of kCheckSlots slot pairs chosen by workgroup index, so the
+// atomics of a large grid spread over many addresses instead of queueing on
+// one; readers take the max over the slots.  Every thread of the workgroup
+// must call it.
+template <class real>
+__device__ __forceinline__ void check_accumulate(real dmax, real amax, typename MaxBits<real>::T* maxbits) {
+    __shared__ real sd[16], sa[16];
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
+    dmax = wave_max(dmax);
+    amax = wave_max(amax);
+    if ((t & 63) == 0) {
+        sd[t >> 6] = dmax;
+        sa[t >> 6] = amax;
+    }
+    __syncthreads();
+    if (t == 0) {
+        real d = sd[0], a = sa[0];
+        for (int w = 1; w < nw; ++w) {
+            d = fmax(d, sd[w]);
+            a = fmax(a, sa[w]);
+        }
+        const unsigned slot = (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % unsigned(kCheckSlots);
+        atomicMax(maxbits + slot, to_bits<real>(d));
+        atomicMax(maxbits + kCheckSlots + slot, to_bits<real>(a));
+    }
+}
+
 // K iterations per launch (K = 2, 3, 4; the adjoint at most 3) on a periodic
 // NX^3 grid, NX = 64 XJ RS.
 //
@@ -113,7 +158,7 @@ __device__ __forceinline__ void row_neighbours(const real (&r)[XJ], real (&m)[XJ
 }
 
 template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST,
-          bool OCC = false, bool PIN = false, bool POUT = false>
+          bool OCC = false, bool PIN = false, bool POUT = false, bool CHECK = false>
 __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real* __restrict__ qa,
                                                                            real* __restrict__ qb,
                                                                            const real* __restrict__ p,
@@ -122,7 +167,8 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
                                                                            real* __restrict__ sbar, WaveK<real> k,
                                                                            const int* __restrict__ halt,
                                                                            const int* __restrict__ occ,
-                                                                           const real* __restrict__ src_empty) {
+                                                                           const real* __restrict__ src_empty,
+                                                                           typename MaxBits<real>::T* __restrict__ maxbits) {
     // a converged solve's remaining launches return at once (uniform, before
     // any barrier)
     if (halt != nullptr && *halt != 0) return;
@@ -136,6 +182,7 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     static_assert(!ADJ || K <= 3, "the adjoint's s-bar order is written for at most three levels");
     static_assert(!(PIN || POUT) || (std::is_same_v<real, float> && XJ == 2 && RS == 1 && XC && !ADJ && !OCC),
                   "pair grids: forward FP32 shapes with two consecutive cells per lane and one wave per row");
+    static_assert(!CHECK || (!ADJ && !PIN && !POUT), "checking launches: forward, split grids");
     __shared__ real lq[K - 1][2][ROWS][NX];
     __shared__ real l0[YL ? 2 : 1][YL ? ROWS : 1][NX];  // (YL) the input plane, for y neighbours
     int b = blockIdx.x;
@@ -271,6 +318,9 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
     real Pv[4][XJ];
     real h1[K + 1][XJ], h2[K + 1][XJ], sh[K][XJ];
     real pq[2][XJ];  // (PIN) phi^(n-1) of the level-1 planes of steps s and s+1, slot s & 1
+    // (CHECK) max|phi^(n+K) - phi^(n+K-1)| and max|phi^(n+K)| over the cells
+    // this workgroup writes: k_wave's check of the last level
+    real dmax = 0, amax = 0;
     {
         if constexpr (PIN) {
             const float2 a = *reinterpret_cast<const float2*>(p + 2 * row(zs - 1, y) + 4 * l);
@@ -389,6 +439,13 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
                     const size_t o = row(Z0 + s - 2 * (K - 1), y);  // wave-uniform
                     const auto ra = through_rsrc(qa + (POUT ? 2 : 1) * o, (POUT ? 2 : 1) * NX * sizeof(real));
                     const auto rb = through_rsrc(qb + (POUT ? 0 : o), NX * sizeof(real));
+                    if constexpr (CHECK) {
+#pragma unroll
+                        for (int j = 0; j < XJ; ++j) {
+                            dmax = fmax(dmax, fabs(nw[K][j] - h1[K - 1][j]));
+                            amax = fmax(amax, fabs(nw[K][j]));
+                        }
+                    }
                     if constexpr (POUT) {
                         const real v4[4] = {nw[K][0], nw[K][1], h1[K - 1][0], h1[K - 1][1]};
                         store_through_run<real, 4>(ra, 16u * unsigned(l), v4);
@@ -451,12 +508,14 @@ __global__ __launch_bounds__(64 * RS * (TY + 2 * K - 2)) void k_wave_fused(real*
         // ring slot s&1 is rewritten at s+2, after every wave passed this barrier
         if (s + 1 < S) __syncthreads();
     }
+    if constexpr (CHECK) check_accumulate<real>(dmax, amax, maxbits);
 }
 
 template <class real, int K, int XJ, int RS, int TY, int ZB, int PD, bool YL, bool XC, bool ADJ, bool FIRST,
-          bool PIN = false, bool POUT = false>
+          bool PIN = false, bool POUT = false, bool CHECK = false>
 void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* po, const real* src, real* sbar,
-                  const WaveK<real>& k, const int* halt, const int* occ, const real* src_empty) {
+                  const WaveK<real>& k, const int* halt, const int* occ, const real* src_empty,
+                  typename MaxBits<real>::T* maxbits = nullptr) {
     constexpr int NX = 64 * XJ * RS;
     constexpr dim3 grid((NX / TY) * (NX / ZB)), block(64 * RS * (TY + 2 * K - 2));
     // source flags: the shapes that use them (FP64 from 128^3, FP32 from
@@ -465,14 +524,14 @@ void launch_shape(hipStream_t s, real* qa, real* qb, const real* p, const real* 
     if constexpr (PIN || POUT) {
         static_assert(!kOcc, "pair grids: shapes without source flags");
         k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST, false, PIN, POUT>
-            <<<grid, block, 0, s>>>(qa, qb, p, po, src, sbar, k, halt, nullptr, nullptr);
+            <<<grid, block, 0, s>>>(qa, qb, p, po, src, sbar, k, halt, nullptr, nullptr, nullptr);
     } else if (kOcc && occ != nullptr) {
         if constexpr (kOcc)
-            k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST, true>
-                <<<grid, block, 0, s>>>(qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+            k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST, true, false, false, CHECK>
+                <<<grid, block, 0, s>>>(qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
     } else {
-        k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST, false>
-            <<<grid, block, 0, s>>>(qa, qb, p, po, src, sbar, k, halt, nullptr, nullptr);
+        k_wave_fused<real, K, XJ, RS, TY, ZB, PD, YL, XC, ADJ, FIRST, false, false, false, CHECK>
+            <<<grid, block, 0, s>>>(qa, qb, p, po, src, sbar, k, halt, nullptr, nullptr, maxbits);
     }
 }
 
@@ -518,11 +577,13 @@ template <class real> int fused_levels(int nx, int bc, int remaining, int cap = 
 
 // One launch of `levels` iterations with the shape above; the caller checks
 // fused_levels(nx, periodic, levels) == levels, and fused_pairs for pin
-// (p is a pair grid) and pout (qa receives one).
-template <class real, bool ADJ, bool FIRST>
+// (p is a pair grid) and pout (qa receives one).  CHECK: a checking launch
+// (split grids), whose last level's maxima go to maxbits.
+template <class real, bool ADJ, bool FIRST, bool CHECK = false>
 void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const real* p, const real* po,
                   const real* src, real* sbar, const WaveK<real>& k, const int* halt = nullptr,
-                  const int* occ = nullptr, const real* src_empty = nullptr, bool pin = false, bool pout = false) {
+                  const int* occ = nullptr, const real* src_empty = nullptr, bool pin = false, bool pout = false,
+                  typename MaxBits<real>::T* maxbits = nullptr) {
     constexpr bool f32 = std::is_same_v<real, float>;
     if constexpr (!ADJ) {
         if (levels == 4) {
@@ -535,49 +596,49 @@ void launch_fused(hipStream_t s, int nx, int levels, real* qa, real* qb, const r
                     else if (pout)
                         launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST, false, true>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
                     else
-                        launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                        launch_shape<real, 4, 2, 1, 8, 8, 2, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
                 } else
-                    launch_shape<real, 4, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                    launch_shape<real, 4, 2, 1, 8, 8, 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
             } else if constexpr (f32) {
-                launch_shape<real, 4, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 4, 4, 1, 8, 32, 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
             }
             return;
         }
     }
     if (levels == 3) {
         if (nx == 64) {
-            launch_shape<real, 3, 1, 1, 4, 4, 1, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+            launch_shape<real, 3, 1, 1, 4, 4, 1, false, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
         } else if (nx == 128) {
-            launch_shape<real, 3, 2, 1, 8, 8, f32 ? 2 : 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+            launch_shape<real, 3, 2, 1, 8, 8, f32 ? 2 : 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
         } else if (nx == 256) {
             if constexpr (f32)
-                launch_shape<real, 3, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 3, 4, 1, 8, 32, 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
             else
-                launch_shape<real, 3, 2, 2, 4, 64, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 3, 2, 2, 4, 64, 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
         } else if constexpr (f32) {
-            launch_shape<real, 3, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+            launch_shape<real, 3, 4, 2, 4, 32, 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
         }
     } else {
         if (nx == 64) {
             if constexpr (f32)
-                launch_shape<real, 2, 1, 1, 4, 4, 2, false, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 2, 1, 1, 4, 4, 2, false, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
             else
-                launch_shape<real, 2, 1, 1, 4, 4, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 2, 1, 1, 4, 4, 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
         } else if (nx == 128) {
             if constexpr (f32)
-                launch_shape<real, 2, 1, 2, 4, 8, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 2, 1, 2, 4, 8, 1, false, false, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
             else
-                launch_shape<real, 2, 2, 1, 8, 8, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 2, 2, 1, 8, 8, 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
         } else if (nx == 256) {
             if constexpr (f32)
-                launch_shape<real, 2, 4, 1, 8, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 2, 4, 1, 8, 32, 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
             else
-                launch_shape<real, 2, 2, 2, 4, 32, 1, false, false, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 2, 2, 2, 4, 32, 1, false, false, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
         } else {
             if constexpr (f32)
-                launch_shape<real, 2, 8, 1, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 2, 8, 1, 4, 32, 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
             else
-                launch_shape<real, 2, 4, 2, 4, 32, 1, true, true, ADJ, FIRST>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty);
+                launch_shape<real, 2, 4, 2, 4, 32, 1, true, true, ADJ, FIRST, false, false, CHECK>(s, qa, qb, p, po, src, sbar, k, halt, occ, src_empty, maxbits);
         }
     }
 }
