@@ -29,10 +29,10 @@ CASES = [
 ]
 
 
-def _run(rrg, nx, dx, tol, real, boundary, graphs, timing=False, check=5):
+def _run(rrg, nx, dx, tol, real, boundary, graphs, timing=False, check=5, setup=400, iterations=60, steps=STEPS):
     pos, vel, m = ics.exp_disk(20_000, seed=1)
-    wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=400, iterations=60, tolerance=tol,
-                steady_check_step=check, steady_hold_steps=2)
+    wave = dict(diss_base=0.785, diss_extra=0.01, wave_speed=C, setup_iterations=setup, iterations=iterations,
+                tolerance=tol, steady_check_step=check, steady_hold_steps=2)
     p = rrg.make_params(dt=5e-3, grid=dict(nx=nx, dx=dx, bc_frac=0.1 if boundary != "periodic" else 0.0),
                         wave=wave, real=real, boundary=boundary, graphs=graphs)
     iters = []
@@ -41,7 +41,7 @@ def _run(rrg, nx, dx, tol, real, boundary, graphs, timing=False, check=5):
         iters.append(sim.stats()["last_wave_iterations"])
         sim.sync_particles_and_grid()
         sim.set_timing(timing)
-        for _ in range(STEPS):
+        for _ in range(steps):
             sim.calc_step(1)
             iters.append(sim.stats()["last_wave_iterations"])
         sim.sync_particles_and_grid()
@@ -100,6 +100,29 @@ def test_device_check_pair_stretches(rrg, orc, graphs):
     for k in ("pos", "vel", "acc", "grav"):
         assert np.array_equal(st[k].astype(np.float32), o[k]), k
     assert np.array_equal(phi.astype(np.float32), o["phi"])
+
+
+@pytest.mark.parametrize("nx,check", [(256, 10), (512, 13)])
+def test_device_check_large_fp64(rrg, orc, nx, check):
+    """The FP64 shapes with several waves per row and source-row flags, whose
+    checking launches are fused: 256^3 (three iterations per launch) and
+    512^3 (two, as in the reference's sample workload), short solves (40
+    setup iterations, two steps of at most 40) at a tolerance the oracle's
+    solves meet after 21 (256^3) or 27 (512^3) iterations and the setup does
+    not, so the fused checks decide both ways; counts and fields bitwise."""
+    dx, tol = 16.0 / nx, 0.03
+    print(f"steady check on the device: nx={nx} fp64 check every {check}", flush=True)
+    (pos, vel, m), its, st, phi, stats = _run(rrg, nx, dx, tol, "fp64", "periodic", graphs=True, check=check,
+                                              setup=40, iterations=40, steps=2)
+    o = orc.pm_run(pos, vel, m, 2, nx=nx, dx=dx, dt=5e-3, diss_base=0.785, diss_extra=0.01, wave_speed=C,
+                   setup_iterations=40, iterations=40, tolerance=tol, check_step=check, hold=2,
+                   fixed_e=stats["fixed_exponent"], dtype=np.float64)
+    want = [int(k) for k in o["iterations"]]
+    assert want[0] == 40 and min(want[1:]) < 40, want
+    assert its == want, (its, want)
+    for k in ("pos", "vel", "acc", "grav"):
+        assert np.array_equal(st[k], o[k]), k
+    assert np.array_equal(phi, o["phi"])
 
 
 def test_device_check_with_stage_graphs(rrg):
