@@ -1647,23 +1647,17 @@ private:
             if (checking) {
                 c = ((iter + check_step - 1) / check_step) * check_step;
                 if (c == 0) c = check_step;
-                c = std::min(c, max_iter);
             }
-            if (c > iter) {
-                *launches += wave_launches(c - iter, ip_, ic_);
-                iter = c;
-                continue;
+            if (!checking || c >= max_iter) {  // no check before the limit
+                *launches += wave_launches(max_iter - iter, ip_, ic_);
+                iter = max_iter;
+                break;
             }
+            // the stretch and the checking iteration after it, the check
+            // reduced by the launch that completes that iteration
             pm::fill_u64(s_, maxbits_.p, 2 * pm::kCheckSlots, 0ull);
-            pm::wave_schedule(
-                1, [](int) { return 1; }, ip_, ic_,
-                [&](int out, int cur, int prev) {
-                    pm::wave_iteration<real>(s_, model_, k_, phi_[out].p, phi_[cur].p, phi_[prev].p, src_.p,
-                                             maxbits_.p);
-                },
-                [](int, int, int, int, int) {});
-            ++*launches;
-            ++iter;
+            *launches += wave_launches(c - iter + 1, ip_, ic_, nullptr, true, maxbits_.p);
+            iter = c + 1;
             host_.ensure(sizeof(MB) * 2 * pm::kCheckSlots);
             RRG_HIP(hipMemcpyAsync(host_.p, maxbits_.p, sizeof(MB) * 2 * pm::kCheckSlots, hipMemcpyDeviceToHost, s_));
             RRG_HIP(hipStreamSynchronize(s_));
