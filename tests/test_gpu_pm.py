@@ -139,6 +139,20 @@ def test_pm_run_odd_grid_bitwise(rrg, orc, dtype, nx, dx):
     assert np.array_equal(gr["acc"].astype(dtype), o["grid_acc"])
 
 
+@pytest.mark.parametrize("iters", [14, 203])
+def test_pm_pair_grid_schedules_bitwise(rrg, orc, iters):
+    """128^3 FP32, where consecutive four-iteration launches pass their two
+    levels as a pair grid: 14 iterations run as 4+4+3+3 (a pair hand-off into
+    a launch that ends the pairing), 203 as fifty four-iteration launches and
+    a three-iteration one; setup solves of 18 (4+4+4+3+3) and fields after
+    two steps bitwise."""
+    print(f"pair-grid schedule: 128^3 fp32, {iters} iterations per step", flush=True)
+    _, st, gr, stats, o = _pm(rrg, orc, np.float32, nx=128, dx=0.125, steps=2, setup=18, iters=iters)
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(st[k].astype(np.float32), o[k]), k
+    assert np.array_equal(gr["phi"].astype(np.float32), o["phi"])
+
+
 def test_pm_steady_check_semantics(rrg, orc):
     """tolerance > 0: the early exit happens at the same iteration as the
     oracle's SteadyStateChecker and the fields stay bitwise equal."""
