@@ -541,8 +541,9 @@ __global__ __launch_bounds__(256) void k_wave(real* __restrict__ pn, const real*
 // solve_wave_equation's SteadyStateChecker (wave.cu:253-287,
 // wave_steady_checker.cu:95-143) with its state in a device control block,
 // so a solve with checks is a fixed launch sequence that a graph can hold:
-// each checking launch reduces max|phi_new - phi| and max|phi_new| into the
-// maxbits pair, a one-thread launch then applies the hold rule; once the
+// each checking launch (a single iteration, or a fused launch for its last
+// level) reduces max|phi_new - phi| and max|phi_new| into the maxbits slots,
+// a one-workgroup launch then applies the hold rule; once the
 // hold is reached `done` is set and every later stencil launch of the solve
 // returns at once, and a last launch moves the two live levels into the
 // buffers the full schedule would have left them in.
