@@ -474,6 +474,33 @@ def wave_scaling(rrg, ics, sizes=(64, 128, 256, 512), steps=2, real="fp32"):
     return out
 
 
+def _sample_setup(rrg, ics):
+    """The sample run's particles and its params (a function of `graphs`)."""
+    pos, vel, m = ics.exp_disk(262_144, seed=6, r_max=8.0)
+    pos, vel = pos * 0.25, vel * 2.0
+    wave = dict(diss_base=0.1, diss_extra=0.01, wave_speed=C, setup_iterations=30000, iterations=3000,
+                tolerance=1e-6, steady_check_step=30, steady_hold_steps=3)
+    return (pos, vel, m), lambda graphs: rrg.make_params(dt=0.002, grid=dict(nx=512, dx=0.025, bc_frac=0.2),
+                                                         wave=wave, real="fp64", graphs=graphs)
+
+
+def profile_pass_sample(rrg, ics, steps):
+    """The sample run for a kernel trace: precalc and `steps` steps with every
+    kernel launched on its own and the check read back by the host after each
+    checking launch (the device-checked form replays graphs)."""
+    (pos, vel, m), params = _sample_setup(rrg, ics)
+    with rrg.Simulation(params(False), pos, vel, m) as sim:
+        sim.precalc()
+        setup_iters = sim.stats()["last_wave_iterations"]
+        sim.sync_particles_and_grid()
+        iters = []
+        for _ in range(steps):
+            sim.calc_step(1)
+            iters.append(sim.stats()["last_wave_iterations"])
+        sim.synchronize()
+    print(json.dumps({"profile_pass": "sample", "setup_iterations": setup_iters, "iterations_each": iters}), flush=True)
+
+
 def sample_bench(rrg, ics, steps=3):
     """The reference's sample run (sample/params.json): 512^3 cells of
     dx = 0.025 (box [-6.4, 6.4]^3), FP64 (the reference's default real),
@@ -486,15 +513,10 @@ def sample_bench(rrg, ics, steps=3):
     blocking read-back per check, as the reference does).  After one warm-up
     step, `steps` steps are timed one at a time, each with its iteration
     count."""
-    pos, vel, m = ics.exp_disk(262_144, seed=6, r_max=8.0)
-    pos, vel = pos * 0.25, vel * 2.0
-    wave = dict(diss_base=0.1, diss_extra=0.01, wave_speed=C, setup_iterations=30000, iterations=3000,
-                tolerance=1e-6, steady_check_step=30, steady_hold_steps=3)
+    (pos, vel, m), params = _sample_setup(rrg, ics)
     out = {}
     for mode, graphs in (("device_check", True), ("host_check", False)):
-        p = rrg.make_params(dt=0.002, grid=dict(nx=512, dx=0.025, bc_frac=0.2), wave=wave, real="fp64",
-                            graphs=graphs)
-        with rrg.Simulation(p, pos, vel, m) as sim:
+        with rrg.Simulation(params(graphs), pos, vel, m) as sim:
             t0 = time.perf_counter()
             sim.precalc()
             sim.synchronize()
@@ -834,9 +856,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-pass", action="store_true", help="short PM run + calibration copy for rocprofv3 --pmc")
     ap.add_argument("--eager", action="store_true", help="with --profile-pass: launch kernels eagerly, not from graphs")
-    ap.add_argument("--workload", default="pm", choices=("pm", "direct"),
-                    help="with --profile-pass: the headline PM run (pm) or the direct-sum kinds on the disk + bulge "
-                         "workload (direct)")
+    ap.add_argument("--workload", default="pm", choices=("pm", "direct", "sample"),
+                    help="with --profile-pass: the headline PM run (pm), the direct-sum kinds on the disk + bulge "
+                         "workload (direct) or the reference's sample run (sample, with --eager)")
     ap.add_argument("--skip", default="", help="comma-separated legs to leave out: direct, c1, scaling, ensemble, "
                                               "adjoint, adjoint_pm, pm_fp64, sample, cpu")
     args = ap.parse_args()
@@ -852,6 +874,10 @@ def main():
         rrg.lib()
         if args.workload == "direct":
             profile_pass_direct(rrg, ics, args.steps, graphs=not args.eager)
+        elif args.workload == "sample":
+            if not args.eager:
+                sys.exit("bench.py: the sample profile pass runs eagerly (--eager)")
+            profile_pass_sample(rrg, ics, args.steps)
         else:
             profile_pass(rrg, ics, args.steps, graphs=not args.eager)
         return
