@@ -363,6 +363,52 @@ __global__ void k_correct(PState<real> p, real dt, int slices, const unsigned ch
     }
 }
 
+// k_correct of one step followed by k_predict of the next in one pass over
+// the particles (FP64 state): the same values as the two launches, since
+// each particle's corrector and predictor touch only that particle.
+template <bool PACK>
+__global__ void k_correct_predict(PState<double> p, double dt, int slices, const unsigned char* __restrict__ ns,
+                                  const double* __restrict__ part, float4* __restrict__ A, float4* __restrict__ B) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const int64_t n = p.n;
+    const int si = slices_of(i, slices, ns);
+    // loads first (see k_predict)
+    double a[3], v[3], vv[3], r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (part) {
+            double s = part[c * n + i];
+            for (int q = 1; q < si; ++q) s += part[(int64_t(q) * 3 + c) * n + i];
+            a[c] = s;
+        } else {
+            a[c] = p.a[c * n + i];
+        }
+        v[c] = p.v[c * n + i];
+        vv[c] = p.w[c * n + i];
+        r[c] = p.x[c * n + i];
+    }
+    const double m = PACK ? p.m[i] : 0.0, s2 = PACK ? p.s2[i] : 0.0;
+    double xn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (part) p.a[c * n + i] = a[c];
+        const double vc = 0.5 * (v[c] + vv[c] + dt * a[c]);  // correct_step_
+        p.v[c * n + i] = vc;
+        const double vp = vc + dt * a[c];  // predict_step_ of the next step
+        p.w[c * n + i] = vp;
+        const double rn = r[c] + (dt * 0.5) * (vc + vp);
+        p.x[c * n + i] = rn;
+        xn[c] = rn;
+    }
+    if constexpr (PACK) {
+        const float xh = float(xn[0]), yh = float(xn[1]), zh = float(xn[2]);
+        A[i] = make_float4(xh, yh, zh, float(m));
+        B[i] = make_float4(float(xn[0] - double(xh)), float(xn[1] - double(yh)), float(xn[2] - double(zh)),
+                           float(0.5 * s2));
+    }
+}
+
 __global__ void k_reduce_slices(int64_t n, int comps, int slices, const unsigned char* __restrict__ ns,
                                 const double* __restrict__ part, double* __restrict__ dst) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -575,6 +621,13 @@ template <class real> void predict(hipStream_t s, const PState<real>& p, real dt
 void correct_reduce(hipStream_t s, const PState<double>& p, double dt, int slices, const unsigned char* ns,
                     const double* part) {
     k_correct<double><<<grid1d(p.n, 256), 256, 0, s>>>(p, dt, slices, ns, part);
+    RRG_LAUNCH_CHECK();
+}
+
+void correct_predict(hipStream_t s, const PState<double>& p, double dt, int slices, const unsigned char* ns,
+                     const double* part, float4* A, float4* B) {
+    if (A && B) k_correct_predict<true><<<grid1d(p.n, 256), 256, 0, s>>>(p, dt, slices, ns, part, A, B);
+    else k_correct_predict<false><<<grid1d(p.n, 256), 256, 0, s>>>(p, dt, slices, ns, part, nullptr, nullptr);
     RRG_LAUNCH_CHECK();
 }
 

@@ -460,7 +460,14 @@ public:
             const double dt = prm_.sim.dt_dynamics;
             for (; nsteps >= kStepsPerGraph; nsteps -= kStepsPerGraph) {
                 graphs_.run(s_, 4, [&] {
-                    for (int q = 0; q < kStepsPerGraph; ++q) enqueue_kernels(dt);
+                    // each step's corrector and the next step's predictor
+                    // run as one launch inside the graph
+                    predict_kernel(dt);
+                    for (int q = 0; q < kStepsPerGraph; ++q) {
+                        accel(false);
+                        if (q + 1 < kStepsPerGraph) correct_predict_kernel(dt);
+                        else correct_kernel(dt);
+                    }
                 });
                 steps_ += kStepsPerGraph;
                 for (int q = 0; q < kStepsPerGraph; ++q) time_ += dt;
@@ -890,6 +897,12 @@ private:
     void correct_kernel(double dt) {
         if (ensemble_) nbody::correct_reduce(s_, P_.view(), dt, 0, ns_.p, part_.p);
         else nbody::correct_reduce(s_, P_.view(), dt, plan_.slices, nullptr, plan_.slices > 1 ? part_.p : nullptr);
+    }
+    void correct_predict_kernel(double dt) {
+        if (ensemble_) nbody::correct_predict(s_, P_.view(), dt, 0, ns_.p, part_.p, A_.p, B_.p);
+        else
+            nbody::correct_predict(s_, P_.view(), dt, plan_.slices, nullptr, plan_.slices > 1 ? part_.p : nullptr,
+                                   A_.p, B_.p);
     }
     // The kernels of one step: predictor, force, corrector.
     void enqueue_kernels(double dt) {

@@ -186,6 +186,10 @@ void predict(hipStream_t s, const PState<real>& p, real dt, float4* A, float4* B
 // from `part` into a[] when part != nullptr).
 void correct_reduce(hipStream_t s, const PState<double>& p, double dt, int slices, const unsigned char* ns,
                     const double* part);
+// correct_reduce followed by predict (the next step's) in one launch, the
+// same values as the two.
+void correct_predict(hipStream_t s, const PState<double>& p, double dt, int slices, const unsigned char* ns,
+                     const double* part, float4* A, float4* B);
 template <class real> void correct(hipStream_t s, const PState<real>& p, real dt);
 
 // Conservation sums (nbody.cu:246-291) as FP64 per-block partials:
