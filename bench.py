@@ -145,11 +145,12 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
     # source and writes its last two levels (20 B per cell), a single launch
     # 16 B per cell; predict + deposit x, v, a, m in and x, v* out (64 B per
     # particle) and the int64 grid read and written (16 B per cell); fixed
-    # point to source 8 B in, the source out, the int64 grid cleared (20 B
-    # per cell); gather + kick (with -grad phi computed from phi on the way)
+    # point to source 8 B in and the source out (12 B per cell; it also
+    # clears the cells that hold mass, a few percent, not counted); gather +
+    # kick (with -grad phi computed from phi on the way)
     # x, v, v* in, a, v out (60 B per particle) and phi read (4 B per cell).
     # The re-sort (every 48 steps) is not counted.
-    own_bytes = (fused * 20 * ncell + sched[1] * 16 * ncell + (16 + 20 + 4) * ncell + (64 + 60) * n)
+    own_bytes = (fused * 20 * ncell + sched[1] * 16 * ncell + (16 + 12 + 4) * ncell + (64 + 60) * n)
     secs = t1 - t0
     out = {}
     if force_err:
@@ -182,8 +183,9 @@ def pm_bench(rrg, ics, steps, warmup, real="fp32", force_err=True):
                 "frac": b / (eager_tm[k][0] * 1e-3 / 3) / 1e9 / HBM_PEAK_GBS, "per_unit": u}
             for k, b, u in (("deposit", 16 * rb * n + 16 * ncell,
                              f"{16 * rb} B per particle (x, v, a, m in; x, v* out) + 16 B per cell (int64 grid)"),
-                            ("source", (16 + rb) * ncell,
-                             f"{16 + rb} B per cell (int64 grid read and cleared, source written)"),
+                            ("source", (8 + rb) * ncell,
+                             f"{8 + rb} B per cell (int64 grid read, source written; the cells holding mass, a "
+                             "few percent, are also cleared for the next deposit, not counted)"),
                             ("gather", 15 * rb * n + rb * ncell,
                              f"{15 * rb} B per particle (x, v, v* in; a, v out) + phi read once"))
             if eager_tm.get(k, (0, 0))[1]},

@@ -432,7 +432,7 @@ __global__ __launch_bounds__(256) void k_fixed_to_source(int64_t ncell, unsigned
     const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const long long q = (long long)mfix[c];
-    mfix[c] = 0ull;
+    if (q != 0) mfix[c] = 0ull;  // cleared for the next deposit (most cells hold no mass: no store)
     if (!LAYER && occ != nullptr) {
         // one wave = one 64-cell segment (c of lane 0 is a multiple of 64)
         const unsigned long long any = __ballot(q != 0);
