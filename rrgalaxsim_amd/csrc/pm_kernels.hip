@@ -449,6 +449,39 @@ __global__ __launch_bounds__(256) void k_fixed_to_source(int64_t ncell, unsigned
     src[c] = source_term<real>(m, dx3, rho_mean, periodic, kf);
 }
 
+// k_fixed_to_source with a uniform K_F and four consecutive cells per thread:
+// the int64 masses in two 16-byte loads and the four source values in one
+// store (ncell a multiple of 4); a 64-cell segment's flag comes from 16 lanes.
+template <class real>
+__global__ __launch_bounds__(256) void k_fixed_to_source4(int64_t ncell, unsigned long long* __restrict__ mfix,
+                                                          double inv_scale, real dx3, real rho_mean, int periodic,
+                                                          real k_f, real* __restrict__ mass, real* __restrict__ src,
+                                                          int* __restrict__ occ, real* __restrict__ src_empty) {
+    const int64_t c = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+    if (c >= ncell) return;  // whole 16-lane segments (ncell is a multiple of 64 where flags are written)
+    const ulonglong2 qa = reinterpret_cast<const ulonglong2*>(mfix + c)[0];
+    const ulonglong2 qb = reinterpret_cast<const ulonglong2*>(mfix + c)[1];
+    const long long q[4] = {(long long)qa.x, (long long)qa.y, (long long)qb.x, (long long)qb.y};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (q[k] != 0) mfix[c + k] = 0ull;  // cleared for the next deposit (the others are already zero)
+    if (occ != nullptr) {
+        const unsigned long long any = __ballot((q[0] | q[1] | q[2] | q[3]) != 0);
+        const int lane = threadIdx.x & 63;
+        if ((lane & 15) == 0) occ[c >> 6] = ((any >> lane) & 0xffffull) != 0ull ? 1 : 0;
+        if (c == 0) *src_empty = source_term<real>((real)((double)0ll * inv_scale), dx3, rho_mean, periodic, k_f);
+    }
+    typename Vec4<real>::T out;
+    real* o = reinterpret_cast<real*>(&out);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const real m = (real)((double)q[k] * inv_scale);
+        if (mass != nullptr) mass[c + k] = m;
+        o[k] = source_term<real>(m, dx3, rho_mean, periodic, k_f);
+    }
+    *reinterpret_cast<typename Vec4<real>::T*>(src + c) = out;
+}
+
 template <class real, bool LAYER>
 __global__ __launch_bounds__(256) void k_mass_to_source(int64_t ncell, const real* __restrict__ mass, real dx3,
                                                         real rho_mean, int periodic, real k_f, real* __restrict__ src,
@@ -1171,6 +1204,10 @@ void fixed_to_source(hipStream_t s, int nx, unsigned long long* mfix, double inv
     if (model) k_fixed_to_source<real, true><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mfix, inv_scale, dx3, rho_mean,
                                                                                periodic ? 1 : 0, k_f, mass, src, w,
                                                                                nullptr, nullptr);
+    else if (ncell % 4 == 0)
+        k_fixed_to_source4<real><<<grid1d(ncell / 4, 256), 256, 0, s>>>(ncell, mfix, inv_scale, dx3, rho_mean,
+                                                                       periodic ? 1 : 0, k_f, mass, src, occ,
+                                                                       src_empty);
     else k_fixed_to_source<real, false><<<grid1d(ncell, 256), 256, 0, s>>>(ncell, mfix, inv_scale, dx3, rho_mean,
                                                                          periodic ? 1 : 0, k_f, mass, src, w, occ,
                                                                          src_empty);
